@@ -1,0 +1,158 @@
+"""Throughput of the parallel cascaded-biquad IIR path on one MI355X.
+
+Metric (the reference's headline, README.md:34-39): GSamples/s and % of the
+HBM roof at 2^25 float32 samples, orders 2-16.  One step = one filter call
+(one grid-wide kernel launch) over 2^25 samples already resident in HBM.
+`value` is the order-2 step of the configured algorithm; every order is
+measured and reported under "orders".  Inputs rotate over enough buffers to
+exceed twice the 256 MB Infinity Cache, so every launch streams from HBM
+(the reference rotates against its L2, test_performance.cu:140-157).
+
+    python bench.py [--steps K] [--warmup W] [--kernel PH] [--geom 64x64]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+BYTES_PER_SAMPLE = 8           # fp32 in + fp32 out
+N_SAMPLES = 1 << 25
+MALL_BYTES = 256 << 20
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--kernel", default="PH")
+    ap.add_argument("--geom", default="64x64")
+    ap.add_argument("--orders", default="2,4,8,16")
+    ap.add_argument("--wn", type=float, default=0.1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--check", action="store_true", help="gate the last output against the FP64 oracle")
+    return ap.parse_args()
+
+
+def time_order(iirx, torch, order, args, bufs, out, stream):
+    B, N = (int(v) for v in args.geom.lower().split("x"))
+    sos, g = iirx.butterworth_sos(order, args.wn)
+    plan = iirx.Plan(args.kernel, sos, N_SAMPLES, block_size=B, n_blocks=N)
+    for i in range(args.warmup):
+        plan.execute(bufs[i % len(bufs)], out, stream)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for i in range(args.steps):
+        plan.execute(bufs[i % len(bufs)], out, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    info = plan.info()
+    res = {"sos": sos, "gain": g, "ms": ms, "wall_ms": 1e3 * wall / args.steps, "info": info,
+           "last_input": (args.warmup + args.steps - 1) % len(bufs), "plan": plan}
+    return res
+
+
+def cpu_baseline(iirx, args):
+    """The oracle's float32 PH restatement (a port of the reference algorithm at
+    its published 32x64 geometry), single thread, over the 2^25-sample
+    order-2 workload, repeated for about --cpu-seconds."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import numpy as np
+    import oracle_lib as O
+    sos, g = iirx.butterworth_sos(2, args.wn)
+    x = (g * iirx.fill_normal(1, N_SAMPLES).astype(np.float64)).astype(np.float32)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        O.ph(sos, x, 32, 64)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= args.cpu_seconds:
+            break
+    return {"value": round(reps * N_SAMPLES / el / 1e9, 4), "unit": "GSamples/s", "cores": 1, "kind": "port",
+            "sample": f"oracle PH float32 restatement (32x64, order 2, Wn={args.wn}) over 2^25 N(0,1) samples, "
+                      f"{reps} passes in {el:.1f} s, single thread"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import cuda_recursive_filtering_amd as iirx
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    nbuf = max(4, math.ceil(2 * MALL_BYTES / (4 * N_SAMPLES)) + 1)
+    base = iirx.fill_normal(1, N_SAMPLES)
+    orders = [int(o) for o in args.orders.split(",")]
+    per = {}
+    for order in orders:
+        sos, g = iirx.butterworth_sos(order, args.wn)
+        host = (g * base.astype(np.float64)).astype(np.float32)
+        src = torch.from_numpy(host).to(dev)
+        bufs = [src] + [src.clone() for _ in range(nbuf - 1)]
+        out = torch.empty_like(src)
+        r = time_order(iirx, torch, order, args, bufs, out, stream)
+        gs = N_SAMPLES / (r["ms"] * 1e-3) / 1e9
+        gbs = BYTES_PER_SAMPLE * N_SAMPLES / (r["ms"] * 1e-3) / 1e9
+        entry = {"gsamples_per_s": round(gs, 3), "ms": round(r["ms"], 5), "achieved_gbs": round(gbs, 1),
+                 "roof_frac": round(gbs / HBM_PEAK_GBS, 4), "vgprs": r["info"]["vgprs"],
+                 "scratch": r["info"]["scratch_bytes"], "chunks": r["info"]["num_chunks"]}
+        if args.check:
+            sys.path.insert(0, os.path.join(HERE, "tests"))
+            import oracle_lib as O
+            y = out.cpu().numpy()
+            ok, err, mref, rel = O.gate(y, O.golden(sos, host))
+            entry["gate_rel"] = float(rel)
+            entry["gate_pass"] = bool(ok)
+        per[str(order)] = entry
+        r["plan"].close()
+        del bufs, src, out
+        torch.cuda.empty_cache()
+        print(f"# order {order}: {entry}", file=sys.stderr, flush=True)
+
+    o2 = per[str(orders[0])]
+    line = {
+        "metric": "GSamples/s & % HBM roof at 2^25, order 2-16",
+        "value": o2["gsamples_per_s"],
+        "unit": "GSamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": o2["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"order-{orders[0]} Butterworth lowpass Wn={args.wn}, 2^25 float32 N(0,1) "
+                               f"samples (seed 1, gain pre-applied), one channel, zero initial state",
+                   "kernel": args.kernel, "geometry": args.geom, "n_samples": N_SAMPLES,
+                   "input_buffers": nbuf},
+        "orders": per,
+        "roofline": {"bound": "hbm", "achieved": o2["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": o2["roof_frac"], "traffic": None},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(iirx, args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

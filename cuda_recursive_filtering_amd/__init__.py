@@ -1,0 +1,196 @@
+"""MI355X-native parallel cascaded-biquad IIR filtering (host-side mirror).
+
+Python view of the C ABI in include/iirx.h.  It mirrors the reference's
+operator interface (Haotian-RA/cuda_recursive_filtering):
+
+* the reference picks KERNEL (PH / STCR / DTCR / PLR), BLOCK_SIZE, N_BLOCKS,
+  N_SECTIONS and HANDUNROLLED at compile time (run_accuracy.sh:22-60) and
+  uploads the sections with setup_kernel_coefficients(sos)
+  (ph_kernels.cuh:151); here `Plan(kernel, sos, n_samples, block_size,
+  n_blocks, unrolled)` does both at run time;
+* one `Plan.execute(d_in, d_out)` is one grid-wide launch
+  (test_accuracy.cu:146);
+* sections follow the reference's contract: b0 == 1, the cascade gain is
+  applied to the input by the caller (ref_generate.py:61-67).
+
+Device memory and streams come from PyTorch (tensors' data pointers and
+torch.cuda streams); the computation is libiirx.so's HIP kernels only.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import IirxError, KERNELS, check, lib
+
+__all__ = [
+    "Plan", "IirxError", "KERNELS", "butterworth_sos", "direct_form_taps", "fill_normal",
+    "plan_tables", "config_check", "version",
+]
+
+
+def version():
+    return lib().iirx_version().decode()
+
+
+def _config(kernel, n_sections, block_size=64, n_blocks=64, unrolled=False, plr_x=0, plr_x_max=0):
+    if isinstance(kernel, str):
+        if kernel.upper() not in KERNELS:
+            raise ValueError(f"unknown kernel {kernel!r}; expected one of {sorted(KERNELS)}")
+        kernel = KERNELS[kernel.upper()]
+    c = _lib.Config()
+    c.kernel = int(kernel)
+    c.block_size = int(block_size)
+    c.n_blocks = int(n_blocks)
+    c.n_sections = int(n_sections)
+    c.unrolled = int(bool(unrolled))
+    c.plr_x = int(plr_x)
+    c.plr_x_max = int(plr_x_max)
+    return c
+
+
+def _sos_array(sos):
+    s = np.ascontiguousarray(np.asarray(sos, dtype=np.float32).reshape(-1, 6))
+    return s
+
+
+def config_check(kernel, n_sections, n_samples, **kw):
+    """Validity of a configuration without a device (gpu_specs::config_is_valid)."""
+    c = _config(kernel, n_sections, **kw)
+    return lib().iirx_config_check(ctypes.byref(c), int(n_samples))
+
+
+def butterworth_sos(order, wn):
+    """Butterworth lowpass as b0-normalised float32 sections and the float64 gain
+    (scipy.signal.butter(order, wn, output='sos') + ref_generate.py:61-67)."""
+    ns = order // 2
+    sos = np.zeros((ns, 6), np.float32)
+    g = ctypes.c_double()
+    check(lib().iirx_butterworth_sos(int(order), float(wn), sos.ctypes.data, ctypes.byref(g)), "butterworth")
+    return sos, g.value
+
+
+def direct_form_taps(sos):
+    """(b, a): FIR taps past b0 == 1 and feedback taps (recurrence sign), FP64."""
+    s = _sos_array(sos)
+    ns = s.shape[0]
+    b = np.zeros(2 * ns, np.float64)
+    a = np.zeros(2 * ns, np.float64)
+    check(lib().iirx_direct_form_taps(s.ctypes.data, ns, b.ctypes.data, a.ctypes.data), "direct_form_taps")
+    return b, a
+
+
+def fill_normal(seed, n):
+    """N(0,1) float32 samples from the library's documented generator."""
+    out = np.empty(int(n), np.float32)
+    check(lib().iirx_fill_normal(int(seed) & (2**64 - 1), int(n), out.ctypes.data), "fill_normal")
+    return out
+
+
+def plan_tables(kernel, sos, n_samples, **kw):
+    """Host copy of the FP64-precomputed tables a plan would upload: (tables, stride)."""
+    s = _sos_array(sos)
+    c = _config(kernel, s.shape[0], **kw)
+    size = ctypes.c_int64()
+    stride = ctypes.c_int32()
+    check(lib().iirx_plan_tables(ctypes.byref(c), s.ctypes.data, int(n_samples), None, 0,
+                                 ctypes.byref(size), ctypes.byref(stride)), "plan_tables")
+    out = np.zeros(size.value, np.float32)
+    check(lib().iirx_plan_tables(ctypes.byref(c), s.ctypes.data, int(n_samples), out.ctypes.data, size.value,
+                                 ctypes.byref(size), ctypes.byref(stride)), "plan_tables")
+    return out, stride.value
+
+
+def _ptr(t):
+    if isinstance(t, int):
+        return t
+    if hasattr(t, "data_ptr"):
+        if not t.is_cuda:
+            raise ValueError("device tensor required")
+        if str(t.dtype) != "torch.float32":
+            raise ValueError("float32 tensor required")
+        if not t.is_contiguous():
+            raise ValueError("contiguous tensor required")
+        return t.data_ptr()
+    raise TypeError(f"expected a device pointer or a torch tensor, got {type(t)}")
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        try:
+            import torch
+            return torch.cuda.current_stream().cuda_stream
+        except Exception:
+            return 0
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class Plan:
+    """A runtime plan: one filter, one geometry, one length, one device.
+
+    kernel       'PH' | 'STCR' | 'DTCR' | 'PLR'             (KERNEL)
+    sos          (n_sections, 6) b0-normalised sections     (FILTER_SOS)
+    n_samples    signal length                              (2^N_SAMPLES_LOG2)
+    block_size   lanes per chunk, 32 or 64                  (BLOCK_SIZE)
+    n_blocks     samples per lane                           (N_BLOCKS)
+    unrolled     literal-index back substitution            (HANDUNROLLED)
+    plr_x, plr_x_max   PLR values per thread                (PLR_X / PLR_X_MAX)
+    """
+
+    def __init__(self, kernel, sos, n_samples, block_size=64, n_blocks=64, unrolled=False, plr_x=0, plr_x_max=0,
+                 plr_taps=None):
+        s = _sos_array(sos)
+        self.kernel = kernel.upper() if isinstance(kernel, str) else kernel
+        self.n_sections = s.shape[0]
+        self.n_samples = int(n_samples)
+        self._cfg = _config(kernel, self.n_sections, block_size, n_blocks, unrolled, plr_x, plr_x_max)
+        h = ctypes.c_void_p()
+        check(lib().iirx_plan_create(ctypes.byref(self._cfg), s.ctypes.data, self.n_samples, ctypes.byref(h)),
+              "plan_create")
+        self._h = h
+        if plr_taps is not None:
+            b, a = (np.ascontiguousarray(np.asarray(v, np.float64)) for v in plr_taps)
+            check(lib().iirx_plan_set_plr_taps(self._h, b.ctypes.data, a.ctypes.data, b.size), "set_plr_taps")
+
+    def execute(self, d_in, d_out, stream=None):
+        """One grid-wide launch: d_out = filter(d_in).  Tensors or raw device pointers."""
+        if self._h is None:
+            raise RuntimeError("plan is closed")
+        for t in (d_in, d_out):
+            if hasattr(t, "numel") and t.numel() != self.n_samples:
+                raise ValueError(f"tensor has {t.numel()} samples, plan expects {self.n_samples}")
+        check(lib().iirx_execute(self._h, _ptr(d_in), _ptr(d_out), _stream_ptr(stream)), "execute")
+
+    __call__ = execute
+
+    def info(self):
+        i = _lib.PlanInfo()
+        check(lib().iirx_plan_info_get(self._h, ctypes.byref(i)), "plan_info")
+        return {f: getattr(i, f) for f, _ in _lib.PlanInfo._fields_}
+
+    def measure(self, inputs, d_out, warmup=1000, batches=20, batch=200, stream=None):
+        """measure_kernel_time: sorted per-call batch means in ms (P20 = [batches // 5])."""
+        ptrs = (ctypes.c_void_p * len(inputs))(*[_ptr(t) for t in inputs])
+        out = np.zeros(batches, np.float32)
+        check(lib().iirx_measure(self._h, ptrs, len(inputs), _ptr(d_out), int(warmup), int(batches), int(batch),
+                                 _stream_ptr(stream), out.ctypes.data), "measure")
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().iirx_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

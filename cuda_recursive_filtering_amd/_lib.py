@@ -1,0 +1,98 @@
+"""ctypes binding of libiirx.so (the C ABI declared in include/iirx.h).
+
+The library is built in-tree by cuda_recursive_filtering_amd.build; there is
+no fallback: if it is missing, loading raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libiirx.so")
+
+IIRX_OK = 0
+IIRX_ERR_INVALID = 1
+IIRX_ERR_GEOMETRY = 2
+IIRX_ERR_CONDITIONING = 3
+IIRX_ERR_HIP = 4
+IIRX_ERR_NO_DEVICE = 5
+IIRX_ERR_OOM = 6
+
+KERNELS = {"PH": 0, "STCR": 1, "DTCR": 2, "PLR": 3}
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("kernel", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("n_blocks", ctypes.c_int32),
+        ("n_sections", ctypes.c_int32),
+        ("unrolled", ctypes.c_int32),
+        ("plr_x", ctypes.c_int32),
+        ("plr_x_max", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_samples", ctypes.c_int64),
+        ("chunk_samples", ctypes.c_int64),
+        ("num_chunks", ctypes.c_int64),
+        ("threads_per_block", ctypes.c_int32),
+        ("lds_bytes", ctypes.c_int32),
+        ("vgprs", ctypes.c_int32),
+        ("sgprs", ctypes.c_int32),
+        ("scratch_bytes", ctypes.c_int32),
+        ("waves_per_simd", ctypes.c_int32),
+        ("launches", ctypes.c_int64),
+    ]
+
+
+# every symbol include/iirx.h declares, with its ctypes signature
+_P = ctypes.c_void_p
+_SIGS = {
+    "iirx_plan_create": (ctypes.c_int, [ctypes.POINTER(Config), _P, ctypes.c_int64, ctypes.POINTER(_P)]),
+    "iirx_plan_set_plr_taps": (ctypes.c_int, [_P, _P, _P, ctypes.c_int]),
+    "iirx_execute": (ctypes.c_int, [_P, _P, _P, _P]),
+    "iirx_plan_info_get": (ctypes.c_int, [_P, ctypes.POINTER(PlanInfo)]),
+    "iirx_plan_destroy": (ctypes.c_int, [_P]),
+    "iirx_config_check": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int64]),
+    "iirx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "iirx_measure": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]),
+    "iirx_version": (ctypes.c_char_p, []),
+    "iirx_butterworth_sos": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, _P, _P]),
+    "iirx_direct_form_taps": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
+    "iirx_fill_normal": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, _P]),
+    "iirx_plan_tables": (ctypes.c_int, [ctypes.POINTER(Config), _P, ctypes.c_int64, _P, ctypes.c_int64,
+                                        ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
+}
+
+_lib = None
+
+
+class IirxError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = lib().iirx_status_string(code).decode() if _lib is not None else str(code)
+        super().__init__(f"{what}: {msg} (status {code})" if what else f"{msg} (status {code})")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run `python -m cuda_recursive_filtering_amd.build` "
+                "(the HIP library is required, there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code, what=""):
+    if code != IIRX_OK:
+        raise IirxError(code, what)

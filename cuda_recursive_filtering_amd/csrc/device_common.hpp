@@ -1,0 +1,223 @@
+// device_common.hpp -- wave64 building blocks shared by the CDNA4 kernels:
+// the monotone workgroup ticket, the self-validating lookback records, the
+// 16-byte coalesced global<->LDS staging and lane primitives.
+//
+// Lookback protocol (replaces the launch-versioned status/partcarry/fullcarry
+// triple of ph_kernels.cuh:16-29, 384-469).
+//   * Ticket: one 64-bit counter per plan, incremented once per workgroup.
+//     ticket = epoch * num_chunks + chunk.  Workgroups of one launch draw
+//     tickets in dispatch order, so a chunk only ever waits on chunks that
+//     already hold a ticket, i.e. are resident or finished (AMD hardware
+//     gives no forward-progress guarantee to workgroups not yet dispatched;
+//     the reference relies on the same ordering, ph_kernels.cuh:227-229).
+//     The epoch replaces the caller-supplied `launch` index, so the kernel
+//     arguments do not change between launches and a captured graph replays
+//     correctly.
+//   * Record: every carry float travels in its own 64-bit word
+//     (flag << 32 | float bits).  A reader accepts a value only if the flag
+//     of that very word is current, so no fence is needed between payload
+//     and flag and a torn view of a multi-word carry is detected per word.
+//     Flags are 2*epoch+1 (partial: chunk result with zero incoming state)
+//     and 2*epoch+2 (full: true end state), in separate words, so anything
+//     left by an earlier launch reads as "not ready"; the buffer is zeroed
+//     once at plan creation.
+//   * Stores and loads are agent-scope atomics: the eight XCDs of MI355X do
+//     not share an L2, and agent-scope atomics are kept coherent across them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace iirx {
+
+constexpr int kWave = 64;
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float as_float(uint32_t u) { return __uint_as_float(u); }
+
+__device__ __forceinline__ uint32_t readfirstlane_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+__device__ __forceinline__ float readlane_f(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ int readlane_i(int v, int lane)
+{
+    return __builtin_amdgcn_readlane(v, lane);
+}
+
+// value of lane (l - d) for l >= d, own value otherwise (CUDA shfl_up rule)
+__device__ __forceinline__ float lane_up(float v, int d) { return __shfl_up(v, (unsigned)d, kWave); }
+
+// ------------------------------------------------------------------------
+// ticket
+// ------------------------------------------------------------------------
+struct Ticket {
+    uint32_t chunk;
+    uint32_t part_flag;
+    uint32_t full_flag;
+};
+
+__device__ __forceinline__ Ticket take_ticket(unsigned long long *ctr, uint32_t num_chunks)
+{
+    __shared__ unsigned long long s_tk;
+    if (threadIdx.x == 0) s_tk = atomicAdd(ctr, 1ull);
+    __syncthreads();
+    const unsigned long long tk = s_tk;
+    const unsigned long long epoch = tk / num_chunks;
+    Ticket t;
+    t.chunk = (uint32_t)(tk - epoch * num_chunks);
+    t.part_flag = (uint32_t)(2ull * epoch + 1ull);
+    t.full_flag = t.part_flag + 1u;
+    return t;
+}
+
+// ------------------------------------------------------------------------
+// lookback records
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void rec_store(uint64_t *p, uint32_t flag, float v)
+{
+    const uint64_t w = ((uint64_t)flag << 32) | (uint64_t)__float_as_uint(v);
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t rec_load(uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t rec_flag(uint64_t w) { return (uint32_t)(w >> 32); }
+__device__ __forceinline__ float rec_val(uint64_t w) { return __uint_as_float((uint32_t)w); }
+
+// Two-float carry (state (y[-2], y[-1]) of a biquad), records of 4 words:
+// [partial y2, partial y1, full y2, full y1].
+struct Carry2 {
+    float y2, y1;
+};
+
+__device__ __forceinline__ uint64_t *rec2(uint64_t *lb, uint32_t chunk, int sec, int ns)
+{
+    return lb + ((size_t)chunk * (size_t)ns + (size_t)sec) * 4u;
+}
+
+// Publish from one lane.
+__device__ __forceinline__ void publish2(uint64_t *r, bool full, uint32_t flag, float y2, float y1)
+{
+    const int o = full ? 2 : 0;
+    rec_store(r + o, flag, y2);
+    rec_store(r + o + 1, flag, y1);
+}
+
+// Decoupled lookback over a 64-chunk window, one predecessor per lane.
+// Returns the full end state of chunk `chunk - 1` (zero state for chunk 0).
+// M is the chunk transition: X_c = M X_{c-1} + partial_c (row-major 2x2).
+// The wave spins until every predecessor from the nearest one up to the
+// nearest full one has published at least its partial, as the reference's
+// spin (ph_kernels.cuh:397-404); the partials in between are folded by
+// Horner's rule, nearest last (ph_kernels.cuh:436-450).
+__device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int sec, int ns,
+                                            uint32_t part_flag, uint32_t full_flag,
+                                            float m0, float m1, float m2, float m3)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const long pred = (long)chunk - 1 - lane;
+    float v2 = 0.0f, v1 = 0.0f;
+    int st = 0;
+    int D;
+    for (;;) {
+        if (pred < 0) {
+            st = 3;
+            v2 = 0.0f;
+            v1 = 0.0f;
+        } else if (st < 2) {
+            uint64_t *r = rec2(lb, (uint32_t)pred, sec, ns);
+            const uint64_t f0 = rec_load(r + 2), f1 = rec_load(r + 3);
+            if (rec_flag(f0) == full_flag && rec_flag(f1) == full_flag) {
+                st = 2;
+                v2 = rec_val(f0);
+                v1 = rec_val(f1);
+            } else if (st == 0) {
+                const uint64_t q0 = rec_load(r), q1 = rec_load(r + 1);
+                if (rec_flag(q0) == part_flag && rec_flag(q1) == part_flag) {
+                    st = 1;
+                    v2 = rec_val(q0);
+                    v1 = rec_val(q1);
+                }
+            }
+        }
+        const uint64_t zmask = __ballot(st == 0);
+        const uint64_t fmask = __ballot(st >= 2);
+        if (fmask) {
+            D = __builtin_ctzll(fmask);
+            const uint64_t need = (D == 63) ? ~0ull : ((1ull << (D + 1)) - 1ull);
+            if ((zmask & need) == 0) break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    Carry2 X;
+    X.y2 = readlane_f(v2, D);
+    X.y1 = readlane_f(v1, D);
+    for (int d = D - 1; d >= 0; --d) {
+        const float p2 = readlane_f(v2, d), p1 = readlane_f(v1, d);
+        const float n2 = fmaf(m1, X.y1, fmaf(m0, X.y2, p2));
+        const float n1 = fmaf(m3, X.y1, fmaf(m2, X.y2, p1));
+        X.y2 = n2;
+        X.y1 = n1;
+    }
+    return X;
+}
+
+// ------------------------------------------------------------------------
+// global <-> LDS staging, 16-byte coalesced accesses
+// Chunk of L lanes x N samples, lane t owns samples [t*N, t*N + N); LDS rows
+// are padded to N+1 floats so that the per-lane row reads hit distinct banks.
+// ------------------------------------------------------------------------
+template <int L, int N>
+__device__ __forceinline__ void stage_in(const float *__restrict__ src, float *lds)
+{
+    constexpr int V = L * N / 4;           // float4s in the chunk
+    constexpr int PER = V / L;             // float4s per thread
+    const int tid = threadIdx.x;
+    const f4 *s4 = reinterpret_cast<const f4 *>(src);
+    f4 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) v[i] = __builtin_nontemporal_load(s4 + i * L + tid);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int s = 4 * (i * L + tid);
+        const int row = s / N, col = s % N;
+        float *d = lds + row * (N + 1) + col;
+        d[0] = v[i].x;
+        d[1] = v[i].y;
+        d[2] = v[i].z;
+        d[3] = v[i].w;
+    }
+}
+
+template <int L, int N>
+__device__ __forceinline__ void stage_out(const float *lds, float *__restrict__ dst)
+{
+    constexpr int V = L * N / 4;
+    constexpr int PER = V / L;
+    const int tid = threadIdx.x;
+    f4 *d4 = reinterpret_cast<f4 *>(dst);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int s = 4 * (i * L + tid);
+        const int row = s / N, col = s % N;
+        const float *p = lds + row * (N + 1) + col;
+        f4 v;
+        v.x = p[0];
+        v.y = p[1];
+        v.z = p[2];
+        v.w = p[3];
+        __builtin_nontemporal_store(v, d4 + i * L + tid);
+    }
+}
+
+}  // namespace iirx

@@ -1,0 +1,278 @@
+// iirx_plan.cpp -- the C ABI of include/iirx.h: runtime plans, FP64 host
+// precompute of the coefficient tables, the once-zeroed lookback state, the
+// launch, and the hipEvent timing harness.
+//
+// Reference counterparts: setup_kernel_coefficients (ph_kernels.cuh:151-200,
+// stcr_kernels.cuh:191-272, dtcr_kernels.cuh:193-289, plr_kernels.cuh:
+// 416-476), the driver's buffer set-up and launch (test_accuracy.cu:128-160,
+// test_performance.cu:140-182) and measure_kernel_time (measure.cuh:38-110).
+#include "../../include/iirx.h"
+#include "kernels.hpp"
+#include "precompute.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+using namespace iirx;
+
+struct iirx_plan {
+    iirx_config cfg;
+    int64_t n_samples;
+    int lanes;            // lanes per workgroup
+    int n_blocks;         // samples per lane
+    int threads;          // threads per workgroup
+    int64_t chunk;        // samples per workgroup
+    uint32_t num_chunks;
+    int device;
+    std::vector<float> sos;
+    std::vector<float> host_coef;
+    int coef_stride;
+    float *d_coef;
+    uint64_t *d_lb;
+    size_t lb_words;
+    unsigned long long *d_ticket;
+    int64_t launches;
+};
+
+namespace {
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int validate(const iirx_config *cfg, int64_t n)
+{
+    if (!cfg) return IIRX_ERR_INVALID;
+    if (cfg->n_sections < 1 || cfg->n_sections > 8) return IIRX_ERR_INVALID;
+    if (n <= 0 || n > (int64_t)1 << 34) return IIRX_ERR_INVALID;
+    switch (cfg->kernel) {
+    case IIRX_KERNEL_PH: {
+        const int B = cfg->block_size, N = cfg->n_blocks;
+        if (!(B == 32 || B == 64)) return IIRX_ERR_GEOMETRY;
+        if (!is_pow2(N) || N < 8 || N > 128) return IIRX_ERR_GEOMETRY;
+        const int64_t chunk = 64LL * N;
+        if (n % chunk != 0) return IIRX_ERR_GEOMETRY;
+        if (n / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
+        return IIRX_OK;
+    }
+    default:
+        return IIRX_ERR_INVALID;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *iirx_version(void) { return "iirx 0.1 (gfx950)"; }
+
+const char *iirx_status_string(int s)
+{
+    switch (s) {
+    case IIRX_OK: return "ok";
+    case IIRX_ERR_INVALID: return "invalid argument or unsupported configuration";
+    case IIRX_ERR_GEOMETRY: return "geometry: length not a multiple of the chunk, or no kernel instance";
+    case IIRX_ERR_CONDITIONING: return "CR factors ill-conditioned for this filter and N_BLOCKS";
+    case IIRX_ERR_HIP: return "HIP runtime error";
+    case IIRX_ERR_NO_DEVICE: return "no GPU device";
+    case IIRX_ERR_OOM: return "out of memory";
+    default: return "unknown status";
+    }
+}
+
+int iirx_config_check(const iirx_config *cfg, int64_t n_samples) { return validate(cfg, n_samples); }
+
+int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples, iirx_plan **out)
+{
+    if (!out || !sos) return IIRX_ERR_INVALID;
+    *out = nullptr;
+    int st = validate(cfg, n_samples);
+    if (st != IIRX_OK) return st;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IIRX_ERR_NO_DEVICE;
+
+    iirx_plan *p = new (std::nothrow) iirx_plan();
+    if (!p) return IIRX_ERR_OOM;
+    p->cfg = *cfg;
+    p->n_samples = n_samples;
+    p->sos.assign(sos, sos + 6 * cfg->n_sections);
+    p->d_coef = nullptr;
+    p->d_lb = nullptr;
+    p->d_ticket = nullptr;
+    p->launches = 0;
+    hipGetDevice(&p->device);
+
+    const int ns = cfg->n_sections;
+    switch (cfg->kernel) {
+    case IIRX_KERNEL_PH:
+        p->lanes = 64;
+        p->n_blocks = cfg->n_blocks;
+        p->threads = 64;
+        p->chunk = (int64_t)p->lanes * p->n_blocks;
+        p->coef_stride = ph_tables(p->sos.data(), ns, p->lanes, p->n_blocks, p->host_coef);
+        p->lb_words = 4;  // 2 partial + 2 full words per (chunk, section)
+        break;
+    default:
+        delete p;
+        return IIRX_ERR_INVALID;
+    }
+    p->num_chunks = (uint32_t)(n_samples / p->chunk);
+    const size_t lb_count = (size_t)p->num_chunks * (size_t)ns * p->lb_words;
+
+    auto fail = [&](int code) {
+        if (p->d_coef) hipFree(p->d_coef);
+        if (p->d_lb) hipFree(p->d_lb);
+        if (p->d_ticket) hipFree(p->d_ticket);
+        delete p;
+        return code;
+    };
+    if (hipMalloc((void **)&p->d_coef, p->host_coef.size() * sizeof(float)) != hipSuccess) return fail(IIRX_ERR_OOM);
+    if (hipMalloc((void **)&p->d_lb, lb_count * sizeof(uint64_t)) != hipSuccess) return fail(IIRX_ERR_OOM);
+    if (hipMalloc((void **)&p->d_ticket, sizeof(unsigned long long)) != hipSuccess) return fail(IIRX_ERR_OOM);
+    if (hipMemcpy(p->d_coef, p->host_coef.data(), p->host_coef.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(IIRX_ERR_HIP);
+    // zeroed once; launch-versioned flags never need a reset afterwards
+    if (hipMemset(p->d_lb, 0, lb_count * sizeof(uint64_t)) != hipSuccess) return fail(IIRX_ERR_HIP);
+    if (hipMemset(p->d_ticket, 0, sizeof(unsigned long long)) != hipSuccess) return fail(IIRX_ERR_HIP);
+    if (hipDeviceSynchronize() != hipSuccess) return fail(IIRX_ERR_HIP);
+    *out = p;
+    return IIRX_OK;
+}
+
+int iirx_plan_tables(const iirx_config *cfg, const float *sos, int64_t n_samples, float *out, int64_t cap,
+                     int64_t *size, int32_t *stride)
+{
+    if (!sos) return IIRX_ERR_INVALID;
+    int st = validate(cfg, n_samples);
+    if (st != IIRX_OK) return st;
+    std::vector<float> t;
+    int str = 0;
+    switch (cfg->kernel) {
+    case IIRX_KERNEL_PH: str = ph_tables(sos, cfg->n_sections, 64, cfg->n_blocks, t); break;
+    default: return IIRX_ERR_INVALID;
+    }
+    if (size) *size = (int64_t)t.size();
+    if (stride) *stride = str;
+    if (out) std::memcpy(out, t.data(), sizeof(float) * (size_t)std::min<int64_t>(cap, (int64_t)t.size()));
+    return IIRX_OK;
+}
+
+int iirx_plan_set_plr_taps(iirx_plan *plan, const double *b, const double *a, int k)
+{
+    (void)plan; (void)b; (void)a; (void)k;
+    return IIRX_ERR_INVALID;
+}
+
+static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
+{
+    LaunchArgs a;
+    a.in = d_in;
+    a.out = d_out;
+    a.coef = p->d_coef;
+    a.lb = p->d_lb;
+    a.ticket = p->d_ticket;
+    a.n_samples = p->n_samples;
+    a.num_chunks = p->num_chunks;
+    a.ns = p->cfg.n_sections;
+    a.coef_stride = p->coef_stride;
+    a.unrolled = p->cfg.unrolled;
+    hipError_t e = hipErrorInvalidValue;
+    switch (p->cfg.kernel) {
+    case IIRX_KERNEL_PH: e = launch_ph(p->lanes, p->n_blocks, a, s); break;
+    default: break;
+    }
+    if (e != hipSuccess) return IIRX_ERR_HIP;
+    p->launches++;
+    return IIRX_OK;
+}
+
+int iirx_execute(iirx_plan *p, const float *d_in, float *d_out, void *stream)
+{
+    if (!p || !d_in || !d_out) return IIRX_ERR_INVALID;
+    if (((uintptr_t)d_in & 15) || ((uintptr_t)d_out & 15)) return IIRX_ERR_INVALID;
+    const uintptr_t ib = (uintptr_t)d_in, ie = ib + (uintptr_t)p->n_samples * 4;
+    const uintptr_t ob = (uintptr_t)d_out, oe = ob + (uintptr_t)p->n_samples * 4;
+    if (ib < oe && ob < ie) return IIRX_ERR_INVALID;  // overlap
+    return launch(p, d_in, d_out, (hipStream_t)stream);
+}
+
+int iirx_plan_info_get(const iirx_plan *p, iirx_plan_info *info)
+{
+    if (!p || !info) return IIRX_ERR_INVALID;
+    std::memset(info, 0, sizeof *info);
+    info->n_samples = p->n_samples;
+    info->chunk_samples = p->chunk;
+    info->num_chunks = p->num_chunks;
+    info->threads_per_block = p->threads;
+    info->launches = p->launches;
+    const void *fn = nullptr;
+    switch (p->cfg.kernel) {
+    case IIRX_KERNEL_PH: fn = kernel_ph(p->lanes, p->n_blocks); break;
+    default: break;
+    }
+    if (fn) {
+        hipFuncAttributes at;
+        if (hipFuncGetAttributes(&at, fn) == hipSuccess) {
+            info->lds_bytes = (int)at.sharedSizeBytes;
+            info->vgprs = at.numRegs;
+            info->scratch_bytes = (int)at.localSizeBytes;
+        }
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, p->threads, 0) == hipSuccess) {
+            const int waves_per_block = (p->threads + 63) / 64;
+            info->waves_per_simd = nb * waves_per_block / 4;
+        }
+    }
+    return IIRX_OK;
+}
+
+int iirx_plan_destroy(iirx_plan *p)
+{
+    if (!p) return IIRX_ERR_INVALID;
+    hipDeviceSynchronize();
+    if (p->d_coef) hipFree(p->d_coef);
+    if (p->d_lb) hipFree(p->d_lb);
+    if (p->d_ticket) hipFree(p->d_ticket);
+    delete p;
+    return IIRX_OK;
+}
+
+int iirx_measure(iirx_plan *p, const float *const *d_inputs, int n_inputs, float *d_out, int warmup, int batches,
+                 int batch, void *stream, float *batch_ms)
+{
+    if (!p || !d_inputs || n_inputs < 1 || !d_out || warmup < 0 || batches < 1 || batch < 1 || !batch_ms)
+        return IIRX_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    for (int i = 0; i < warmup; ++i) {
+        int st = iirx_execute(p, d_inputs[i % n_inputs], d_out, stream);
+        if (st != IIRX_OK) return st;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return IIRX_ERR_HIP;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return IIRX_ERR_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); return IIRX_ERR_HIP; }
+    int it = 0, st = IIRX_OK;
+    for (int b = 0; b < batches && st == IIRX_OK; ++b) {
+        hipEventRecord(e0, s);
+        for (int i = 0; i < batch; ++i, ++it) {
+            st = iirx_execute(p, d_inputs[it % n_inputs], d_out, stream);
+            if (st != IIRX_OK) break;
+        }
+        hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) { st = IIRX_ERR_HIP; break; }
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        batch_ms[b] = ms / (float)batch;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (st != IIRX_OK) return st;
+    std::sort(batch_ms, batch_ms + batches);
+    return IIRX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+// ph.hip -- PH (partial-LU factorisation) for gfx950.
+//
+// Algorithm of include/ph_kernels.cuh (PH_32 208-498, PH_64 501-820): every
+// lane filters its N samples with zero incoming recursive state, the lane end
+// states are combined by a scan with powers of the N-step transition C, the
+// chunk's incoming state comes from the decoupled lookback, and every sample
+// is corrected by the zero-input response to the lane's incoming state.
+//
+// CDNA4 mapping.  One workgroup is one wave64 holding a chunk of 64 lanes x N
+// samples: the reference's 32-lane chunks are packed two per wave (their
+// combine is part of the wave's scan), its 64-lane chunk is the wave.  The
+// lane scan is a Kogge-Stone scan with the uniform operators C^(2^k) held in
+// scalar registers (the reference uses a segmented scan with per-lane
+// operators loaded through 32-lane shuffles).  Samples stay in VGPRs between
+// sections; HBM is touched once on the way in and once on the way out, with
+// 16-byte accesses staged through LDS.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace iirx {
+
+template <int N>
+__global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
+{
+    constexpr int L = 64;
+    constexpr PhLayout Y = ph_layout(L, N);
+    __shared__ float lds[L * (N + 1)];
+
+    const int lane = threadIdx.x;
+    const Ticket tk = take_ticket(a.ticket, a.num_chunks);
+    const uint32_t chunk = tk.chunk;
+    const long base = (long)chunk * (L * N);
+
+    stage_in<L, N>(a.in + base, lds);
+    __syncthreads();
+
+    float r[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) r[n] = lds[lane * (N + 1) + n];
+
+    // inputs of the first section just before this lane's samples
+    float xi2, xi1;
+    if (lane > 0) {
+        xi2 = lds[(lane - 1) * (N + 1) + N - 2];
+        xi1 = lds[(lane - 1) * (N + 1) + N - 1];
+    } else if (chunk > 0) {
+        xi2 = a.in[base - 2];
+        xi1 = a.in[base - 1];
+    } else {
+        xi2 = 0.0f;
+        xi1 = 0.0f;
+    }
+
+    for (int sec = 0; sec < a.ns; ++sec) {
+        const float *__restrict__ c = a.coef + (size_t)sec * a.coef_stride;
+        const float b1 = c[Y.kB + 0], b2 = c[Y.kB + 1], a1 = c[Y.kB + 2], a2 = c[Y.kB + 3];
+
+        // hot loop 1: zero-state biquad (ph_kernels.cuh:265-287)
+        {
+            float x2 = xi2, x1 = xi1, y2 = 0.0f, y1 = 0.0f;
+#pragma unroll
+            for (int n = 0; n < N; ++n) {
+                const float xc = r[n];
+                float yc = fmaf(b1, x1, fmaf(b2, x2, xc));
+                if (n == 1) yc = fmaf(a1, y1, yc);
+                else if (n > 1) yc = fmaf(a1, y1, fmaf(a2, y2, yc));
+                r[n] = yc;
+                x2 = x1;
+                x1 = xc;
+                y2 = y1;
+                y1 = yc;
+            }
+        }
+
+        // lane scan of the end states with C^(2^k)
+        float s2 = r[N - 2], s1 = r[N - 1];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int d = 1 << k;
+            const float *P = c + Y.kCpow + 4 * k;
+            const float t2 = lane_up(s2, d), t1 = lane_up(s1, d);
+            if (lane >= d) {
+                s2 = fmaf(P[1], t1, fmaf(P[0], t2, s2));
+                s1 = fmaf(P[3], t1, fmaf(P[2], t2, s1));
+            }
+        }
+
+        // chunk partial, lookback, chunk full (ph_kernels.cuh:384-469)
+        const float m0 = c[Y.kM + 0], m1 = c[Y.kM + 1], m2 = c[Y.kM + 2], m3 = c[Y.kM + 3];
+        const float p2 = readlane_f(s2, L - 1), p1 = readlane_f(s1, L - 1);
+        uint64_t *rec = rec2(a.lb, chunk, sec, a.ns);
+        if (lane == 0) publish2(rec, false, tk.part_flag, p2, p1);
+        const Carry2 X = lookback2(a.lb, chunk, sec, a.ns, tk.part_flag, tk.full_flag, m0, m1, m2, m3);
+        if (lane == 0) {
+            const float f2 = fmaf(m1, X.y1, fmaf(m0, X.y2, p2));
+            const float f1 = fmaf(m3, X.y1, fmaf(m2, X.y2, p1));
+            publish2(rec, true, tk.full_flag, f2, f1);
+        }
+
+        // per-lane end state with the chunk carry injected, C^(lane+1) X
+        const float4 cf = *reinterpret_cast<const float4 *>(c + Y.kCfin + 4 * lane);
+        const float e2 = fmaf(cf.y, X.y1, fmaf(cf.x, X.y2, s2));
+        const float e1 = fmaf(cf.w, X.y1, fmaf(cf.z, X.y2, s1));
+
+        // incoming state of each lane = end state of the previous lane
+        float u2 = lane_up(e2, 1), u1 = lane_up(e1, 1);
+        if (lane == 0) {
+            u2 = X.y2;
+            u1 = X.y1;
+        }
+
+        // hot loop 2: back-correction (ph_kernels.cuh:471-480)
+#pragma unroll
+        for (int n = 0; n < N - 2; ++n) r[n] = fmaf(c[Y.kH1 + n], u1, fmaf(c[Y.kH2 + n], u2, r[n]));
+        r[N - 2] = e2;
+        r[N - 1] = e1;
+
+        // the next section's input history is this section's output
+        xi2 = u2;
+        xi1 = u1;
+    }
+
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < N; ++n) lds[lane * (N + 1) + n] = r[n];
+    __syncthreads();
+    stage_out<L, N>(lds, a.out + base);
+}
+
+template <int N>
+static hipError_t launch_n(const LaunchArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(ph_kernel<N>, dim3(a.num_chunks), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s)
+{
+    if (lanes != 64) return hipErrorInvalidValue;
+    switch (n_blocks) {
+    case 8: return launch_n<8>(a, s);
+    case 16: return launch_n<16>(a, s);
+    case 32: return launch_n<32>(a, s);
+    case 64: return launch_n<64>(a, s);
+    case 128: return launch_n<128>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+const void *kernel_ph(int lanes, int n_blocks)
+{
+    if (lanes != 64) return nullptr;
+    switch (n_blocks) {
+    case 8: return (const void *)ph_kernel<8>;
+    case 16: return (const void *)ph_kernel<16>;
+    case 32: return (const void *)ph_kernel<32>;
+    case 64: return (const void *)ph_kernel<64>;
+    case 128: return (const void *)ph_kernel<128>;
+    default: return nullptr;
+    }
+}
+
+}  // namespace iirx

@@ -1,0 +1,33 @@
+// precompute.hpp -- FP64 host precompute of the per-section coefficient
+// tables (rounded once to float32).  The reference computes the same tables
+// in float32 on the host (ph_kernels.cuh:55-200, stcr_kernels.cuh:78-272,
+// dtcr_kernels.cuh:92-289, plr_kernels.cuh:149-476); computing them in FP64
+// removes the accumulated rounding of the matrix powers and recurrences.
+#pragma once
+
+#include <vector>
+
+namespace iirx {
+
+// 2x2 row-major double matrices
+struct M2 {
+    double a, b, c, d;
+};
+
+inline M2 m2_mul(const M2 &p, const M2 &q)
+{
+    return {p.a * q.a + p.b * q.c, p.a * q.b + p.b * q.d, p.c * q.a + p.d * q.c, p.c * q.b + p.d * q.d};
+}
+
+inline M2 m2_eye() { return {1.0, 0.0, 0.0, 1.0}; }
+
+// N-step zero-input response of the all-pole part 1/(1 - a1 z^-1 - a2 z^-2):
+// h1[n] = h0[n+1], h2[n] = a2 h0[n] (ph_kernels.cuh:55-68) and the N-step
+// transition C = [[h2[N-2], h1[N-2]], [h2[N-1], h1[N-1]]] (ph:180-181).
+void zir_tables(double a1, double a2, int N, std::vector<double> &h1, std::vector<double> &h2, M2 &C);
+
+// PH tables for all sections (layout kernels.hpp:ph_layout); returns the
+// per-section stride in floats.
+int ph_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &out);
+
+}  // namespace iirx

@@ -1,0 +1,199 @@
+/*
+ * iirx.h -- C ABI of the MI355X-native parallel cascaded-biquad IIR library
+ * (libiirx.so, built from cuda_recursive_filtering_amd/csrc).
+ *
+ * The reference (Haotian-RA/cuda_recursive_filtering) has no library
+ * boundary: a harness includes one kernel header chosen at compile time,
+ * calls setup_kernel_coefficients(sos) and launches
+ *     KERNEL_FUNC<<<grid, tb>>>(in, out, status, partcarry, fullcarry, launch)
+ * The entry points below replace that compile-time contract with a runtime
+ * plan.  Each declaration cites the reference interface it replaces.
+ *
+ * Filter contract (ref_generate.py:61-67, ph_kernels.cuh:157-161): sections
+ * are rows {b0, b1, b2, a0, a1, a2} in scipy's sign convention, normalised to
+ * b0 == 1 and a0 == 1; the removed cascade gain is applied to the input by
+ * the caller.  Each section computes
+ *     y[n] = u[n] + b1 u[n-1] + b2 u[n-2] - a1 y[n-1] - a2 y[n-2]
+ * with zero initial state, float32 in and float32 out.
+ *
+ * All device pointers are plain device addresses (hipMalloc or any
+ * allocator of the same device); `stream` is a hipStream_t passed as void*
+ * (NULL = the null stream).  Input and output must not overlap: a chunk
+ * reads input samples just before its own start.
+ *
+ * Every function returns an iirx_status; nothing aborts.
+ */
+#ifndef IIRX_H
+#define IIRX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (the reference asserts/aborts instead: ph_kernels.cuh:186-199,
+ * test_accuracy.cu:136-159). */
+typedef enum {
+    IIRX_OK = 0,
+    IIRX_ERR_INVALID = 1,        /* bad argument or unsupported configuration */
+    IIRX_ERR_GEOMETRY = 2,       /* length not a multiple of the chunk, or
+                                    no kernel instance for this geometry */
+    IIRX_ERR_CONDITIONING = 3,   /* CR factors ill-conditioned for this
+                                    (filter, N_BLOCKS): |e_n| too small */
+    IIRX_ERR_HIP = 4,            /* a HIP runtime call failed */
+    IIRX_ERR_NO_DEVICE = 5,      /* no GPU visible */
+    IIRX_ERR_OOM = 6
+} iirx_status;
+
+/* Algorithm selection (the KERNEL_* build flag, test_accuracy.cu:55-66). */
+typedef enum {
+    IIRX_KERNEL_PH = 0,    /* partial-LU factorisation, ph_kernels.cuh */
+    IIRX_KERNEL_STCR = 1,  /* single-stage tridiagonal CR, stcr_kernels.cuh */
+    IIRX_KERNEL_DTCR = 2,  /* decoupled two-sided CR, dtcr_kernels.cuh */
+    IIRX_KERNEL_PLR = 3    /* direct-form hierarchical merge, plr_kernels.cuh */
+} iirx_kernel;
+
+/*
+ * Plan configuration: the compile-time -D flags of run_accuracy.sh:155-166
+ * (ph_kernels.cuh:6-14, stcr_kernels.cuh:7-22, dtcr_kernels.cuh:11-26,
+ * plr_kernels.cuh:14-21) as runtime fields.
+ *
+ *   block_size  BLOCK_SIZE: lanes per chunk for PH/STCR/DTCR (32 or 64; the
+ *               wave64 build also accepts 128 and 256: several waves per
+ *               workgroup).  PLR: threads per block, 0 = per-order default.
+ *   n_blocks    N_BLOCKS: samples per lane, a power of two (PH: 8..128,
+ *               STCR/DTCR: 8..128).  Ignored by PLR.
+ *   n_sections  N_SECTIONS = order / 2 (1..8).
+ *   unrolled    STCR_HANDUNROLLED / DTCR_HANDUNROLLED: select the
+ *               template-expanded literal-index back substitution
+ *               (stcr_kernels.cuh:3613-3633).
+ *   plr_x       PLR values per thread; 0 = adaptive rule of plr:96-101
+ *               capped by plr_x_max.
+ *   plr_x_max   PLR_X_MAX; 0 = the build's default ceiling.
+ */
+typedef struct {
+    int32_t kernel;
+    int32_t block_size;
+    int32_t n_blocks;
+    int32_t n_sections;
+    int32_t unrolled;
+    int32_t plr_x;
+    int32_t plr_x_max;
+    int32_t reserved;
+} iirx_config;
+
+typedef struct iirx_plan iirx_plan;
+
+/* Launch facts of a plan, the printout of test_accuracy.cu:86-101 and the
+ * occupancy model of gpu_specs.hpp:57-116. */
+typedef struct {
+    int64_t n_samples;
+    int64_t chunk_samples;      /* samples per workgroup (lookback unit) */
+    int64_t num_chunks;         /* grid size */
+    int32_t threads_per_block;
+    int32_t lds_bytes;
+    int32_t vgprs;              /* from the code object metadata, 0 if unknown */
+    int32_t sgprs;
+    int32_t scratch_bytes;      /* private segment (spills) per lane */
+    int32_t waves_per_simd;     /* occupancy from VGPR/LDS/wave limits */
+    int64_t launches;           /* launches executed so far on this plan */
+} iirx_plan_info;
+
+/*
+ * Create a plan: host FP64 precompute of the coefficient tables, upload,
+ * allocation of the lookback state (status words, carries, ticket counter),
+ * zeroed once here and never again (test_accuracy.cu:145,
+ * test_performance.cu:170).
+ * Replaces setup_kernel_coefficients(sos) (ph_kernels.cuh:151,
+ * stcr_kernels.cuh:191, dtcr_kernels.cuh:193, plr_kernels.cuh:416) together
+ * with the driver's cudaMalloc/cudaMemset of status/partcarry/fullcarry
+ * (test_accuracy.cu:133-145).  `sos` is n_sections rows of 6 floats.  PLR
+ * derives its direct-form taps from `sos` in FP64 unless
+ * iirx_plan_set_plr_taps() supplies them (the reference reads them from the
+ * generated header, plr_kernels.cuh:416-427).
+ * The plan is bound to the current HIP device.
+ */
+int iirx_plan_create(const iirx_config *cfg, const float *sos,
+                     int64_t n_samples, iirx_plan **out);
+
+/* Direct-form taps for PLR (PLR_FILTER_B / PLR_FILTER_A of filter_taps.hpp,
+ * ref_generate.py:80-81, consumed at plr_kernels.cuh:423-427): b = FIR taps
+ * past the implicit b0 == 1, a = feedback taps in the recurrence sign
+ * y[n] += a[j] y[n-1-j]; k = order.  Recomputes the PLR tables. */
+int iirx_plan_set_plr_taps(iirx_plan *plan, const double *b, const double *a,
+                           int k);
+
+/*
+ * One filter call = exactly one grid-wide kernel launch on `stream`.
+ * Replaces KERNEL_FUNC<<<KERNEL_GRID_DIM, KERNEL_TB_DIM>>>(in, out, status,
+ * partcarry, fullcarry, launch) (test_accuracy.cu:146,
+ * test_performance.cu:177-180).  The launch index of the reference's
+ * versioned flags (ph_kernels.cuh:16-29) is derived on the device from the
+ * plan's ticket counter, so the caller passes none.  Calls on one plan must
+ * be ordered (one stream, or explicit dependencies); distinct plans are
+ * independent.
+ */
+int iirx_execute(iirx_plan *plan, const float *d_in, float *d_out,
+                 void *stream);
+
+int iirx_plan_info_get(const iirx_plan *plan, iirx_plan_info *info);
+
+int iirx_plan_destroy(iirx_plan *plan);
+
+/* Validity check of a configuration without a device, the static_assert of
+ * gpu_specs::config_is_valid (gpu_specs.hpp:106-113, test_accuracy.cu:41). */
+int iirx_config_check(const iirx_config *cfg, int64_t n_samples);
+
+const char *iirx_status_string(int status);
+
+/*
+ * measure_kernel_time (measure.cuh:38-110): `warmup` untimed calls, then
+ * `batches` batches of `batch` back-to-back calls, each batch bracketed by
+ * HIP events; inputs rotate over n_inputs buffers.  Writes the per-call
+ * mean of every batch, sorted ascending, to batch_ms[batches] (the P20 is
+ * batch_ms[batches/5]).
+ */
+int iirx_measure(iirx_plan *plan, const float *const *d_inputs, int n_inputs,
+                 float *d_out, int warmup, int batches, int batch,
+                 void *stream, float *batch_ms);
+
+/* Library version string. */
+const char *iirx_version(void);
+
+/* ---- host-side helpers (no device needed) ---------------------------- */
+
+/*
+ * Butterworth lowpass of even `order` at cutoff `wn` (x Nyquist), as
+ * scipy.signal.butter(order, wn, output='sos') followed by the b0 == 1
+ * normalisation of ref_generate.py:61-67: rows {1, 2, 1, 1, a1, a2} in float32
+ * and the removed cascade gain in *gain (float64).  Replaces the Python
+ * design step of tools/ref_generate.py:56-67.
+ */
+int iirx_butterworth_sos(int order, double wn, float *sos, double *gain);
+
+/* Direct-form taps of the cascade (the PLR input of ref_generate.py:74-81):
+ * FP64 polynomial product of the sections; b[0..2ns) FIR taps past b0 == 1,
+ * a[0..2ns) feedback taps in the recurrence sign. */
+int iirx_direct_form_taps(const float *sos, int ns, double *b, double *a);
+
+/* The documented N(0,1) generator of the synthetic workloads (SplitMix64 +
+ * Box-Muller in float64, rounded to float32). */
+int iirx_fill_normal(uint64_t seed, int64_t n, float *out);
+
+/*
+ * Host copy of the coefficient tables a plan of this configuration would
+ * upload (FP64 precompute rounded to float32).  Writes min(cap, size)
+ * floats to out (out may be NULL to query), the total size to *size and the
+ * per-section stride to *stride.  No device needed.
+ */
+int iirx_plan_tables(const iirx_config *cfg, const float *sos,
+                     int64_t n_samples, float *out, int64_t cap,
+                     int64_t *size, int32_t *stride);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IIRX_H */

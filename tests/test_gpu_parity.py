@@ -1,0 +1,122 @@
+"""GPU parity of the HIP kernels, called through the C ABI (libiirx.so).
+
+Bar: every cascaded kernel passes the reference's FP64
+gate max|out - ref| <= 1e-4 max|ref| (test_accuracy.cu:178-190) at every
+order, and agrees with the oracle's float32 restatement of the same algorithm
+within TOL_ULPS float32 ulps of max|ref| (the kernels scan lanes in a
+different order and use FP64-precomputed tables, so agreement is to rounding,
+not bit-exact).
+"""
+import numpy as np
+import pytest
+
+import cuda_recursive_filtering_amd as iirx
+import oracle_lib as O
+from workloads import normal_workload, reference_workload, ulp_scaled
+
+pytestmark = pytest.mark.gpu
+
+GEOMS = [(32, 32), (32, 64), (32, 128), (64, 64)]
+ORDERS = [2, 4, 8, 16]
+# agreement with the float32 restatement, in float32 eps * max|ref|
+TOL_ULPS = {"PH": 64.0}
+RESTATE = {"PH": O.ph}
+
+
+def run_gpu(device, kernel, sos, x, B, N, launches=1, **kw):
+    import torch
+    xin = torch.from_numpy(x).to(device)
+    out = torch.empty_like(xin)
+    with iirx.Plan(kernel, sos, x.size, block_size=B, n_blocks=N, **kw) as p:
+        for _ in range(launches):
+            p.execute(xin, out)
+        torch.cuda.synchronize()
+        info = p.info()
+    return out.cpu().numpy(), info
+
+
+@pytest.mark.parametrize("kernel", ["PH"])
+@pytest.mark.parametrize("order", ORDERS)
+@pytest.mark.parametrize("B,N", GEOMS)
+def test_parity_normal_input(device, kernel, order, B, N):
+    """Plain fp32 config at 2^20 samples (Wn=0.1, N(0,1) seed 1)."""
+    print(f"case {kernel} order={order} {B}x{N}", flush=True)
+    sos, g, x = normal_workload(order, 1 << 20, seed=1, wn=0.1)
+    y, info = run_gpu(device, kernel, sos, x, B, N)
+    ref = O.golden(sos, x)
+    ok, err, mref, rel = O.gate(y, ref)
+    assert ok, f"gate failed: rel={rel:.3e}"
+    emu = RESTATE[kernel](sos, x, B, N)
+    u = ulp_scaled(y, emu, ref)
+    print(f"  rel={rel:.2e} vs-restatement={u:.1f} ulps vgprs={info['vgprs']} scratch={info['scratch_bytes']}")
+    assert u <= TOL_ULPS[kernel], u
+    assert info["scratch_bytes"] == 0, "register spills"
+
+
+@pytest.mark.parametrize("kernel", ["PH"])
+@pytest.mark.parametrize("order", ORDERS)
+def test_parity_reference_workload(device, kernel, order):
+    """The reference's own workload (Wn=0.2, cos input) at 2^20."""
+    print(f"case ref-workload {kernel} order={order}", flush=True)
+    sos, g, x = reference_workload(order, 1 << 20)
+    y, _ = run_gpu(device, kernel, sos, x, 64, 64)
+    ref = O.golden(sos, x)
+    assert O.gate(y, ref)[0]
+
+
+@pytest.mark.parametrize("kernel", ["PH"])
+def test_launch_versioning_stale_flags(device, kernel):
+    """Repeated launches on one plan reuse the once-zeroed status records with
+    a new epoch each time (ph_kernels.cuh:16-29); alternate two inputs so a
+    stale carry from the previous launch would corrupt the result."""
+    import torch
+    print(f"case launch-versioning {kernel}", flush=True)
+    sos, g, xa = normal_workload(4, 1 << 18, seed=7)
+    _, _, xb = normal_workload(4, 1 << 18, seed=8)
+    ra, rb = O.golden(sos, xa), O.golden(sos, xb)
+    ta, tb = torch.from_numpy(xa).to(device), torch.from_numpy(xb).to(device)
+    out = torch.empty_like(ta)
+    with iirx.Plan(kernel, sos, xa.size, block_size=64, n_blocks=32) as p:
+        for i in range(9):
+            src, ref = (ta, ra) if i % 2 == 0 else (tb, rb)
+            p.execute(src, out)
+            torch.cuda.synchronize()
+            assert O.gate(out.cpu().numpy(), ref)[0], f"launch {i}"
+        assert p.info()["launches"] == 9
+
+
+@pytest.mark.parametrize("kernel", ["PH"])
+@pytest.mark.parametrize("n_chunks", [1, 2, 63, 64, 65, 200])
+def test_small_grids(device, kernel, n_chunks):
+    """Lookback window edges: fewer chunks than the 64-wide window, exactly
+    one window, and more."""
+    N = 16
+    n = 64 * N * n_chunks
+    print(f"case small-grid {kernel} chunks={n_chunks}", flush=True)
+    sos, g, x = normal_workload(8, n, seed=3, wn=0.1)
+    y, _ = run_gpu(device, kernel, sos, x, 64, N)
+    assert O.gate(y, O.golden(sos, x))[0]
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("kernel", ["PH"])
+@pytest.mark.parametrize("order", ORDERS)
+def test_full_size_gate(device, kernel, order):
+    """2^25 samples, the measured configuration, against the FP64 oracle."""
+    print(f"case full-size {kernel} order={order}", flush=True)
+    sos, g, x = normal_workload(order, 1 << 25, seed=1, wn=0.1)
+    y, _ = run_gpu(device, kernel, sos, x, 64, 64)
+    ok, err, mref, rel = O.gate(y, O.golden(sos, x))
+    print(f"  rel={rel:.2e}")
+    assert ok, rel
+
+
+def test_rejects_overlap_and_misalignment(device):
+    import torch
+    sos, g = iirx.butterworth_sos(2, 0.1)
+    buf = torch.zeros(2 * 64 * 64 + 4, device=device)
+    with iirx.Plan("PH", sos, 64 * 64, block_size=64, n_blocks=64) as p:
+        with pytest.raises(iirx.IirxError):
+            p.execute(buf.data_ptr(), buf.data_ptr() + 4 * 100)          # overlap
+        with pytest.raises(iirx.IirxError):
+            p.execute(buf.data_ptr() + 4, buf.data_ptr() + 4 * 64 * 64 + 16)  # misaligned input

@@ -1,0 +1,109 @@
+"""Host-side logic of libiirx.so, no GPU: the library loads and exports every
+symbol include/iirx.h declares; filter design, direct-form taps and the
+synthetic generator; FP64 table precompute against closed forms; geometry
+validation (gpu_specs::config_is_valid, test_accuracy.cu:37-41)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy import signal
+
+import cuda_recursive_filtering_amd as iirx
+from cuda_recursive_filtering_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIX = np.load(os.path.join(ROOT, "tests", "golden", "scipy_fixtures.npz"), allow_pickle=False)
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "iirx.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(iirx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    syms = header_symbols()
+    assert len(syms) >= 10
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(_lib._SIGS), "ctypes signatures out of sync with include/iirx.h"
+
+
+def test_version_and_status_strings():
+    assert "gfx950" in iirx.version()
+    for code in range(7):
+        assert _lib.lib().iirx_status_string(code)
+
+
+@pytest.mark.parametrize("order", [2, 4, 8, 16])
+@pytest.mark.parametrize("wn", [0.02, 0.1, 0.2, 0.4])
+def test_butterworth_matches_scipy_bit_exact_float32(order, wn):
+    ref = FIX[f"sos_o{order}_w{wn}"]
+    g_ref = float(np.prod(ref[:, 0]))
+    rn = ref.copy()
+    for k in range(rn.shape[0]):
+        rn[k, 0:3] = ref[k, 0:3] / ref[k, 0]
+    sos, g = iirx.butterworth_sos(order, wn)
+    assert np.array_equal(sos, rn.astype(np.float32))
+    assert abs(g - g_ref) <= 1e-12 * g_ref
+
+
+@pytest.mark.parametrize("order", [2, 4, 8, 16])
+def test_direct_form_taps_match_scipy_ba(order):
+    b, a = FIX[f"ba_b_o{order}_w0.2"], FIX[f"ba_a_o{order}_w0.2"]
+    sos, g = iirx.butterworth_sos(order, 0.2)
+    bt, at = iirx.direct_form_taps(sos)
+    # float32-rounded sections vs the float64 design: agree to float32 level
+    assert np.allclose(bt, b[1:] / b[0], rtol=0, atol=1e-6 * np.max(np.abs(b / b[0])))
+    assert np.allclose(at, -a[1:], rtol=0, atol=2e-5 * np.max(np.abs(a)))
+    # ref_generate.py:76-79: direct-form b0 equals the cascade gain
+    assert abs(b[0] - g) <= 1e-9 * g
+
+
+def test_fill_normal_deterministic_and_standard():
+    a = iirx.fill_normal(1, 1 << 18)
+    b = iirx.fill_normal(1, 1 << 18)
+    c = iirx.fill_normal(2, 1 << 18)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    assert abs(a.mean()) < 0.01 and abs(a.std() - 1.0) < 0.01
+    assert abs(np.mean(a.astype(np.float64) ** 4) - 3.0) < 0.05
+    assert np.array_equal(iirx.fill_normal(1, 1001), a[:1001])
+
+
+def test_ph_tables_match_closed_form():
+    """FP64 tables: C^(t+1) and the zero-input responses against numpy matrix
+    powers of the companion form (ph_kernels.cuh:55-144, Appendix A.4)."""
+    sos, g = iirx.butterworth_sos(4, 0.1)
+    N = 32
+    tab, stride = iirx.plan_tables("PH", sos, 64 * N * 4, block_size=64, n_blocks=N)
+    assert tab.size == stride * 2
+    for s in range(2):
+        t = tab[s * stride:(s + 1) * stride]
+        a1, a2 = -float(sos[s, 4]), -float(sos[s, 5])
+        A = np.array([[0.0, 1.0], [a2, a1]])  # (y[n-2], y[n-1]) -> (y[n-1], y[n])
+        C = np.linalg.matrix_power(A, N)
+        assert np.allclose(t[4:8], C.reshape(-1), rtol=1e-6, atol=1e-7)
+        kcfin = 4 + 4 * 6 + 4
+        for l in (0, 1, 31, 63):
+            ref = np.linalg.matrix_power(C, l + 1).reshape(-1)
+            assert np.allclose(t[kcfin + 4 * l:kcfin + 4 * l + 4], ref, rtol=1e-5, atol=1e-6 * np.abs(C).max())
+        # h1[n] = h0[n+1]: response at n to y[-1] = 1
+        kh1 = kcfin + 4 * 64
+        zi = signal.lfiltic([1.0], [1.0, -a1, -a2], y=[1.0, 0.0])
+        h1 = signal.lfilter([1.0], [1.0, -a1, -a2], np.zeros(N), zi=zi)[0]
+        assert np.allclose(t[kh1:kh1 + N], h1, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("kernel,B,N,n,ok", [
+    ("PH", 64, 64, 1 << 20, True),
+    ("PH", 32, 32, 1 << 20, True),
+    ("PH", 48, 64, 1 << 20, False),       # BLOCK_SIZE not 32/64
+    ("PH", 64, 48, 1 << 20, False),       # N_BLOCKS not a power of two
+    ("PH", 64, 64, (1 << 20) + 64, False),  # no tail path for cascaded kernels
+])
+def test_config_check(kernel, B, N, n, ok):
+    st = iirx.config_check(kernel, 2, n, block_size=B, n_blocks=N)
+    assert (st == 0) == ok, st
