@@ -33,6 +33,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--kernel", default="PH")
     ap.add_argument("--geom", default="64x64")
+    ap.add_argument("--schedule", default="persistent")
     ap.add_argument("--orders", default="2,4,8,16")
     ap.add_argument("--wn", type=float, default=0.1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -44,7 +45,7 @@ def parse():
 def time_order(iirx, torch, order, args, bufs, out, stream):
     B, N = (int(v) for v in args.geom.lower().split("x"))
     sos, g = iirx.butterworth_sos(order, args.wn)
-    plan = iirx.Plan(args.kernel, sos, N_SAMPLES, block_size=B, n_blocks=N)
+    plan = iirx.Plan(args.kernel, sos, N_SAMPLES, block_size=B, n_blocks=N, schedule=args.schedule)
     for i in range(args.warmup):
         plan.execute(bufs[i % len(bufs)], out, stream)
     torch.cuda.synchronize()
@@ -58,6 +59,7 @@ def time_order(iirx, torch, order, args, bufs, out, stream):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1) / args.steps
+    plan.check()
     info = plan.info()
     res = {"sos": sos, "gain": g, "ms": ms, "wall_ms": 1e3 * wall / args.steps, "info": info,
            "last_input": (args.warmup + args.steps - 1) % len(bufs), "plan": plan}
@@ -113,7 +115,8 @@ def main():
         gbs = BYTES_PER_SAMPLE * N_SAMPLES / (r["ms"] * 1e-3) / 1e9
         entry = {"gsamples_per_s": round(gs, 3), "ms": round(r["ms"], 5), "achieved_gbs": round(gbs, 1),
                  "roof_frac": round(gbs / HBM_PEAK_GBS, 4), "vgprs": r["info"]["vgprs"],
-                 "scratch": r["info"]["scratch_bytes"], "chunks": r["info"]["num_chunks"]}
+                 "scratch": r["info"]["scratch_bytes"], "chunks": r["info"]["num_chunks"],
+                 "grid": r["info"]["grid"]}
         if args.check:
             sys.path.insert(0, os.path.join(HERE, "tests"))
             import oracle_lib as O
@@ -143,7 +146,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"order-{orders[0]} Butterworth lowpass Wn={args.wn}, 2^25 float32 N(0,1) "
                                f"samples (seed 1, gain pre-applied), one channel, zero initial state",
-                   "kernel": args.kernel, "geometry": args.geom, "n_samples": N_SAMPLES,
+                   "kernel": args.kernel, "geometry": args.geom, "schedule": args.schedule,
+                   "n_samples": N_SAMPLES,
                    "input_buffers": nbuf},
         "orders": per,
         "roofline": {"bound": "hbm", "achieved": o2["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

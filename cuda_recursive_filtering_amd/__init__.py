@@ -33,7 +33,8 @@ def version():
     return lib().iirx_version().decode()
 
 
-def _config(kernel, n_sections, block_size=64, n_blocks=64, unrolled=False, plr_x=0, plr_x_max=0):
+def _config(kernel, n_sections, block_size=64, n_blocks=64, unrolled=False, plr_x=0, plr_x_max=0,
+            schedule="persistent"):
     if isinstance(kernel, str):
         if kernel.upper() not in KERNELS:
             raise ValueError(f"unknown kernel {kernel!r}; expected one of {sorted(KERNELS)}")
@@ -46,6 +47,7 @@ def _config(kernel, n_sections, block_size=64, n_blocks=64, unrolled=False, plr_
     c.unrolled = int(bool(unrolled))
     c.plr_x = int(plr_x)
     c.plr_x_max = int(plr_x_max)
+    c.schedule = _lib.SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
     return c
 
 
@@ -137,15 +139,16 @@ class Plan:
     n_blocks     samples per lane                           (N_BLOCKS)
     unrolled     literal-index back substitution            (HANDUNROLLED)
     plr_x, plr_x_max   PLR values per thread                (PLR_X / PLR_X_MAX)
+    schedule     'persistent' (default) or 'ticket'         (no counterpart)
     """
 
     def __init__(self, kernel, sos, n_samples, block_size=64, n_blocks=64, unrolled=False, plr_x=0, plr_x_max=0,
-                 plr_taps=None):
+                 plr_taps=None, schedule="persistent"):
         s = _sos_array(sos)
         self.kernel = kernel.upper() if isinstance(kernel, str) else kernel
         self.n_sections = s.shape[0]
         self.n_samples = int(n_samples)
-        self._cfg = _config(kernel, self.n_sections, block_size, n_blocks, unrolled, plr_x, plr_x_max)
+        self._cfg = _config(kernel, self.n_sections, block_size, n_blocks, unrolled, plr_x, plr_x_max, schedule)
         h = ctypes.c_void_p()
         check(lib().iirx_plan_create(ctypes.byref(self._cfg), s.ctypes.data, self.n_samples, ctypes.byref(h)),
               "plan_create")
@@ -169,6 +172,14 @@ class Plan:
         i = _lib.PlanInfo()
         check(lib().iirx_plan_info_get(self._h, ctypes.byref(i)), "plan_info")
         return {f: getattr(i, f) for f, _ in _lib.PlanInfo._fields_}
+
+    def check(self):
+        """Synchronise and raise IirxError(IIRX_ERR_TIMEOUT) if a lookback wait
+        exceeded the spin limit since the last check."""
+        check(lib().iirx_plan_check(self._h), "plan_check")
+
+    def set_spin_limit(self, polls):
+        check(lib().iirx_plan_set_spin_limit(self._h, int(polls)), "set_spin_limit")
 
     def measure(self, inputs, d_out, warmup=1000, batches=20, batch=200, stream=None):
         """measure_kernel_time: sorted per-call batch means in ms (P20 = [batches // 5])."""

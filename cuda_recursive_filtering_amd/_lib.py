@@ -16,6 +16,9 @@ IIRX_ERR_CONDITIONING = 3
 IIRX_ERR_HIP = 4
 IIRX_ERR_NO_DEVICE = 5
 IIRX_ERR_OOM = 6
+IIRX_ERR_TIMEOUT = 7
+
+SCHEDULES = {"persistent": 0, "ticket": 1}
 
 KERNELS = {"PH": 0, "STCR": 1, "DTCR": 2, "PLR": 3}
 
@@ -29,7 +32,7 @@ class Config(ctypes.Structure):
         ("unrolled", ctypes.c_int32),
         ("plr_x", ctypes.c_int32),
         ("plr_x_max", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("schedule", ctypes.c_int32),
     ]
 
 
@@ -45,6 +48,9 @@ class PlanInfo(ctypes.Structure):
         ("scratch_bytes", ctypes.c_int32),
         ("waves_per_simd", ctypes.c_int32),
         ("launches", ctypes.c_int64),
+        ("grid", ctypes.c_int64),
+        ("schedule", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
     ]
 
 
@@ -56,6 +62,8 @@ _SIGS = {
     "iirx_execute": (ctypes.c_int, [_P, _P, _P, _P]),
     "iirx_plan_info_get": (ctypes.c_int, [_P, ctypes.POINTER(PlanInfo)]),
     "iirx_plan_destroy": (ctypes.c_int, [_P]),
+    "iirx_plan_check": (ctypes.c_int, [_P]),
+    "iirx_plan_set_spin_limit": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "iirx_config_check": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int64]),
     "iirx_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "iirx_measure": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]),

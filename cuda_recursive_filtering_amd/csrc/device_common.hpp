@@ -4,7 +4,16 @@
 //
 // Lookback protocol (replaces the launch-versioned status/partcarry/fullcarry
 // triple of ph_kernels.cuh:16-29, 384-469).
-//   * Ticket: one 64-bit counter per plan, incremented once per workgroup.
+//   * Persistent schedule (default): the grid holds as many workgroups as the
+//     device keeps resident at once; workgroup b processes chunks b, b+G,
+//     b+2G, ...  A chunk waits only on lower chunks, all owned by resident
+//     workgroups, without a per-chunk atomic.  The launch epoch is a device
+//     word read by every workgroup at its start and advanced by the
+//     workgroup that finishes the last chunk (every other workgroup has read
+//     it by then: the last chunk's lookback observed its predecessor's
+//     record, which observed its own predecessor's, and so on).  A bounded
+//     spin turns a violated co-residency assumption into an error flag.
+//   * Ticket schedule: one 64-bit counter per plan, incremented once per workgroup.
 //     ticket = epoch * num_chunks + chunk.  Workgroups of one launch draw
 //     tickets in dispatch order, so a chunk only ever waits on chunks that
 //     already hold a ticket, i.e. are resident or finished (AMD hardware
@@ -34,6 +43,14 @@ constexpr int kWave = 64;
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+// Coefficient tables are read through the constant address space so that
+// uniform indices compile to scalar loads (s_load) into SGPRs; through a
+// generic pointer the compiler cannot prove the table unclobbered by the
+// lookback's atomics and falls back to per-lane vector loads.
+typedef const __attribute__((address_space(4))) float cfloat;
+
+__device__ __forceinline__ const cfloat *as_const(const float *p) { return (const cfloat *)p; }
+
 __device__ __forceinline__ float as_float(uint32_t u) { return __uint_as_float(u); }
 
 __device__ __forceinline__ uint32_t readfirstlane_u32(uint32_t v)
@@ -62,6 +79,16 @@ struct Ticket {
     uint32_t part_flag;
     uint32_t full_flag;
 };
+
+__device__ __forceinline__ uint32_t epoch_load(uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void epoch_store(uint32_t *p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ Ticket take_ticket(unsigned long long *ctr, uint32_t num_chunks)
 {
@@ -114,27 +141,40 @@ __device__ __forceinline__ void publish2(uint64_t *r, bool full, uint32_t flag, 
 }
 
 // Decoupled lookback over a 64-chunk window, one predecessor per lane.
-// Returns the full end state of chunk `chunk - 1` (zero state for chunk 0).
+// Returns the end state of chunk `chunk - 1` (zero state for chunk 0).
 // M is the chunk transition: X_c = M X_{c-1} + partial_c (row-major 2x2).
-// The wave spins until every predecessor from the nearest one up to the
-// nearest full one has published at least its partial, as the reference's
-// spin (ph_kernels.cuh:397-404); the partials in between are folded by
-// Horner's rule, nearest last (ph_kernels.cuh:436-450).
+//
+// `depth` bounds the fold: the plan sets it to the smallest d with
+// ||M^d||_inf <= 2^-36, so the end state of chunk c-1-d contributes less
+// than 2^-36 of its own magnitude to X and the partials of the d nearest
+// predecessors determine X far below float32 rounding.  The wave therefore
+// never waits for a predecessor to become full: it folds `depth` partials
+// from a zero state, or starts from the nearest full record if one is closer.
+// depth == 0 means unbounded (poles too close to the unit circle for this
+// chunk length): the wave spins until every predecessor up to the nearest
+// full one has published, as the reference does (ph_kernels.cuh:397-404).
+// The fold is Horner's rule, nearest predecessor last (ph:436-450).
+// Watchdog: after `spin_limit` unsuccessful polls the wave raises *err and
+// proceeds with a zero carry, so a broken co-residency assumption ends the
+// launch with an error instead of hanging the device.
 __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int sec, int ns,
                                             uint32_t part_flag, uint32_t full_flag,
-                                            float m0, float m1, float m2, float m3)
+                                            float m0, float m1, float m2, float m3, int depth,
+                                            uint32_t spin_limit, uint32_t *err)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const long pred = (long)chunk - 1 - lane;
+    const int window = (depth > 0 && depth < kWave) ? depth : kWave;
+    const bool active = lane < window;
+    const uint64_t below_depth = (depth >= kWave) ? ~0ull : ((1ull << depth) - 1ull);
     float v2 = 0.0f, v1 = 0.0f;
-    int st = 0;
-    int D;
+    // 0 not ready, 1 partial, 2 full, 3 before the signal start, 4 outside the window
+    int st = !active ? 4 : (pred < 0 ? 3 : 0);
+    int D = 0;
+    bool from_full = false;
+    uint32_t spins = 0;
     for (;;) {
-        if (pred < 0) {
-            st = 3;
-            v2 = 0.0f;
-            v1 = 0.0f;
-        } else if (st < 2) {
+        if (st < 2) {
             uint64_t *r = rec2(lb, (uint32_t)pred, sec, ns);
             const uint64_t f0 = rec_load(r + 2), f1 = rec_load(r + 3);
             if (rec_flag(f0) == full_flag && rec_flag(f1) == full_flag) {
@@ -151,17 +191,31 @@ __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int se
             }
         }
         const uint64_t zmask = __ballot(st == 0);
-        const uint64_t fmask = __ballot(st >= 2);
+        const uint64_t fmask = __ballot(st == 2 || st == 3);
         if (fmask) {
-            D = __builtin_ctzll(fmask);
-            const uint64_t need = (D == 63) ? ~0ull : ((1ull << (D + 1)) - 1ull);
-            if ((zmask & need) == 0) break;
+            const int f = __builtin_ctzll(fmask);
+            const uint64_t upto = (f == kWave - 1) ? ~0ull : ((1ull << (f + 1)) - 1ull);
+            if ((zmask & upto) == 0) {
+                D = f;
+                from_full = true;
+                break;
+            }
+        }
+        if (depth > 0 && (zmask & below_depth) == 0) {
+            D = depth;
+            break;
+        }
+        if (++spins > spin_limit) {
+            if (lane == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            D = 0;
+            from_full = false;
+            break;
         }
         __builtin_amdgcn_s_sleep(1);
     }
     Carry2 X;
-    X.y2 = readlane_f(v2, D);
-    X.y1 = readlane_f(v1, D);
+    X.y2 = from_full ? readlane_f(v2, D) : 0.0f;
+    X.y1 = from_full ? readlane_f(v1, D) : 0.0f;
     for (int d = D - 1; d >= 0; --d) {
         const float p2 = readlane_f(v2, d), p1 = readlane_f(v1, d);
         const float n2 = fmaf(m1, X.y1, fmaf(m0, X.y2, p2));
@@ -170,6 +224,27 @@ __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int se
         X.y1 = n1;
     }
     return X;
+}
+
+// ------------------------------------------------------------------------
+// chunk driver: runs body(chunk, part_flag, full_flag) for every chunk this
+// workgroup owns under the plan's schedule (see the protocol notes above)
+// ------------------------------------------------------------------------
+template <typename Body>
+__device__ __forceinline__ void for_each_chunk(unsigned long long *ticket, uint32_t *epoch_word,
+                                               uint32_t num_chunks, int persistent, Body &&body)
+{
+    if (persistent) {
+        const uint32_t epoch = epoch_load(epoch_word);
+        const uint32_t pf = 2u * epoch + 1u, ff = pf + 1u;
+        uint32_t c = blockIdx.x;
+        for (; c < num_chunks; c += gridDim.x) body(c, pf, ff);
+        // the owner of the last chunk advances the epoch after publishing it
+        if (threadIdx.x == 0 && (num_chunks - 1u) % gridDim.x == blockIdx.x) epoch_store(epoch_word, epoch + 1u);
+    } else {
+        const Ticket tk = take_ticket(ticket, num_chunks);
+        body(tk.chunk, tk.part_flag, tk.full_flag);
+    }
 }
 
 // ------------------------------------------------------------------------

@@ -36,7 +36,12 @@ struct iirx_plan {
     float *d_coef;
     uint64_t *d_lb;
     size_t lb_words;
-    unsigned long long *d_ticket;
+    unsigned long long *d_ticket;  // control block: [ticket u64][epoch u32][err u32]
+    uint32_t *d_epoch;
+    uint32_t *d_err;
+    uint32_t spin_limit;
+    uint32_t grid;
+    const void *fn;
     int64_t launches;
 };
 
@@ -49,8 +54,10 @@ int validate(const iirx_config *cfg, int64_t n)
     if (!cfg) return IIRX_ERR_INVALID;
     if (cfg->n_sections < 1 || cfg->n_sections > 8) return IIRX_ERR_INVALID;
     if (n <= 0 || n > (int64_t)1 << 34) return IIRX_ERR_INVALID;
+    if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && cfg->schedule != IIRX_SCHEDULE_TICKET) return IIRX_ERR_INVALID;
     switch (cfg->kernel) {
-    case IIRX_KERNEL_PH: {
+    case IIRX_KERNEL_PH:
+    case IIRX_KERNEL_STCR: {
         const int B = cfg->block_size, N = cfg->n_blocks;
         if (!(B == 32 || B == 64)) return IIRX_ERR_GEOMETRY;
         if (!is_pow2(N) || N < 8 || N > 128) return IIRX_ERR_GEOMETRY;
@@ -80,6 +87,7 @@ const char *iirx_status_string(int s)
     case IIRX_ERR_HIP: return "HIP runtime error";
     case IIRX_ERR_NO_DEVICE: return "no GPU device";
     case IIRX_ERR_OOM: return "out of memory";
+    case IIRX_ERR_TIMEOUT: return "lookback watchdog: a wait exceeded the spin limit (workgroups not co-resident)";
     default: return "unknown status";
     }
 }
@@ -103,6 +111,9 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     p->d_coef = nullptr;
     p->d_lb = nullptr;
     p->d_ticket = nullptr;
+    p->d_epoch = nullptr;
+    p->d_err = nullptr;
+    p->spin_limit = 1u << 22;
     p->launches = 0;
     hipGetDevice(&p->device);
 
@@ -115,6 +126,18 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
         p->chunk = (int64_t)p->lanes * p->n_blocks;
         p->coef_stride = ph_tables(p->sos.data(), ns, p->lanes, p->n_blocks, p->host_coef);
         p->lb_words = 4;  // 2 partial + 2 full words per (chunk, section)
+        break;
+    case IIRX_KERNEL_STCR:
+        p->lanes = 64;
+        p->n_blocks = cfg->n_blocks;
+        p->threads = 64;
+        p->chunk = (int64_t)p->lanes * p->n_blocks;
+        p->coef_stride = cr_tables(p->sos.data(), ns, p->lanes, p->n_blocks, p->host_coef);
+        if (p->coef_stride < 0) {
+            delete p;
+            return IIRX_ERR_CONDITIONING;
+        }
+        p->lb_words = 4;
         break;
     default:
         delete p;
@@ -132,12 +155,31 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     };
     if (hipMalloc((void **)&p->d_coef, p->host_coef.size() * sizeof(float)) != hipSuccess) return fail(IIRX_ERR_OOM);
     if (hipMalloc((void **)&p->d_lb, lb_count * sizeof(uint64_t)) != hipSuccess) return fail(IIRX_ERR_OOM);
-    if (hipMalloc((void **)&p->d_ticket, sizeof(unsigned long long)) != hipSuccess) return fail(IIRX_ERR_OOM);
+    if (hipMalloc((void **)&p->d_ticket, 16) != hipSuccess) return fail(IIRX_ERR_OOM);
+    p->d_epoch = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(p->d_ticket) + 8);
+    p->d_err = p->d_epoch + 1;
     if (hipMemcpy(p->d_coef, p->host_coef.data(), p->host_coef.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
         return fail(IIRX_ERR_HIP);
     // zeroed once; launch-versioned flags never need a reset afterwards
     if (hipMemset(p->d_lb, 0, lb_count * sizeof(uint64_t)) != hipSuccess) return fail(IIRX_ERR_HIP);
-    if (hipMemset(p->d_ticket, 0, sizeof(unsigned long long)) != hipSuccess) return fail(IIRX_ERR_HIP);
+    if (hipMemset(p->d_ticket, 0, 16) != hipSuccess) return fail(IIRX_ERR_HIP);
+    // grid: one workgroup per chunk (ticket) or per resident slot (persistent)
+    switch (cfg->kernel) {
+    case IIRX_KERNEL_PH: p->fn = kernel_ph(p->lanes, p->n_blocks); break;
+    case IIRX_KERNEL_STCR: p->fn = kernel_stcr(p->lanes, p->n_blocks, cfg->unrolled); break;
+    default: p->fn = nullptr; break;
+    }
+    if (!p->fn) return fail(IIRX_ERR_GEOMETRY);
+    p->grid = p->num_chunks;
+    if (cfg->schedule == IIRX_SCHEDULE_PERSISTENT) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->fn, p->threads, 0) != hipSuccess)
+            return fail(IIRX_ERR_HIP);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess)
+            return fail(IIRX_ERR_HIP);
+        const int64_t slots = (int64_t)std::max(per_cu, 1) * std::max(cus, 1);
+        p->grid = (uint32_t)std::min<int64_t>(slots, p->num_chunks);
+    }
     if (hipDeviceSynchronize() != hipSuccess) return fail(IIRX_ERR_HIP);
     *out = p;
     return IIRX_OK;
@@ -153,6 +195,10 @@ int iirx_plan_tables(const iirx_config *cfg, const float *sos, int64_t n_samples
     int str = 0;
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH: str = ph_tables(sos, cfg->n_sections, 64, cfg->n_blocks, t); break;
+    case IIRX_KERNEL_STCR:
+        str = cr_tables(sos, cfg->n_sections, 64, cfg->n_blocks, t);
+        if (str < 0) return IIRX_ERR_CONDITIONING;
+        break;
     default: return IIRX_ERR_INVALID;
     }
     if (size) *size = (int64_t)t.size();
@@ -175,14 +221,20 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
     a.coef = p->d_coef;
     a.lb = p->d_lb;
     a.ticket = p->d_ticket;
+    a.epoch = p->d_epoch;
+    a.err = p->d_err;
     a.n_samples = p->n_samples;
     a.num_chunks = p->num_chunks;
+    a.spin_limit = p->spin_limit;
+    a.persistent = p->cfg.schedule == IIRX_SCHEDULE_PERSISTENT;
+    a.grid = p->grid;
     a.ns = p->cfg.n_sections;
     a.coef_stride = p->coef_stride;
     a.unrolled = p->cfg.unrolled;
     hipError_t e = hipErrorInvalidValue;
     switch (p->cfg.kernel) {
     case IIRX_KERNEL_PH: e = launch_ph(p->lanes, p->n_blocks, a, s); break;
+    case IIRX_KERNEL_STCR: e = launch_stcr(p->lanes, p->n_blocks, a, s); break;
     default: break;
     }
     if (e != hipSuccess) return IIRX_ERR_HIP;
@@ -209,11 +261,9 @@ int iirx_plan_info_get(const iirx_plan *p, iirx_plan_info *info)
     info->num_chunks = p->num_chunks;
     info->threads_per_block = p->threads;
     info->launches = p->launches;
-    const void *fn = nullptr;
-    switch (p->cfg.kernel) {
-    case IIRX_KERNEL_PH: fn = kernel_ph(p->lanes, p->n_blocks); break;
-    default: break;
-    }
+    info->grid = p->grid;
+    info->schedule = p->cfg.schedule;
+    const void *fn = p->fn;
     if (fn) {
         hipFuncAttributes at;
         if (hipFuncGetAttributes(&at, fn) == hipSuccess) {
@@ -227,6 +277,27 @@ int iirx_plan_info_get(const iirx_plan *p, iirx_plan_info *info)
             info->waves_per_simd = nb * waves_per_block / 4;
         }
     }
+    return IIRX_OK;
+}
+
+int iirx_plan_check(iirx_plan *p)
+{
+    if (!p) return IIRX_ERR_INVALID;
+    if (hipDeviceSynchronize() != hipSuccess) return IIRX_ERR_HIP;
+    uint32_t err = 0;
+    if (hipMemcpy(&err, p->d_err, sizeof err, hipMemcpyDeviceToHost) != hipSuccess) return IIRX_ERR_HIP;
+    if (err) {
+        const uint32_t zero = 0;
+        if (hipMemcpy(p->d_err, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) return IIRX_ERR_HIP;
+        return IIRX_ERR_TIMEOUT;
+    }
+    return IIRX_OK;
+}
+
+int iirx_plan_set_spin_limit(iirx_plan *p, uint32_t polls)
+{
+    if (!p || polls == 0) return IIRX_ERR_INVALID;
+    p->spin_limit = polls;
     return IIRX_OK;
 }
 

@@ -13,9 +13,14 @@ struct LaunchArgs {
     float *out;
     const float *coef;           // per-section tables, `coef_stride` floats each
     uint64_t *lb;                // lookback records
-    unsigned long long *ticket;  // 64-bit workgroup ticket counter
+    unsigned long long *ticket;  // 64-bit workgroup ticket counter (ticket schedule)
+    uint32_t *epoch;             // launch epoch (persistent schedule)
+    uint32_t *err;               // watchdog flag
     int64_t n_samples;
-    uint32_t num_chunks;         // grid size
+    uint32_t num_chunks;         // chunks per launch
+    uint32_t spin_limit;         // lookback polls before the watchdog trips
+    int32_t persistent;          // 1: persistent schedule, 0: ticket schedule
+    uint32_t grid;               // workgroups launched
     int32_t ns;                  // sections (PH/STCR/DTCR) or order k (PLR)
     int32_t coef_stride;
     int32_t unrolled;
@@ -24,6 +29,9 @@ struct LaunchArgs {
 constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v >> 1); }
 constexpr int align4_c(int v) { return (v + 3) & ~3; }
 
+// ||M^d||_inf falls below this before the fold of the lookback stops
+constexpr double kLookbackTail = 1.4551915228366852e-11;  // 2^-36
+
 // ---------------------------------------------------------------------------
 // PH tables, one block per section (L = lanes per workgroup, N = samples per
 // lane).  C is the N-step transition of the section state (y[n-2], y[n-1]),
@@ -31,12 +39,14 @@ constexpr int align4_c(int v) { return (v + 3) & ~3; }
 //   [kB]     b1, b2, a1, a2          (a in recurrence sign, a = -sos[4:6])
 //   [kCpow]  C^(2^k), k < log2 L     uniform lane-scan operators
 //   [kM]     C^L                     chunk transition used by the lookback
+//   [kDepth] lookback depth d (smallest d with ||(C^L)^d||_inf <= 2^-36,
+//            0 = unbounded), stored as a float
 //   [kCfin]  C^(t+1), t < L          per-lane carry injection (C_inner)
 //   [kH1]    h1[n] = h0[n+1], n < N  zero-input response to y[-1]
 //   [kH2]    h2[n] = a2 h0[n]        zero-input response to y[-2]
 // ---------------------------------------------------------------------------
 struct PhLayout {
-    int kB, kCpow, kM, kCfin, kH1, kH2, stride;
+    int kB, kCpow, kM, kDepth, kCfin, kH1, kH2, stride;
 };
 
 constexpr PhLayout ph_layout(int L, int N)
@@ -45,7 +55,8 @@ constexpr PhLayout ph_layout(int L, int N)
     p.kB = 0;
     p.kCpow = 4;
     p.kM = p.kCpow + 4 * ilog2_c(L);
-    p.kCfin = p.kM + 4;
+    p.kDepth = p.kM + 4;
+    p.kCfin = p.kDepth + 4;
     p.kH1 = p.kCfin + 4 * L;
     p.kH2 = p.kH1 + N;
     p.stride = align4_c(p.kH2 + N);
@@ -56,5 +67,51 @@ constexpr PhLayout ph_layout(int L, int N)
 // instance.  Return hipErrorInvalidValue when no instance exists.
 hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s);
 const void *kernel_ph(int lanes, int n_blocks);
+
+}  // namespace iirx
+
+namespace iirx {
+
+// ---------------------------------------------------------------------------
+// CR tables (STCR and DTCR), one block per section; LG = log2 N (levels
+// 0..LG), L = lanes of the chunk.  CR factors of stcr_kernels.cuh:93-125:
+// e_n = 2 f_{n-1} - e_{n-1}^2, f_n = f_{n-1}^2, fde = f/e, h, g, p, q
+// (terminal recurrences), d, c (back-substitution), in FP64 then float32.
+//   [kScal]  b1, b2, cb, db, fb, eb, hbc, gbc  (DTCR decoupled first round,
+//            dtcr_kernels.cuh:134-139, 251-257)
+//   [kE] [kF] [kFde] [kH] [kG] [kP] [kQ] [kD] [kC]   8 floats each (n <= 7)
+//   [kApow]  A^(2^k), k < log2 L: lane-recurrence operator A = [[0,1],[-f_LG,-e_LG]]
+//            (the block filter of stcr:388-405 as a lane scan)
+//   [kM]     chunk transition C^L of the section state (lookback)
+//   [kDepth] lookback depth
+//   [kTerm]  per lane t: {he2, he1, hb2, hb1}[t]  (stcr:127-147)
+// ---------------------------------------------------------------------------
+struct CrLayout {
+    int kScal, kE, kF, kFde, kH, kG, kP, kQ, kD, kC, kApow, kM, kDepth, kTerm, stride;
+};
+
+constexpr CrLayout cr_layout(int L)
+{
+    CrLayout p{};
+    p.kScal = 0;
+    p.kE = 8;
+    p.kF = p.kE + 8;
+    p.kFde = p.kF + 8;
+    p.kH = p.kFde + 8;
+    p.kG = p.kH + 8;
+    p.kP = p.kG + 8;
+    p.kQ = p.kP + 8;
+    p.kD = p.kQ + 8;
+    p.kC = p.kD + 8;
+    p.kApow = p.kC + 8;
+    p.kM = p.kApow + 4 * ilog2_c(L);
+    p.kDepth = p.kM + 4;
+    p.kTerm = p.kDepth + 4;
+    p.stride = align4_c(p.kTerm + 4 * L);
+    return p;
+}
+
+hipError_t launch_stcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s);
+const void *kernel_stcr(int lanes, int n_blocks, int unrolled);
 
 }  // namespace iirx

@@ -27,10 +27,11 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
     __shared__ float lds[L * (N + 1)];
 
     const int lane = threadIdx.x;
-    const Ticket tk = take_ticket(a.ticket, a.num_chunks);
-    const uint32_t chunk = tk.chunk;
+    for_each_chunk(a.ticket, a.epoch, a.num_chunks, a.persistent,
+                   [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
     const long base = (long)chunk * (L * N);
 
+    __syncthreads();  // LDS of the previous chunk fully consumed
     stage_in<L, N>(a.in + base, lds);
     __syncthreads();
 
@@ -52,7 +53,8 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
     }
 
     for (int sec = 0; sec < a.ns; ++sec) {
-        const float *__restrict__ c = a.coef + (size_t)sec * a.coef_stride;
+        const float *cv = a.coef + (size_t)sec * a.coef_stride;  // per-lane vector loads
+        const cfloat *c = as_const(cv);                           // uniform scalar loads
         const float b1 = c[Y.kB + 0], b2 = c[Y.kB + 1], a1 = c[Y.kB + 2], a2 = c[Y.kB + 3];
 
         // hot loop 1: zero-state biquad (ph_kernels.cuh:265-287)
@@ -77,7 +79,7 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             const int d = 1 << k;
-            const float *P = c + Y.kCpow + 4 * k;
+            const cfloat *P = c + Y.kCpow + 4 * k;
             const float t2 = lane_up(s2, d), t1 = lane_up(s1, d);
             if (lane >= d) {
                 s2 = fmaf(P[1], t1, fmaf(P[0], t2, s2));
@@ -89,16 +91,18 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
         const float m0 = c[Y.kM + 0], m1 = c[Y.kM + 1], m2 = c[Y.kM + 2], m3 = c[Y.kM + 3];
         const float p2 = readlane_f(s2, L - 1), p1 = readlane_f(s1, L - 1);
         uint64_t *rec = rec2(a.lb, chunk, sec, a.ns);
-        if (lane == 0) publish2(rec, false, tk.part_flag, p2, p1);
-        const Carry2 X = lookback2(a.lb, chunk, sec, a.ns, tk.part_flag, tk.full_flag, m0, m1, m2, m3);
+        if (lane == 0) publish2(rec, false, part_flag, p2, p1);
+        const int depth = (int)c[Y.kDepth];
+        const Carry2 X = lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
+                                   a.spin_limit, a.err);
         if (lane == 0) {
             const float f2 = fmaf(m1, X.y1, fmaf(m0, X.y2, p2));
             const float f1 = fmaf(m3, X.y1, fmaf(m2, X.y2, p1));
-            publish2(rec, true, tk.full_flag, f2, f1);
+            publish2(rec, true, full_flag, f2, f1);
         }
 
         // per-lane end state with the chunk carry injected, C^(lane+1) X
-        const float4 cf = *reinterpret_cast<const float4 *>(c + Y.kCfin + 4 * lane);
+        const float4 cf = *reinterpret_cast<const float4 *>(cv + Y.kCfin + 4 * lane);
         const float e2 = fmaf(cf.y, X.y1, fmaf(cf.x, X.y2, s2));
         const float e1 = fmaf(cf.w, X.y1, fmaf(cf.z, X.y2, s1));
 
@@ -125,12 +129,13 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
     for (int n = 0; n < N; ++n) lds[lane * (N + 1) + n] = r[n];
     __syncthreads();
     stage_out<L, N>(lds, a.out + base);
+    });
 }
 
 template <int N>
 static hipError_t launch_n(const LaunchArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(ph_kernel<N>, dim3(a.num_chunks), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(ph_kernel<N>, dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

@@ -3,6 +3,9 @@
 
 #include "kernels.hpp"
 
+#include <algorithm>
+#include <cmath>
+
 namespace iirx {
 
 void zir_tables(double a1, double a2, int N, std::vector<double> &h1, std::vector<double> &h2, M2 &C)
@@ -18,6 +21,17 @@ void zir_tables(double a1, double a2, int N, std::vector<double> &h1, std::vecto
         h2[n] = a2 * h0[n];
     }
     C = {h2[N - 2], h1[N - 2], h2[N - 1], h1[N - 1]};
+}
+
+int lookback_depth(const M2 &M)
+{
+    M2 P = M;
+    for (int d = 1; d <= 64; ++d) {
+        const double n = std::max(std::fabs(P.a) + std::fabs(P.b), std::fabs(P.c) + std::fabs(P.d));
+        if (n <= kLookbackTail) return d;
+        P = m2_mul(P, M);
+    }
+    return 0;
 }
 
 static void put(float *dst, const M2 &m)
@@ -52,7 +66,10 @@ int ph_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &ou
         M2 Q = C;
         for (int l = 0; l < lanes; ++l) {
             put(t + Y.kCfin + 4 * l, Q);
-            if (l == lanes - 1) put(t + Y.kM, Q);
+            if (l == lanes - 1) {
+                put(t + Y.kM, Q);
+                t[Y.kDepth] = (float)lookback_depth(Q);
+            }
             Q = m2_mul(Q, C);
         }
         for (int n = 0; n < N; ++n) {
@@ -60,6 +77,100 @@ int ph_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &ou
             t[Y.kH2 + n] = (float)h2[n];
         }
     }
+    return Y.stride;
+}
+
+}  // namespace iirx
+
+namespace iirx {
+
+int cr_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &out, double *worst)
+{
+    const CrLayout Y = cr_layout(lanes);
+    const int LG = ilog2_c(N);
+    out.assign((size_t)Y.stride * ns, 0.0f);
+    double wmax = 0.0;
+    bool bad = false;
+    for (int s = 0; s < ns; ++s) {
+        float *t = out.data() + (size_t)s * Y.stride;
+        const float *row = sos + 6 * s;
+        const double b1 = row[1], b2 = row[2];
+        const double a1 = -(double)row[4], a2 = -(double)row[5];
+        double e[9], f[9], fde[9], h[9], g[9], p[9], q[9], d[9], c[9];
+        for (int i = 0; i < 9; ++i) e[i] = f[i] = fde[i] = h[i] = g[i] = p[i] = q[i] = d[i] = c[i] = 0.0;
+        f[0] = -a2;
+        e[0] = -a1;
+        fde[0] = f[0] / e[0];
+        h[0] = f[0];
+        g[0] = e[0];
+        p[1] = f[0];
+        q[1] = e[0];
+        for (int n = 1; n <= LG; ++n) {
+            f[n] = f[n - 1] * f[n - 1];
+            e[n] = 2.0 * f[n - 1] - e[n - 1] * e[n - 1];
+            fde[n] = f[n] / e[n];
+            h[n] = -e[n - 1] * h[n - 1];
+            g[n] = f[n - 1] - e[n - 1] * g[n - 1];
+            d[n] = -f[n - 1] * f[n - 1] / e[n - 1];
+            c[n] = e[n - 1] - f[n - 1] / e[n - 1];
+            if (n > 1) {
+                q[n] = -e[n - 1] * q[n - 1];
+                p[n] = f[n - 1] - e[n - 1] * p[n - 1];
+            }
+        }
+        for (int n = 1; n < LG; ++n) wmax = std::max(wmax, std::fabs(fde[n]));
+        for (int n = 1; n <= LG; ++n) wmax = std::max({wmax, std::fabs(d[n]), std::fabs(c[n])});
+        t[Y.kScal + 0] = (float)b1;
+        t[Y.kScal + 1] = (float)b2;
+        t[Y.kScal + 2] = (float)(f[0] * b1 - e[0] * b2);   // cb
+        t[Y.kScal + 3] = (float)(b1 - e[0]);               // db
+        t[Y.kScal + 4] = (float)(b2 * f[0]);               // fb
+        t[Y.kScal + 5] = (float)(b2 + f[0] - b1 * e[0]);   // eb
+        t[Y.kScal + 6] = (float)(b2 - e[0] * b1);          // hb (chunk-start odd side)
+        t[Y.kScal + 7] = (float)(-e[0] * b2);              // gb
+        for (int n = 0; n <= LG; ++n) {
+            t[Y.kE + n] = (float)e[n];
+            t[Y.kF + n] = (float)f[n];
+            t[Y.kFde + n] = (float)fde[n];
+            t[Y.kH + n] = (float)h[n];
+            t[Y.kG + n] = (float)g[n];
+            t[Y.kP + n] = (float)p[n];
+            t[Y.kQ + n] = (float)q[n];
+            t[Y.kD + n] = (float)d[n];
+            t[Y.kC + n] = (float)c[n];
+        }
+        // lane recurrence Y_t = R_t - e_LG Y_{t-1} - f_LG Y_{t-2}
+        const M2 A = {0.0, 1.0, -f[LG], -e[LG]};
+        M2 P = A;
+        for (int k = 0; k < ilog2_c(lanes); ++k) {
+            put(t + Y.kApow + 4 * k, P);
+            P = m2_mul(P, P);
+        }
+        // impulse response over lanes and the terminal corrections (stcr:127-147)
+        std::vector<double> h0((size_t)lanes);
+        h0[0] = 1.0;
+        if (lanes > 1) h0[1] = -e[LG];
+        for (int l = 2; l < lanes; ++l) h0[l] = -e[LG] * h0[l - 1] - f[LG] * h0[l - 2];
+        for (int l = 0; l < lanes; ++l) {
+            const double prev = (l == 0) ? 0.0 : h0[l - 1];
+            t[Y.kTerm + 4 * l + 0] = (float)(f[LG] * prev + p[LG] * h0[l]);  // he2
+            t[Y.kTerm + 4 * l + 1] = (float)(q[LG] * h0[l]);                 // he1
+            t[Y.kTerm + 4 * l + 2] = (float)(h[LG] * h0[l]);                 // hb2
+            t[Y.kTerm + 4 * l + 3] = (float)(f[LG] * prev + g[LG] * h0[l]);  // hb1
+        }
+        // chunk transition of the section state for the lookback
+        std::vector<double> hh1, hh2;
+        M2 C;
+        zir_tables(a1, a2, N, hh1, hh2, C);
+        M2 Q = m2_eye();
+        for (int l = 0; l < lanes; ++l) Q = m2_mul(Q, C);
+        put(t + Y.kM, Q);
+        t[Y.kDepth] = (float)lookback_depth(Q);
+        for (int i = 0; i < Y.stride; ++i)
+            if (!std::isfinite(t[i])) bad = true;
+    }
+    if (worst) *worst = wmax;
+    if (bad || wmax > kCrLimit) return -1;
     return Y.stride;
 }
 

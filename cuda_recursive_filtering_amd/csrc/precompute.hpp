@@ -26,8 +26,18 @@ inline M2 m2_eye() { return {1.0, 0.0, 0.0, 1.0}; }
 // transition C = [[h2[N-2], h1[N-2]], [h2[N-1], h1[N-1]]] (ph:180-181).
 void zir_tables(double a1, double a2, int N, std::vector<double> &h1, std::vector<double> &h2, M2 &C);
 
+// Smallest d in [1, 64] with ||M^d||_inf <= 2^-36, 0 if there is none.
+int lookback_depth(const M2 &M);
+
 // PH tables for all sections (layout kernels.hpp:ph_layout); returns the
 // per-section stride in floats.
 int ph_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &out);
+
+// CR tables for all sections (layout kernels.hpp:cr_layout).  Returns the
+// stride, or -1 when the CR factors are ill-conditioned for this (filter, N):
+// some |e_n| is so small that the elimination multipliers f/e, d, c exceed
+// kCrLimit (e_n = -2 r^(2^n) cos(2^n theta) for a pole pair r e^(+-i theta)).
+constexpr double kCrLimit = 1.0e4;
+int cr_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &out, double *worst = nullptr);
 
 }  // namespace iirx

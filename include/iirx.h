@@ -44,8 +44,20 @@ typedef enum {
                                     (filter, N_BLOCKS): |e_n| too small */
     IIRX_ERR_HIP = 4,            /* a HIP runtime call failed */
     IIRX_ERR_NO_DEVICE = 5,      /* no GPU visible */
-    IIRX_ERR_OOM = 6
+    IIRX_ERR_OOM = 6,
+    IIRX_ERR_TIMEOUT = 7         /* a lookback wait exceeded the plan's spin
+                                    limit (workgroups not co-resident); the
+                                    output of that call is invalid */
 } iirx_status;
+
+/* Workgroup scheduling of a plan (no counterpart in the reference, which
+ * always launches one block per chunk with an atomic ticket). */
+typedef enum {
+    IIRX_SCHEDULE_PERSISTENT = 0, /* one workgroup per resident slot, chunks
+                                     assigned round-robin; no per-chunk atomic */
+    IIRX_SCHEDULE_TICKET = 1      /* one workgroup per chunk, chunk order from
+                                     an atomic ticket (ph_kernels.cuh:227-229) */
+} iirx_schedule;
 
 /* Algorithm selection (the KERNEL_* build flag, test_accuracy.cu:55-66). */
 typedef enum {
@@ -72,6 +84,12 @@ typedef enum {
  *   plr_x       PLR values per thread; 0 = adaptive rule of plr:96-101
  *               capped by plr_x_max.
  *   plr_x_max   PLR_X_MAX; 0 = the build's default ceiling.
+ *   schedule    iirx_schedule.  PERSISTENT launches as many workgroups as
+ *               the device holds at once (occupancy API) and relies on them
+ *               being co-resident; a bounded-spin watchdog turns a violated
+ *               assumption into IIRX_ERR_TIMEOUT instead of a hang.  TICKET
+ *               needs no co-residency but pays one serialised device-scope
+ *               atomic per chunk (about 12 ns each on MI355X).
  */
 typedef struct {
     int32_t kernel;
@@ -81,7 +99,7 @@ typedef struct {
     int32_t unrolled;
     int32_t plr_x;
     int32_t plr_x_max;
-    int32_t reserved;
+    int32_t schedule;
 } iirx_config;
 
 typedef struct iirx_plan iirx_plan;
@@ -99,6 +117,9 @@ typedef struct {
     int32_t scratch_bytes;      /* private segment (spills) per lane */
     int32_t waves_per_simd;     /* occupancy from VGPR/LDS/wave limits */
     int64_t launches;           /* launches executed so far on this plan */
+    int64_t grid;               /* workgroups per launch */
+    int32_t schedule;
+    int32_t pad;
 } iirx_plan_info;
 
 /*
@@ -130,8 +151,10 @@ int iirx_plan_set_plr_taps(iirx_plan *plan, const double *b, const double *a,
  * Replaces KERNEL_FUNC<<<KERNEL_GRID_DIM, KERNEL_TB_DIM>>>(in, out, status,
  * partcarry, fullcarry, launch) (test_accuracy.cu:146,
  * test_performance.cu:177-180).  The launch index of the reference's
- * versioned flags (ph_kernels.cuh:16-29) is derived on the device from the
- * plan's ticket counter, so the caller passes none.  Calls on one plan must
+ * versioned flags (ph_kernels.cuh:16-29) is kept on the device (an epoch word
+ * advanced by the workgroup that finishes the last chunk, or the ticket
+ * counter under IIRX_SCHEDULE_TICKET), so the caller passes none and a
+ * captured graph replays correctly.  Calls on one plan must
  * be ordered (one stream, or explicit dependencies); distinct plans are
  * independent.
  */
@@ -139,6 +162,16 @@ int iirx_execute(iirx_plan *plan, const float *d_in, float *d_out,
                  void *stream);
 
 int iirx_plan_info_get(const iirx_plan *plan, iirx_plan_info *info);
+
+/* Waits for the plan's launches to finish and reports IIRX_ERR_TIMEOUT if any
+ * lookback wait since the last check exceeded the spin limit (then clears the
+ * flag).  The reference has no failure detection (its spin can hang,
+ * ph_kernels.cuh:397-404). */
+int iirx_plan_check(iirx_plan *plan);
+
+/* Lookback spin limit in polls (default 1 << 22, several seconds).  Tests
+ * force the watchdog path with a tiny limit. */
+int iirx_plan_set_spin_limit(iirx_plan *plan, uint32_t polls);
 
 int iirx_plan_destroy(iirx_plan *plan);
 

@@ -19,8 +19,9 @@ pytestmark = pytest.mark.gpu
 GEOMS = [(32, 32), (32, 64), (32, 128), (64, 64)]
 ORDERS = [2, 4, 8, 16]
 # agreement with the float32 restatement, in float32 eps * max|ref|
-TOL_ULPS = {"PH": 64.0}
-RESTATE = {"PH": O.ph}
+TOL_ULPS = {"PH": 64.0, "STCR": 256.0}
+RESTATE = {"PH": O.ph, "STCR": O.stcr}
+KERNELS = ["PH", "STCR"]
 
 
 def run_gpu(device, kernel, sos, x, B, N, launches=1, **kw):
@@ -35,7 +36,7 @@ def run_gpu(device, kernel, sos, x, B, N, launches=1, **kw):
     return out.cpu().numpy(), info
 
 
-@pytest.mark.parametrize("kernel", ["PH"])
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("order", ORDERS)
 @pytest.mark.parametrize("B,N", GEOMS)
 def test_parity_normal_input(device, kernel, order, B, N):
@@ -53,7 +54,7 @@ def test_parity_normal_input(device, kernel, order, B, N):
     assert info["scratch_bytes"] == 0, "register spills"
 
 
-@pytest.mark.parametrize("kernel", ["PH"])
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("order", ORDERS)
 def test_parity_reference_workload(device, kernel, order):
     """The reference's own workload (Wn=0.2, cos input) at 2^20."""
@@ -64,42 +65,117 @@ def test_parity_reference_workload(device, kernel, order):
     assert O.gate(y, ref)[0]
 
 
-@pytest.mark.parametrize("kernel", ["PH"])
-def test_launch_versioning_stale_flags(device, kernel):
+@pytest.mark.parametrize("schedule", ["persistent", "ticket"])
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_launch_versioning_stale_flags(device, kernel, schedule):
     """Repeated launches on one plan reuse the once-zeroed status records with
     a new epoch each time (ph_kernels.cuh:16-29); alternate two inputs so a
     stale carry from the previous launch would corrupt the result."""
     import torch
-    print(f"case launch-versioning {kernel}", flush=True)
+    print(f"case launch-versioning {kernel} {schedule}", flush=True)
     sos, g, xa = normal_workload(4, 1 << 18, seed=7)
     _, _, xb = normal_workload(4, 1 << 18, seed=8)
     ra, rb = O.golden(sos, xa), O.golden(sos, xb)
     ta, tb = torch.from_numpy(xa).to(device), torch.from_numpy(xb).to(device)
     out = torch.empty_like(ta)
-    with iirx.Plan(kernel, sos, xa.size, block_size=64, n_blocks=32) as p:
+    with iirx.Plan(kernel, sos, xa.size, block_size=64, n_blocks=32, schedule=schedule) as p:
         for i in range(9):
             src, ref = (ta, ra) if i % 2 == 0 else (tb, rb)
             p.execute(src, out)
             torch.cuda.synchronize()
             assert O.gate(out.cpu().numpy(), ref)[0], f"launch {i}"
-        assert p.info()["launches"] == 9
+        p.check()
+        info = p.info()
+        assert info["launches"] == 9
+        if schedule == "ticket":
+            assert info["grid"] == info["num_chunks"]
+        else:
+            assert info["grid"] <= info["num_chunks"]
 
 
-@pytest.mark.parametrize("kernel", ["PH"])
-@pytest.mark.parametrize("n_chunks", [1, 2, 63, 64, 65, 200])
-def test_small_grids(device, kernel, n_chunks):
+@pytest.mark.parametrize("schedule", ["persistent", "ticket"])
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("n_chunks", [1, 2, 63, 64, 65, 200, 5000])
+def test_small_grids(device, kernel, n_chunks, schedule):
     """Lookback window edges: fewer chunks than the 64-wide window, exactly
-    one window, and more."""
+    one window, more, and more chunks than resident workgroups."""
     N = 16
     n = 64 * N * n_chunks
-    print(f"case small-grid {kernel} chunks={n_chunks}", flush=True)
+    print(f"case small-grid {kernel} chunks={n_chunks} {schedule}", flush=True)
     sos, g, x = normal_workload(8, n, seed=3, wn=0.1)
-    y, _ = run_gpu(device, kernel, sos, x, 64, N)
+    y, info = run_gpu(device, kernel, sos, x, 64, N, schedule=schedule)
     assert O.gate(y, O.golden(sos, x))[0]
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_watchdog_terminates_and_reports(device, kernel):
+    """Failure handling forced from the host: with a one-poll spin limit the
+    lookback gives up instead of waiting, the launch still ends, and
+    plan.check() reports IIRX_ERR_TIMEOUT (or OK if no wait needed a second
+    poll); the flag is cleared by the check.  Then the default limit gives a
+    correct result on the same plan."""
+    import torch
+    print(f"case watchdog {kernel}", flush=True)
+    sos, g, x = normal_workload(4, 64 * 16 * 4096, seed=9, wn=0.1)
+    xin = torch.from_numpy(x).to(device)
+    out = torch.empty_like(xin)
+    with iirx.Plan(kernel, sos, x.size, block_size=64, n_blocks=16) as p:
+        p.set_spin_limit(1)
+        p.execute(xin, out)
+        try:
+            p.check()
+            tripped = False
+        except iirx.IirxError as e:
+            assert e.code == iirx._lib.IIRX_ERR_TIMEOUT
+            tripped = True
+        print(f"  tripped={tripped}")
+        p.check()  # cleared
+        p.set_spin_limit(1 << 22)
+        p.execute(xin, out)
+        p.check()
+        torch.cuda.synchronize()
+        assert O.gate(out.cpu().numpy(), O.golden(sos, x))[0]
+
+
+# offset of the lookback depth in the first section's table (kernels.hpp)
+DEPTH_AT = {"PH": 4 + 4 * 6 + 4, "STCR": 8 * 10 + 4 * 6 + 4}
+
+
+def resonator(r, theta):
+    return np.array([[1.0, 2.0, 1.0, 1.0, -2.0 * r * np.cos(theta), r * r]], np.float32)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("filt,N,depth", [(("butter", 0.005), 8, 5), (("butter", 0.002), 16, 7),
+                                          (("resonator", 0.9995), 8, 0)])
+def test_lookback_depths(device, kernel, filt, N, depth):
+    """Poles near the unit circle: the bounded lookback folds several partials
+    (depth > 1), and for a resonator at r = 0.9995 with 512-sample lanes it is
+    unbounded (depth 0) and waits for full records.  Narrow lowpass filters
+    amplify float32 rounding (even a sequential float32 cascade can miss the
+    1e-4 gate), so the bar is the gate or 10x the sequential float32 cascade's
+    own error."""
+    n = 64 * N * 300
+    print(f"case lookback-depth {kernel} {filt} N={N}", flush=True)
+    if filt[0] == "butter":
+        sos, g, x = normal_workload(2, n, seed=11, wn=filt[1])
+    else:
+        sos = resonator(filt[1], 1.0)
+        x = (1e-3 * iirx.fill_normal(11, n)).astype(np.float32)
+    tab, stride = iirx.plan_tables(kernel, sos, n, block_size=64, n_blocks=N)
+    got_depth = int(tab[DEPTH_AT[kernel]])
+    print(f"  depth={got_depth}")
+    assert got_depth == depth
+    y, _ = run_gpu(device, kernel, sos, x, 64, N)
+    ref = O.golden(sos, x)
+    ok, err, mref, rel = O.gate(y, ref)
+    casc = O.gate(O.cascade_f32(sos, x), ref)[3]
+    print(f"  rel={rel:.2e} sequential-f32={casc:.2e}")
+    assert ok or rel <= 10.0 * casc, rel
+
+
 @pytest.mark.slow
-@pytest.mark.parametrize("kernel", ["PH"])
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("order", ORDERS)
 def test_full_size_gate(device, kernel, order):
     """2^25 samples, the measured configuration, against the FP64 oracle."""
@@ -109,6 +185,17 @@ def test_full_size_gate(device, kernel, order):
     ok, err, mref, rel = O.gate(y, O.golden(sos, x))
     print(f"  rel={rel:.2e}")
     assert ok, rel
+
+
+@pytest.mark.parametrize("N", [8, 32, 64, 128])
+def test_stcr_loop_equals_unrolled_on_device(device, N):
+    """The loop and literal-index back substitutions execute the same operation
+    sequence (stcr_kernels.cuh:17-19): bit-identical on the device."""
+    print(f"case stcr loop==unrolled N={N}", flush=True)
+    sos, g, x = normal_workload(8, 64 * N * 64, seed=5, wn=0.1)
+    y0, _ = run_gpu(device, "STCR", sos, x, 64, N, unrolled=False)
+    y1, _ = run_gpu(device, "STCR", sos, x, 64, N, unrolled=True)
+    assert np.array_equal(y0, y1)
 
 
 def test_rejects_overlap_and_misalignment(device):

@@ -86,7 +86,7 @@ def test_ph_tables_match_closed_form():
         A = np.array([[0.0, 1.0], [a2, a1]])  # (y[n-2], y[n-1]) -> (y[n-1], y[n])
         C = np.linalg.matrix_power(A, N)
         assert np.allclose(t[4:8], C.reshape(-1), rtol=1e-6, atol=1e-7)
-        kcfin = 4 + 4 * 6 + 4
+        kcfin = 4 + 4 * 6 + 4 + 4
         for l in (0, 1, 31, 63):
             ref = np.linalg.matrix_power(C, l + 1).reshape(-1)
             assert np.allclose(t[kcfin + 4 * l:kcfin + 4 * l + 4], ref, rtol=1e-5, atol=1e-6 * np.abs(C).max())
@@ -95,6 +95,22 @@ def test_ph_tables_match_closed_form():
         zi = signal.lfiltic([1.0], [1.0, -a1, -a2], y=[1.0, 0.0])
         h1 = signal.lfilter([1.0], [1.0, -a1, -a2], np.zeros(N), zi=zi)[0]
         assert np.allclose(t[kh1:kh1 + N], h1, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("wn,N,lo,hi", [(0.1, 64, 1, 1), (0.005, 8, 2, 8), (0.0001, 8, 0, 0)])
+def test_lookback_depth(wn, N, lo, hi):
+    """Depth of the bounded lookback: smallest d with ||(C^64)^d||_inf <= 2^-36
+    (0 = unbounded), checked against numpy matrix powers."""
+    sos, g = iirx.butterworth_sos(2, wn)
+    tab, stride = iirx.plan_tables("PH", sos, 64 * N * 4, block_size=64, n_blocks=N)
+    kM = 4 + 4 * 6
+    depth = int(tab[kM + 4])
+    assert lo <= depth <= hi, depth
+    a1, a2 = -float(sos[0, 4]), -float(sos[0, 5])
+    M = np.linalg.matrix_power(np.array([[0.0, 1.0], [a2, a1]]), 64 * N)
+    norms = [np.abs(np.linalg.matrix_power(M, d)).sum(axis=1).max() for d in range(1, 65)]
+    ref = next((d + 1 for d, v in enumerate(norms) if v <= 2.0 ** -36), 0)
+    assert depth == ref
 
 
 @pytest.mark.parametrize("kernel,B,N,n,ok", [
