@@ -37,6 +37,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 namespace iirx {
 
 constexpr int kWave = 64;
@@ -70,6 +72,32 @@ __device__ __forceinline__ int readlane_i(int v, int lane)
 
 // value of lane (l - d) for l >= d, own value otherwise (CUDA shfl_up rule)
 __device__ __forceinline__ float lane_up(float v, int d) { return __shfl_up(v, (unsigned)d, kWave); }
+
+// ------------------------------------------------------------------------
+// compile-time loops: every index a literal (the "hand-unrolled" forms)
+// ------------------------------------------------------------------------
+template <int I>
+using ic = std::integral_constant<int, I>;
+
+// f(ic<I>) for I = B, B+S, ... < E, every index a compile-time literal
+template <int B, int E, int S, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(ic<B>{});
+        static_for<B + S, E, S>(f);
+    }
+}
+
+// descending: f(ic<I>) for I = B, B-1, ... > E
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for_down(F &&f)
+{
+    if constexpr (B > E) {
+        f(ic<B>{});
+        static_for_down<B - 1, E>(f);
+    }
+}
 
 // ------------------------------------------------------------------------
 // ticket
@@ -252,19 +280,20 @@ __device__ __forceinline__ void for_each_chunk(unsigned long long *ticket, uint3
 // Chunk of L lanes x N samples, lane t owns samples [t*N, t*N + N); LDS rows
 // are padded to N+1 floats so that the per-lane row reads hit distinct banks.
 // ------------------------------------------------------------------------
-template <int L, int N>
+template <int L, int N, int T = L>
 __device__ __forceinline__ void stage_in(const float *__restrict__ src, float *lds)
 {
     constexpr int V = L * N / 4;           // float4s in the chunk
-    constexpr int PER = V / L;             // float4s per thread
+    constexpr int PER = V / T;             // float4s per thread
+    static_assert(V % T == 0, "chunk must split evenly over the threads");
     const int tid = threadIdx.x;
     const f4 *s4 = reinterpret_cast<const f4 *>(src);
     f4 v[PER];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) v[i] = __builtin_nontemporal_load(s4 + i * L + tid);
+    for (int i = 0; i < PER; ++i) v[i] = __builtin_nontemporal_load(s4 + i * T + tid);
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        const int s = 4 * (i * L + tid);
+        const int s = 4 * (i * T + tid);
         const int row = s / N, col = s % N;
         float *d = lds + row * (N + 1) + col;
         d[0] = v[i].x;
@@ -274,16 +303,16 @@ __device__ __forceinline__ void stage_in(const float *__restrict__ src, float *l
     }
 }
 
-template <int L, int N>
+template <int L, int N, int T = L>
 __device__ __forceinline__ void stage_out(const float *lds, float *__restrict__ dst)
 {
     constexpr int V = L * N / 4;
-    constexpr int PER = V / L;
+    constexpr int PER = V / T;
     const int tid = threadIdx.x;
     f4 *d4 = reinterpret_cast<f4 *>(dst);
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        const int s = 4 * (i * L + tid);
+        const int s = 4 * (i * T + tid);
         const int row = s / N, col = s % N;
         const float *p = lds + row * (N + 1) + col;
         f4 v;
@@ -291,7 +320,7 @@ __device__ __forceinline__ void stage_out(const float *lds, float *__restrict__ 
         v.y = p[1];
         v.z = p[2];
         v.w = p[3];
-        __builtin_nontemporal_store(v, d4 + i * L + tid);
+        __builtin_nontemporal_store(v, d4 + i * T + tid);
     }
 }
 

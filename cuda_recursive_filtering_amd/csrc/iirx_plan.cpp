@@ -57,7 +57,8 @@ int validate(const iirx_config *cfg, int64_t n)
     if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && cfg->schedule != IIRX_SCHEDULE_TICKET) return IIRX_ERR_INVALID;
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH:
-    case IIRX_KERNEL_STCR: {
+    case IIRX_KERNEL_STCR:
+    case IIRX_KERNEL_DTCR: {
         const int B = cfg->block_size, N = cfg->n_blocks;
         if (!(B == 32 || B == 64)) return IIRX_ERR_GEOMETRY;
         if (!is_pow2(N) || N < 8 || N > 128) return IIRX_ERR_GEOMETRY;
@@ -139,6 +140,18 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
         }
         p->lb_words = 4;
         break;
+    case IIRX_KERNEL_DTCR:
+        p->lanes = 64;
+        p->n_blocks = cfg->n_blocks;
+        p->threads = 128;  // two sibling waves: even and odd sides
+        p->chunk = (int64_t)p->lanes * p->n_blocks;
+        p->coef_stride = cr_tables(p->sos.data(), ns, p->lanes, p->n_blocks, p->host_coef);
+        if (p->coef_stride < 0) {
+            delete p;
+            return IIRX_ERR_CONDITIONING;
+        }
+        p->lb_words = 4;
+        break;
     default:
         delete p;
         return IIRX_ERR_INVALID;
@@ -167,6 +180,7 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH: p->fn = kernel_ph(p->lanes, p->n_blocks); break;
     case IIRX_KERNEL_STCR: p->fn = kernel_stcr(p->lanes, p->n_blocks, cfg->unrolled); break;
+    case IIRX_KERNEL_DTCR: p->fn = kernel_dtcr(p->lanes, p->n_blocks, cfg->unrolled); break;
     default: p->fn = nullptr; break;
     }
     if (!p->fn) return fail(IIRX_ERR_GEOMETRY);
@@ -196,6 +210,7 @@ int iirx_plan_tables(const iirx_config *cfg, const float *sos, int64_t n_samples
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH: str = ph_tables(sos, cfg->n_sections, 64, cfg->n_blocks, t); break;
     case IIRX_KERNEL_STCR:
+    case IIRX_KERNEL_DTCR:
         str = cr_tables(sos, cfg->n_sections, 64, cfg->n_blocks, t);
         if (str < 0) return IIRX_ERR_CONDITIONING;
         break;
@@ -235,6 +250,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
     switch (p->cfg.kernel) {
     case IIRX_KERNEL_PH: e = launch_ph(p->lanes, p->n_blocks, a, s); break;
     case IIRX_KERNEL_STCR: e = launch_stcr(p->lanes, p->n_blocks, a, s); break;
+    case IIRX_KERNEL_DTCR: e = launch_dtcr(p->lanes, p->n_blocks, a, s); break;
     default: break;
     }
     if (e != hipSuccess) return IIRX_ERR_HIP;

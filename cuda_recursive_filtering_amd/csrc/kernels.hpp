@@ -113,5 +113,7 @@ constexpr CrLayout cr_layout(int L)
 
 hipError_t launch_stcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s);
 const void *kernel_stcr(int lanes, int n_blocks, int unrolled);
+hipError_t launch_dtcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s);
+const void *kernel_dtcr(int lanes, int n_blocks, int unrolled);
 
 }  // namespace iirx

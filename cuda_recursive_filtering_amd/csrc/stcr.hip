@@ -24,35 +24,10 @@
 // literal-index form is generated by compile-time index recursion
 // (static_for) instead of being written out; both forms execute the same
 // operation sequence and are tested for bit-identity on the device.
-#include <utility>
-
 #include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace iirx {
-
-template <int I>
-using ic = std::integral_constant<int, I>;
-
-// f(ic<I>) for I = B, B+S, ... < E, every index a compile-time literal
-template <int B, int E, int S, typename F>
-__device__ __forceinline__ void static_for(F &&f)
-{
-    if constexpr (B < E) {
-        f(ic<B>{});
-        static_for<B + S, E, S>(f);
-    }
-}
-
-// descending: f(ic<I>) for I = B, B-1, ... > E
-template <int B, int E, typename F>
-__device__ __forceinline__ void static_for_down(F &&f)
-{
-    if constexpr (B > E) {
-        f(ic<B>{});
-        static_for_down<B - 1, E>(f);
-    }
-}
 
 // forward CR round `ro`, parity i (stcr:364-386): loop form
 template <int N>
