@@ -32,13 +32,16 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--kernel", default="PH")
-    ap.add_argument("--geom", default="64x64")
+    ap.add_argument("--geom", default="64x32")
     ap.add_argument("--schedule", default="persistent")
     ap.add_argument("--orders", default="2,4,8,16")
     ap.add_argument("--wn", type=float, default=0.1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--check", action="store_true", help="gate the last output against the FP64 oracle")
+    ap.add_argument("--copy-steps", type=int, default=50,
+                    help="device-to-device copies of one 2^25-sample buffer, timed as the measured HBM ceiling "
+                         "(also the known-size kernel that scales rocprof FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -64,6 +67,37 @@ def time_order(iirx, torch, order, args, bufs, out, stream):
     res = {"sos": sos, "gain": g, "ms": ms, "wall_ms": 1e3 * wall / args.steps, "info": info,
            "last_input": (args.warmup + args.steps - 1) % len(bufs), "plan": plan}
     return res
+
+
+def copy_ceiling(torch, args, dev, stream):
+    """Measured copy bandwidth: read 2^25 floats + write 2^25 floats per copy."""
+    a = torch.empty(N_SAMPLES, dtype=torch.float32, device=dev).normal_()
+    bufs = [a.clone() for _ in range(4)]
+    dst = torch.empty_like(a)
+    for i in range(5):
+        dst.copy_(bufs[i % 4])
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for i in range(args.copy_steps):
+        dst.copy_(bufs[i % 4])
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / max(args.copy_steps, 1)
+    return {"ms": round(ms, 5), "gbs": round(BYTES_PER_SAMPLE * N_SAMPLES / (ms * 1e-3) / 1e9, 1)}
+
+
+def measured_traffic(args):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary of this
+    kernel/geometry (profiles/traffic.json), or None."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(f"{args.kernel}_{args.geom}_order2", None)
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_baseline(iirx, args):
@@ -130,6 +164,7 @@ def main():
         torch.cuda.empty_cache()
         print(f"# order {order}: {entry}", file=sys.stderr, flush=True)
 
+    ceiling = copy_ceiling(torch, args, dev, stream) if args.copy_steps > 0 else None
     o2 = per[str(orders[0])]
     line = {
         "metric": "GSamples/s & % HBM roof at 2^25, order 2-16",
@@ -151,7 +186,9 @@ def main():
                    "input_buffers": nbuf},
         "orders": per,
         "roofline": {"bound": "hbm", "achieved": o2["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": o2["roof_frac"], "traffic": None},
+                     "frac": o2["roof_frac"], "traffic": measured_traffic(args),
+                     "algorithmic_bytes": BYTES_PER_SAMPLE * N_SAMPLES},
+        "copy_ceiling": ceiling,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(iirx, args)
