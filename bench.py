@@ -48,7 +48,10 @@ def parse():
 def time_order(iirx, torch, order, args, bufs, out, stream):
     B, N = (int(v) for v in args.geom.lower().split("x"))
     sos, g = iirx.butterworth_sos(order, args.wn)
-    plan = iirx.Plan(args.kernel, sos, N_SAMPLES, block_size=B, n_blocks=N, schedule=args.schedule)
+    if args.kernel.upper() == "PLR":  # --geom BLOCKxX, 0 = automatic
+        plan = iirx.Plan("PLR", sos, N_SAMPLES, block_size=B, plr_x=N, schedule=args.schedule)
+    else:
+        plan = iirx.Plan(args.kernel, sos, N_SAMPLES, block_size=B, n_blocks=N, schedule=args.schedule)
     for i in range(args.warmup):
         plan.execute(bufs[i % len(bufs)], out, stream)
     torch.cuda.synchronize()

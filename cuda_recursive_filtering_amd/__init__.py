@@ -135,17 +135,20 @@ class Plan:
     kernel       'PH' | 'STCR' | 'DTCR' | 'PLR'             (KERNEL)
     sos          (n_sections, 6) b0-normalised sections     (FILTER_SOS)
     n_samples    signal length                              (2^N_SAMPLES_LOG2)
-    block_size   lanes per chunk, 32 or 64                  (BLOCK_SIZE)
+    block_size   lanes per chunk, 32 or 64; PLR: threads per
+                 block (0/None = per-order default)         (BLOCK_SIZE / PLR_BLOCK_SIZE)
     n_blocks     samples per lane                           (N_BLOCKS)
     unrolled     literal-index back substitution            (HANDUNROLLED)
     plr_x, plr_x_max   PLR values per thread                (PLR_X / PLR_X_MAX)
     schedule     'persistent' (default) or 'ticket'         (no counterpart)
     """
 
-    def __init__(self, kernel, sos, n_samples, block_size=64, n_blocks=64, unrolled=False, plr_x=0, plr_x_max=0,
+    def __init__(self, kernel, sos, n_samples, block_size=None, n_blocks=64, unrolled=False, plr_x=0, plr_x_max=0,
                  plr_taps=None, schedule="persistent"):
         s = _sos_array(sos)
         self.kernel = kernel.upper() if isinstance(kernel, str) else kernel
+        if block_size is None:
+            block_size = 0 if self.kernel in ("PLR", KERNELS["PLR"]) else 64
         self.n_sections = s.shape[0]
         self.n_samples = int(n_samples)
         self._cfg = _config(kernel, self.n_sections, block_size, n_blocks, unrolled, plr_x, plr_x_max, schedule)

@@ -21,6 +21,18 @@
 
 using namespace iirx;
 
+namespace {
+// per-kernel launch shape and tables of a configuration
+struct Shape {
+    int lanes, n_blocks, threads, plr_block, plr_x;  // plr_*: threads per block, values per thread
+    int64_t chunk;
+    uint32_t num_chunks;
+    size_t lb_count;
+    int coef_stride;
+};
+
+}  // namespace
+
 struct iirx_plan {
     iirx_config cfg;
     int64_t n_samples;
@@ -35,7 +47,8 @@ struct iirx_plan {
     int coef_stride;
     float *d_coef;
     uint64_t *d_lb;
-    size_t lb_words;
+    Shape shape;
+    std::vector<double> plr_b, plr_a;  // direct-form taps supplied by the caller
     unsigned long long *d_ticket;  // control block: [ticket u64][epoch u32][err u32]
     uint32_t *d_epoch;
     uint32_t *d_err;
@@ -67,8 +80,107 @@ int validate(const iirx_config *cfg, int64_t n)
         if (n / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
         return IIRX_OK;
     }
+    case IIRX_KERNEL_PLR: {
+        const int ns = cfg->n_sections;
+        if (!(ns == 1 || ns == 2 || ns == 4 || ns == 8)) return IIRX_ERR_GEOMETRY;
+        const int B = cfg->block_size;
+        if (!(B == 0 || B == 64 || B == 128 || B == 256 || B == 512)) return IIRX_ERR_GEOMETRY;
+        const int X = cfg->plr_x;
+        if (!(X == 0 || X == 1 || X == 2 || X == 4 || X == 8 || X == 16)) return IIRX_ERR_GEOMETRY;
+        if (cfg->plr_x_max < 0) return IIRX_ERR_INVALID;
+        return IIRX_OK;
+    }
     default:
         return IIRX_ERR_INVALID;
+    }
+}
+
+// PLR geometry (plr_kernels.cuh:49-112): threads per block by order unless
+// given, values per thread from the adaptive rule X = n / (CUs_p2 * BLOCK)
+// clamped to [1, X_MAX] and rounded down to an instantiated power of two.
+void plr_geometry(const iirx_config *cfg, int64_t n, int *block, int *x)
+{
+    const int k = 2 * cfg->n_sections;
+    int B = cfg->block_size;
+    if (B == 0) B = (k <= 4) ? 256 : (k == 8 ? 128 : 64);
+    int X = cfg->plr_x;
+    if (X == 0) {
+        const int xmax = (cfg->plr_x_max > 0) ? std::min(cfg->plr_x_max, 16) : 16;
+        int64_t raw = n / (256LL * B);
+        raw = std::max<int64_t>(1, std::min<int64_t>(raw, xmax));
+        X = 1;
+        while (X * 2 <= raw) X *= 2;
+    }
+    *block = B;
+    *x = X;
+}
+
+int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double *plr_b, const double *plr_a,
+               Shape &sh, std::vector<float> &tables)
+{
+    const int ns = cfg->n_sections;
+    sh.lanes = 64;
+    sh.n_blocks = cfg->n_blocks;
+    sh.threads = 64;
+    sh.plr_block = sh.plr_x = 0;
+    switch (cfg->kernel) {
+    case IIRX_KERNEL_PH:
+        sh.coef_stride = ph_tables(sos, ns, 64, cfg->n_blocks, tables);
+        break;
+    case IIRX_KERNEL_STCR:
+    case IIRX_KERNEL_DTCR:
+        if (cfg->kernel == IIRX_KERNEL_DTCR) sh.threads = 128;  // even and odd sibling waves
+        sh.coef_stride = cr_tables(sos, ns, 64, cfg->n_blocks, tables);
+        if (sh.coef_stride < 0) return IIRX_ERR_CONDITIONING;
+        break;
+    case IIRX_KERNEL_PLR: {
+        const int k = 2 * ns;
+        plr_geometry(cfg, n, &sh.plr_block, &sh.plr_x);
+        sh.threads = sh.plr_block;
+        sh.chunk = (int64_t)sh.plr_block * sh.plr_x;
+        std::vector<double> b((size_t)k), a((size_t)k);
+        if (plr_b && plr_a) {
+            b.assign(plr_b, plr_b + k);
+            a.assign(plr_a, plr_a + k);
+        } else if (iirx_direct_form_taps(sos, ns, b.data(), a.data()) != IIRX_OK) {
+            return IIRX_ERR_INVALID;
+        }
+        // PLR_X_MAX calibration (run_accuracy.sh:95-151 greps ptxas for spills):
+        // halve X while the instance needs scratch, from the code object's
+        // own resource metadata (only when a device is present)
+        if (cfg->plr_x == 0) {
+            for (;;) {
+                const void *fn = kernel_plr(k, sh.plr_block, sh.plr_x);
+                hipFuncAttributes at;
+                if (sh.plr_x <= 1 || !fn || hipFuncGetAttributes(&at, fn) != hipSuccess || at.localSizeBytes == 0)
+                    break;
+                sh.plr_x /= 2;
+            }
+            (void)hipGetLastError();
+            sh.chunk = (int64_t)sh.plr_block * sh.plr_x;
+        }
+        sh.coef_stride = plr_tables(b.data(), a.data(), k, (int)sh.chunk, tables);
+        sh.num_chunks = (uint32_t)((n + sh.chunk - 1) / sh.chunk);
+        sh.lb_count = (size_t)sh.num_chunks * 2u * (size_t)k;
+        return IIRX_OK;
+    }
+    default:
+        return IIRX_ERR_INVALID;
+    }
+    sh.chunk = (int64_t)sh.lanes * sh.n_blocks;
+    sh.num_chunks = (uint32_t)(n / sh.chunk);
+    sh.lb_count = (size_t)sh.num_chunks * (size_t)ns * 4u;  // 2 partial + 2 full words per (chunk, section)
+    return IIRX_OK;
+}
+
+const void *kernel_fn(const iirx_config *cfg, const Shape &sh)
+{
+    switch (cfg->kernel) {
+    case IIRX_KERNEL_PH: return kernel_ph(sh.lanes, sh.n_blocks);
+    case IIRX_KERNEL_STCR: return kernel_stcr(sh.lanes, sh.n_blocks, cfg->unrolled);
+    case IIRX_KERNEL_DTCR: return kernel_dtcr(sh.lanes, sh.n_blocks, cfg->unrolled);
+    case IIRX_KERNEL_PLR: return kernel_plr(2 * cfg->n_sections, sh.plr_block, sh.plr_x);
+    default: return nullptr;
     }
 }
 
@@ -118,46 +230,20 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     p->launches = 0;
     hipGetDevice(&p->device);
 
-    const int ns = cfg->n_sections;
-    switch (cfg->kernel) {
-    case IIRX_KERNEL_PH:
-        p->lanes = 64;
-        p->n_blocks = cfg->n_blocks;
-        p->threads = 64;
-        p->chunk = (int64_t)p->lanes * p->n_blocks;
-        p->coef_stride = ph_tables(p->sos.data(), ns, p->lanes, p->n_blocks, p->host_coef);
-        p->lb_words = 4;  // 2 partial + 2 full words per (chunk, section)
-        break;
-    case IIRX_KERNEL_STCR:
-        p->lanes = 64;
-        p->n_blocks = cfg->n_blocks;
-        p->threads = 64;
-        p->chunk = (int64_t)p->lanes * p->n_blocks;
-        p->coef_stride = cr_tables(p->sos.data(), ns, p->lanes, p->n_blocks, p->host_coef);
-        if (p->coef_stride < 0) {
-            delete p;
-            return IIRX_ERR_CONDITIONING;
-        }
-        p->lb_words = 4;
-        break;
-    case IIRX_KERNEL_DTCR:
-        p->lanes = 64;
-        p->n_blocks = cfg->n_blocks;
-        p->threads = 128;  // two sibling waves: even and odd sides
-        p->chunk = (int64_t)p->lanes * p->n_blocks;
-        p->coef_stride = cr_tables(p->sos.data(), ns, p->lanes, p->n_blocks, p->host_coef);
-        if (p->coef_stride < 0) {
-            delete p;
-            return IIRX_ERR_CONDITIONING;
-        }
-        p->lb_words = 4;
-        break;
-    default:
+    Shape sh;
+    st = make_shape(cfg, p->sos.data(), n_samples, nullptr, nullptr, sh, p->host_coef);
+    if (st != IIRX_OK) {
         delete p;
-        return IIRX_ERR_INVALID;
+        return st;
     }
-    p->num_chunks = (uint32_t)(n_samples / p->chunk);
-    const size_t lb_count = (size_t)p->num_chunks * (size_t)ns * p->lb_words;
+    p->shape = sh;
+    p->lanes = sh.lanes;
+    p->n_blocks = sh.n_blocks;
+    p->threads = sh.threads;
+    p->chunk = sh.chunk;
+    p->num_chunks = sh.num_chunks;
+    p->coef_stride = sh.coef_stride;
+    const size_t lb_count = sh.lb_count;
 
     auto fail = [&](int code) {
         if (p->d_coef) hipFree(p->d_coef);
@@ -177,12 +263,7 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     if (hipMemset(p->d_lb, 0, lb_count * sizeof(uint64_t)) != hipSuccess) return fail(IIRX_ERR_HIP);
     if (hipMemset(p->d_ticket, 0, 16) != hipSuccess) return fail(IIRX_ERR_HIP);
     // grid: one workgroup per chunk (ticket) or per resident slot (persistent)
-    switch (cfg->kernel) {
-    case IIRX_KERNEL_PH: p->fn = kernel_ph(p->lanes, p->n_blocks); break;
-    case IIRX_KERNEL_STCR: p->fn = kernel_stcr(p->lanes, p->n_blocks, cfg->unrolled); break;
-    case IIRX_KERNEL_DTCR: p->fn = kernel_dtcr(p->lanes, p->n_blocks, cfg->unrolled); break;
-    default: p->fn = nullptr; break;
-    }
+    p->fn = kernel_fn(cfg, sh);
     if (!p->fn) return fail(IIRX_ERR_GEOMETRY);
     p->grid = p->num_chunks;
     if (cfg->schedule == IIRX_SCHEDULE_PERSISTENT) {
@@ -206,26 +287,32 @@ int iirx_plan_tables(const iirx_config *cfg, const float *sos, int64_t n_samples
     int st = validate(cfg, n_samples);
     if (st != IIRX_OK) return st;
     std::vector<float> t;
-    int str = 0;
-    switch (cfg->kernel) {
-    case IIRX_KERNEL_PH: str = ph_tables(sos, cfg->n_sections, 64, cfg->n_blocks, t); break;
-    case IIRX_KERNEL_STCR:
-    case IIRX_KERNEL_DTCR:
-        str = cr_tables(sos, cfg->n_sections, 64, cfg->n_blocks, t);
-        if (str < 0) return IIRX_ERR_CONDITIONING;
-        break;
-    default: return IIRX_ERR_INVALID;
-    }
+    Shape sh;
+    st = make_shape(cfg, sos, n_samples, nullptr, nullptr, sh, t);
+    if (st != IIRX_OK) return st;
     if (size) *size = (int64_t)t.size();
-    if (stride) *stride = str;
+    if (stride) *stride = sh.coef_stride;
     if (out) std::memcpy(out, t.data(), sizeof(float) * (size_t)std::min<int64_t>(cap, (int64_t)t.size()));
     return IIRX_OK;
 }
 
-int iirx_plan_set_plr_taps(iirx_plan *plan, const double *b, const double *a, int k)
+int iirx_plan_set_plr_taps(iirx_plan *p, const double *b, const double *a, int k)
 {
-    (void)plan; (void)b; (void)a; (void)k;
-    return IIRX_ERR_INVALID;
+    if (!p || !b || !a) return IIRX_ERR_INVALID;
+    if (p->cfg.kernel != IIRX_KERNEL_PLR || k != 2 * p->cfg.n_sections) return IIRX_ERR_INVALID;
+    for (int j = 0; j < k; ++j)
+        if (!std::isfinite(b[j]) || !std::isfinite(a[j])) return IIRX_ERR_INVALID;
+    std::vector<float> t;
+    Shape sh;
+    const int st = make_shape(&p->cfg, p->sos.data(), p->n_samples, b, a, sh, t);
+    if (st != IIRX_OK) return st;
+    if (hipDeviceSynchronize() != hipSuccess) return IIRX_ERR_HIP;  // no launch may read the old tables
+    if (hipMemcpy(p->d_coef, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return IIRX_ERR_HIP;
+    p->host_coef.swap(t);
+    p->plr_b.assign(b, b + k);
+    p->plr_a.assign(a, a + k);
+    return IIRX_OK;
 }
 
 static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
@@ -243,7 +330,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
     a.spin_limit = p->spin_limit;
     a.persistent = p->cfg.schedule == IIRX_SCHEDULE_PERSISTENT;
     a.grid = p->grid;
-    a.ns = p->cfg.n_sections;
+    a.ns = (p->cfg.kernel == IIRX_KERNEL_PLR) ? 2 * p->cfg.n_sections : p->cfg.n_sections;
     a.coef_stride = p->coef_stride;
     a.unrolled = p->cfg.unrolled;
     hipError_t e = hipErrorInvalidValue;
@@ -251,6 +338,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
     case IIRX_KERNEL_PH: e = launch_ph(p->lanes, p->n_blocks, a, s); break;
     case IIRX_KERNEL_STCR: e = launch_stcr(p->lanes, p->n_blocks, a, s); break;
     case IIRX_KERNEL_DTCR: e = launch_dtcr(p->lanes, p->n_blocks, a, s); break;
+    case IIRX_KERNEL_PLR: e = launch_plr(2 * p->cfg.n_sections, p->shape.plr_block, p->shape.plr_x, a, s); break;
     default: break;
     }
     if (e != hipSuccess) return IIRX_ERR_HIP;

@@ -117,3 +117,35 @@ hipError_t launch_dtcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t
 const void *kernel_dtcr(int lanes, int n_blocks, int unrolled);
 
 }  // namespace iirx
+
+namespace iirx {
+
+// ---------------------------------------------------------------------------
+// PLR tables (one block; K = direct-form order, CH = chunk samples).  The
+// n-nacci factors of plr_kernels.cuh:149-409 in FP64, rounded to float32:
+// fac[m][i] = output i of y[n] = sum_l a[l] y[n-1-l] started from the unit
+// history e_m (slot m = y[-K+m]).
+//   [kB]     b[0..K)  FIR taps past b0 == 1 (float32 of the direct-form taps)
+//   [kF]     F[j][m] = fac[m][CH-K+j]: chunk transition of the K-sample state
+//   [kDepth] lookback depth (||F^d||_inf <= 2^-36, 0 = unbounded)
+//   [kFac]   fac[m][i], i < CH
+// ---------------------------------------------------------------------------
+struct PlrLayout {
+    int kB, kF, kDepth, kFac, stride;
+};
+
+constexpr PlrLayout plr_layout(int K, int CH)
+{
+    PlrLayout p{};
+    p.kB = 0;
+    p.kF = 16;
+    p.kDepth = p.kF + K * K;
+    p.kFac = align4_c(p.kDepth + 4);
+    p.stride = align4_c(p.kFac + K * CH);
+    return p;
+}
+
+hipError_t launch_plr(int k, int block, int x, const LaunchArgs &a, hipStream_t s);
+const void *kernel_plr(int k, int block, int x);
+
+}  // namespace iirx

@@ -1,0 +1,276 @@
+// plr.hip -- PLR (Maleki & Burtscher's direct-form hierarchical merge) for
+// gfx950.
+//
+// Algorithm of include/plr_kernels.cuh (PLR_2/4/8/16, 483-3108): the whole
+// cascade is one direct-form recurrence of order K = 2 N_SECTIONS.  A chunk
+// of BLOCK threads x X values, thread t owning samples t + v BLOCK
+// (plr:524-537, tail-safe), goes through
+//   1. the K-tap FIR (plr:539-562);
+//   2. a Sklansky merge over the BLOCK lanes of each segment v: at half width
+//      h, lane i of every upper half adds fac[m][i % h] times the last
+//      min(K, h) values of the lower half, m ascending (plr:564-785);
+//   3. the serial chain over the X segments (plr:787-811);
+//   4. the lookback for the K-value chunk carry and its injection
+//      val += sum_m fac[m][t + v BLOCK] X_m (plr:813-899).
+//
+// CDNA4 mapping.  Merge levels h <= 32 stay inside a wave64 (lane shuffles);
+// levels h >= 64 cross waves through LDS, as do the segment chain and the
+// carry broadcast.  The reference's intra-warp levels stop at 32 lanes and
+// cross warps from h = 32 on; the arithmetic rule is the same.  The lookback
+// scans predecessors one chunk at a time with the K words of a record on K
+// lanes and folds partials with the FP64-precomputed chunk transition F.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace iirx {
+
+template <int K, int B, int X>
+__global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
+{
+    constexpr int W = B / 64;
+    constexpr int CH = B * X;
+    constexpr PlrLayout Y = plr_layout(K, CH);
+    __shared__ float xs[K + CH];         // raw chunk with K samples of history
+    __shared__ float sp[W * X * K];      // last K values of every wave and segment
+    __shared__ float sv[K];              // segment-chain and carry broadcast
+    __shared__ float lp[64 * K];         // lookback partials, nearest first
+
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const float *tab = a.coef;
+    const cfloat *ctab = as_const(a.coef);
+    const float *fac = tab + Y.kFac;
+
+    for_each_chunk(a.ticket, a.epoch, a.num_chunks, a.persistent,
+                   [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
+    const long cs = (long)chunk * CH;
+    __syncthreads();
+    float val[X];
+#pragma unroll
+    for (int v = 0; v < X; ++v) {
+        const long idx = cs + t + (long)v * B;
+        val[v] = (idx < a.n_samples) ? a.in[idx] : 0.0f;
+        xs[K + t + v * B] = val[v];
+    }
+    if (t < K) {
+        const long hi = cs - K + t;
+        xs[t] = (hi >= 0) ? a.in[hi] : 0.0f;
+    }
+    __syncthreads();
+
+    // 1. FIR: val += b[j] x[i-1-j], j ascending
+#pragma unroll
+    for (int v = 0; v < X; ++v) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) val[v] = fmaf(ctab[Y.kB + j], xs[K + t + v * B - 1 - j], val[v]);
+    }
+
+    // 2a. merge levels inside the wave
+#pragma unroll
+    for (int h = 1; h < 64 && h < B; h <<= 1) {
+        const bool up = (lane & h) != 0;
+        const int base = lane & ~(2 * h - 1);
+        const int i = lane & (h - 1);
+#pragma unroll
+        for (int m = (K > h ? K - h : 0); m < K; ++m) {
+            const float f = fac[m * CH + i];
+            const int src = base + h - K + m;
+#pragma unroll
+            for (int v = 0; v < X; ++v) {
+                const float s = __shfl(val[v], src, 64);
+                if (up) val[v] = fmaf(f, s, val[v]);
+            }
+        }
+    }
+
+    // 2b. merge levels across waves (h >= 64 >= K: all K terms)
+#pragma unroll
+    for (int h = 64; h < B; h <<= 1) {
+        __syncthreads();
+        if (lane >= 64 - K) {
+#pragma unroll
+            for (int v = 0; v < X; ++v) sp[(w * X + v) * K + lane - (64 - K)] = val[v];
+        }
+        __syncthreads();
+        if (t & h) {
+            const int sw = ((t & ~(2 * h - 1)) + h - 1) >> 6;
+            const int i = t & (h - 1);
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                const float f = fac[m * CH + i];
+#pragma unroll
+                for (int v = 0; v < X; ++v) val[v] = fmaf(f, sp[(sw * X + v) * K + m], val[v]);
+            }
+        }
+    }
+
+    // 3. serial chain over the segments
+#pragma unroll
+    for (int v = 1; v < X; ++v) {
+        __syncthreads();
+        if (t >= B - K) sv[t - (B - K)] = val[v - 1];
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < K; ++m) val[v] = fmaf(fac[m * CH + t], sv[m], val[v]);
+    }
+
+    // 4. chunk partial = last K values; lookback by wave 0
+    uint64_t *rec = a.lb + (size_t)chunk * (2 * K);
+    if (t >= B - K) rec_store(rec + (t - (B - K)), part_flag, val[X - 1]);
+    __syncthreads();  // everyone is done with sv
+    if (w == 0) {
+        const int depth = (int)ctab[Y.kDepth];
+        const int maxd = depth > 0 ? depth : 64;
+        float Xj = 0.0f;
+        int D = 0;
+        bool done = false;
+        uint32_t spins = 0;
+        for (int d = 0; d < maxd && !done; ++d) {
+            const long pred = (long)chunk - 1 - d;
+            if (pred < 0) {
+                D = d;
+                done = true;  // signal start: zero state
+                break;
+            }
+            uint64_t *pr = a.lb + (size_t)pred * (2 * K);
+            for (;;) {
+                // full record: every one of its K words carries the full flag
+                const uint64_t f0 = (lane < K) ? rec_load(pr + K + lane) : 0ull;
+                if (__ballot(lane < K && rec_flag(f0) != full_flag) == 0) {
+                    Xj = rec_val(f0);
+                    D = d;
+                    done = true;
+                    break;
+                }
+                // otherwise the K partial words (never mixed with full words);
+                // the last slot of an unbounded fold must be a full record
+                const uint64_t p0 = (lane < K) ? rec_load(pr + lane) : 0ull;
+                if (__ballot(lane < K && rec_flag(p0) != part_flag) == 0 && !(depth == 0 && d == maxd - 1)) {
+                    if (lane < K) lp[d * K + lane] = rec_val(p0);
+                    break;
+                }
+                if (++spins > a.spin_limit) {
+                    if (lane == 0) __hip_atomic_store(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    D = d;
+                    Xj = 0.0f;
+                    done = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (!done) D = d + 1;  // truncated fold reaches this far
+        }
+        // Horner fold, farthest partial first: X <- F X + p_d
+        float Frow[K];
+#pragma unroll
+        for (int m = 0; m < K; ++m) Frow[m] = (lane < K) ? tab[Y.kF + lane * K + m] : 0.0f;
+        for (int d = D - 1; d >= 0; --d) {
+            float acc = (lane < K) ? lp[d * K + lane] : 0.0f;
+#pragma unroll
+            for (int m = 0; m < K; ++m) acc = fmaf(Frow[m], readlane_f(Xj, m), acc);
+            Xj = acc;
+        }
+        if (lane < K) sv[lane] = Xj;
+    }
+    __syncthreads();
+
+    // carry injection (plr:891-899)
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        const float Xm = sv[m];
+#pragma unroll
+        for (int v = 0; v < X; ++v) val[v] = fmaf(fac[m * CH + t + v * B], Xm, val[v]);
+    }
+    if (t >= B - K) rec_store(rec + K + (t - (B - K)), full_flag, val[X - 1]);
+
+#pragma unroll
+    for (int v = 0; v < X; ++v) {
+        const long idx = cs + t + (long)v * B;
+        if (idx < a.n_samples) a.out[idx] = val[v];
+    }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// instance table: K in {2, 4, 8, 16}, BLOCK in {64, 128, 256, 512},
+// X in {1, 2, 4, 8, 16}
+// ---------------------------------------------------------------------------
+template <int K, int B, int X>
+static hipError_t launch_kbx(const LaunchArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((plr_kernel<K, B, X>), dim3(a.grid), dim3(B), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int K, int B>
+static const void *fn_kb(int x)
+{
+    switch (x) {
+    case 1: return (const void *)plr_kernel<K, B, 1>;
+    case 2: return (const void *)plr_kernel<K, B, 2>;
+    case 4: return (const void *)plr_kernel<K, B, 4>;
+    case 8: return (const void *)plr_kernel<K, B, 8>;
+    case 16: return (const void *)plr_kernel<K, B, 16>;
+    default: return nullptr;
+    }
+}
+
+template <int K, int B>
+static hipError_t launch_kb(int x, const LaunchArgs &a, hipStream_t s)
+{
+    switch (x) {
+    case 1: return launch_kbx<K, B, 1>(a, s);
+    case 2: return launch_kbx<K, B, 2>(a, s);
+    case 4: return launch_kbx<K, B, 4>(a, s);
+    case 8: return launch_kbx<K, B, 8>(a, s);
+    case 16: return launch_kbx<K, B, 16>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int K>
+static hipError_t launch_k(int block, int x, const LaunchArgs &a, hipStream_t s)
+{
+    switch (block) {
+    case 64: return launch_kb<K, 64>(x, a, s);
+    case 128: return launch_kb<K, 128>(x, a, s);
+    case 256: return launch_kb<K, 256>(x, a, s);
+    case 512: return launch_kb<K, 512>(x, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int K>
+static const void *fn_k(int block, int x)
+{
+    switch (block) {
+    case 64: return fn_kb<K, 64>(x);
+    case 128: return fn_kb<K, 128>(x);
+    case 256: return fn_kb<K, 256>(x);
+    case 512: return fn_kb<K, 512>(x);
+    default: return nullptr;
+    }
+}
+
+hipError_t launch_plr(int k, int block, int x, const LaunchArgs &a, hipStream_t s)
+{
+    switch (k) {
+    case 2: return launch_k<2>(block, x, a, s);
+    case 4: return launch_k<4>(block, x, a, s);
+    case 8: return launch_k<8>(block, x, a, s);
+    case 16: return launch_k<16>(block, x, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+const void *kernel_plr(int k, int block, int x)
+{
+    switch (k) {
+    case 2: return fn_k<2>(block, x);
+    case 4: return fn_k<4>(block, x);
+    case 8: return fn_k<8>(block, x);
+    case 16: return fn_k<16>(block, x);
+    default: return nullptr;
+    }
+}
+
+}  // namespace iirx

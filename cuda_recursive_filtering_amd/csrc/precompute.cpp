@@ -175,3 +175,57 @@ int cr_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &ou
 }
 
 }  // namespace iirx
+
+namespace iirx {
+
+int plr_tables(const double *b, const double *a, int K, int CH, std::vector<float> &out)
+{
+    const PlrLayout Y = plr_layout(K, CH);
+    out.assign((size_t)Y.stride, 0.0f);
+    std::vector<double> af((size_t)K);
+    for (int j = 0; j < K; ++j) {
+        out[Y.kB + j] = (float)b[j];
+        af[j] = (double)(float)a[j];
+    }
+    // n-nacci factors (plr_kernels.cuh:149-409): fac[m][i], history slot m = y[-K+m]
+    std::vector<double> v((size_t)(CH + K));
+    std::vector<double> F((size_t)K * K);
+    for (int m = 0; m < K; ++m) {
+        std::fill(v.begin(), v.end(), 0.0);
+        v[m] = 1.0;
+        for (int i = K; i < CH + K; ++i) {
+            double acc = 0.0;
+            for (int l = 0; l < K; ++l) acc += af[l] * v[i - 1 - l];
+            v[i] = acc;
+        }
+        for (int i = 0; i < CH; ++i) out[(size_t)Y.kFac + (size_t)m * CH + i] = (float)v[i + K];
+        for (int j = 0; j < K; ++j) F[(size_t)j * K + m] = v[CH + j];  // fac[m][CH-K+j]
+    }
+    for (int j = 0; j < K * K; ++j) out[Y.kF + j] = (float)F[j];
+    // depth: smallest d with ||F^d||_inf <= 2^-36 (0 = unbounded)
+    std::vector<double> P(F), T((size_t)K * K);
+    int depth = 0;
+    for (int d = 1; d <= 64; ++d) {
+        double nrm = 0.0;
+        for (int r = 0; r < K; ++r) {
+            double s = 0.0;
+            for (int c = 0; c < K; ++c) s += std::fabs(P[(size_t)r * K + c]);
+            nrm = std::max(nrm, s);
+        }
+        if (!(nrm > kLookbackTail)) {
+            depth = std::isfinite(nrm) ? d : 0;
+            break;
+        }
+        for (int r = 0; r < K; ++r)
+            for (int c = 0; c < K; ++c) {
+                double s = 0.0;
+                for (int q = 0; q < K; ++q) s += P[(size_t)r * K + q] * F[(size_t)q * K + c];
+                T[(size_t)r * K + c] = s;
+            }
+        P.swap(T);
+    }
+    out[Y.kDepth] = (float)depth;
+    return Y.stride;
+}
+
+}  // namespace iirx

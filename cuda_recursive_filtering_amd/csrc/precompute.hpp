@@ -40,4 +40,10 @@ int ph_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &ou
 constexpr double kCrLimit = 1.0e4;
 int cr_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &out, double *worst = nullptr);
 
+// PLR tables (layout kernels.hpp:plr_layout) from the direct-form taps
+// b[0..K), a[0..K) (recurrence sign); the kernel applies float32 taps, so the
+// FP64 factors are those of the float32-rounded feedback taps.  Returns the
+// table size in floats.
+int plr_tables(const double *b, const double *a, int K, int CH, std::vector<float> &out);
+
 }  // namespace iirx
