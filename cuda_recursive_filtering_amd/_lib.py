@@ -20,7 +20,7 @@ IIRX_ERR_TIMEOUT = 7
 
 SCHEDULES = {"persistent": 0, "ticket": 1}
 
-KERNELS = {"PH": 0, "STCR": 1, "DTCR": 2, "PLR": 3}
+KERNELS = {"PH": 0, "STCR": 1, "DTCR": 2, "PLR": 3, "BS": 4}
 
 
 class Config(ctypes.Structure):

@@ -254,6 +254,70 @@ __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int se
     return X;
 }
 
+// Lookback over K-word records ([K partial words][K full words] per chunk,
+// `rec_stride` words apart), for carries of K floats (PLR's direct-form
+// history, the block-state form's state vector).  Predecessors are scanned
+// one at a time with word j on lane j; a record is used whole, all-full or
+// all-partial, never mixed.  The fold X <- F X + p runs farthest first, with
+// row j of the chunk transition F (K x K, row-major in `F`) on lane j.
+// `lp` is LDS scratch of 64*K floats.  Returns X_j on lane j < K (0 elsewhere).
+template <int K>
+__device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint32_t chunk, uint32_t part_flag,
+                                          uint32_t full_flag, const float *F, int depth, float *lp,
+                                          uint32_t spin_limit, uint32_t *err)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int maxd = depth > 0 ? depth : kWave;
+    float Xj = 0.0f;
+    int D = 0;
+    bool done = false;
+    uint32_t spins = 0;
+    for (int d = 0; d < maxd && !done; ++d) {
+        const long pred = (long)chunk - 1 - d;
+        if (pred < 0) {
+            D = d;  // signal start: zero state
+            done = true;
+            break;
+        }
+        uint64_t *pr = lb + (size_t)pred * rec_stride;
+        for (;;) {
+            const uint64_t f0 = (lane < K) ? rec_load(pr + K + lane) : 0ull;
+            if (__ballot(lane < K && rec_flag(f0) != full_flag) == 0) {
+                Xj = rec_val(f0);
+                D = d;
+                done = true;
+                break;
+            }
+            // the last slot of an unbounded fold must be a full record
+            const uint64_t p0 = (lane < K) ? rec_load(pr + lane) : 0ull;
+            if (__ballot(lane < K && rec_flag(p0) != part_flag) == 0 && !(depth == 0 && d == maxd - 1)) {
+                if (lane < K) lp[d * K + lane] = rec_val(p0);
+                break;
+            }
+            if (++spins > spin_limit) {
+                if (lane == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                D = d;
+                Xj = 0.0f;
+                done = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (!done) D = d + 1;
+    }
+    float Frow[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) Frow[m] = (lane < K) ? F[lane * K + m] : 0.0f;
+    if (lane >= K) Xj = 0.0f;
+    for (int d = D - 1; d >= 0; --d) {
+        float acc = (lane < K) ? lp[d * K + lane] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < K; ++m) acc = fmaf(Frow[m], readlane_f(Xj, m), acc);
+        Xj = acc;
+    }
+    return Xj;
+}
+
 // ------------------------------------------------------------------------
 // chunk driver: runs body(chunk, part_flag, full_flag) for every chunk this
 // workgroup owns under the plan's schedule (see the protocol notes above)

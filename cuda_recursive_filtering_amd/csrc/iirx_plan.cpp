@@ -80,6 +80,13 @@ int validate(const iirx_config *cfg, int64_t n)
         if (n / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
         return IIRX_OK;
     }
+    case IIRX_KERNEL_BLOCKSTATE: {
+        const int L = cfg->n_blocks;
+        if (!(L == 32 || L == 64)) return IIRX_ERR_GEOMETRY;
+        const int64_t chunk = 64LL * L;
+        if (n % chunk != 0 || n / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
+        return IIRX_OK;
+    }
     case IIRX_KERNEL_PLR: {
         const int ns = cfg->n_sections;
         if (!(ns == 1 || ns == 2 || ns == 4 || ns == 8)) return IIRX_ERR_GEOMETRY;
@@ -133,6 +140,12 @@ int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double
         sh.coef_stride = cr_tables(sos, ns, 64, cfg->n_blocks, tables);
         if (sh.coef_stride < 0) return IIRX_ERR_CONDITIONING;
         break;
+    case IIRX_KERNEL_BLOCKSTATE:
+        sh.coef_stride = bs_tables(sos, ns, cfg->n_blocks, tables);
+        sh.chunk = (int64_t)kBsTiles * cfg->n_blocks;
+        sh.num_chunks = (uint32_t)(n / sh.chunk);
+        sh.lb_count = (size_t)sh.num_chunks * 32u;  // 16 partial + 16 full state words
+        return IIRX_OK;
     case IIRX_KERNEL_PLR: {
         const int k = 2 * ns;
         plr_geometry(cfg, n, &sh.plr_block, &sh.plr_x);
@@ -180,6 +193,7 @@ const void *kernel_fn(const iirx_config *cfg, const Shape &sh)
     case IIRX_KERNEL_STCR: return kernel_stcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_DTCR: return kernel_dtcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_PLR: return kernel_plr(2 * cfg->n_sections, sh.plr_block, sh.plr_x);
+    case IIRX_KERNEL_BLOCKSTATE: return kernel_bs(cfg->n_blocks);
     default: return nullptr;
     }
 }
@@ -339,6 +353,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
     case IIRX_KERNEL_STCR: e = launch_stcr(p->lanes, p->n_blocks, a, s); break;
     case IIRX_KERNEL_DTCR: e = launch_dtcr(p->lanes, p->n_blocks, a, s); break;
     case IIRX_KERNEL_PLR: e = launch_plr(2 * p->cfg.n_sections, p->shape.plr_block, p->shape.plr_x, a, s); break;
+    case IIRX_KERNEL_BLOCKSTATE: e = launch_bs(p->n_blocks, a, s); break;
     default: break;
     }
     if (e != hipSuccess) return IIRX_ERR_HIP;

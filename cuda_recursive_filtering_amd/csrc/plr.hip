@@ -17,8 +17,8 @@
 // levels h >= 64 cross waves through LDS, as do the segment chain and the
 // carry broadcast.  The reference's intra-warp levels stop at 32 lanes and
 // cross warps from h = 32 on; the arithmetic rule is the same.  The lookback
-// scans predecessors one chunk at a time with the K words of a record on K
-// lanes and folds partials with the FP64-precomputed chunk transition F.
+// (lookbackK) scans predecessors one chunk at a time with the K words of a
+// record on K lanes and folds partials with the FP64 chunk transition F.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -118,57 +118,8 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
     if (t >= B - K) rec_store(rec + (t - (B - K)), part_flag, val[X - 1]);
     __syncthreads();  // everyone is done with sv
     if (w == 0) {
-        const int depth = (int)ctab[Y.kDepth];
-        const int maxd = depth > 0 ? depth : 64;
-        float Xj = 0.0f;
-        int D = 0;
-        bool done = false;
-        uint32_t spins = 0;
-        for (int d = 0; d < maxd && !done; ++d) {
-            const long pred = (long)chunk - 1 - d;
-            if (pred < 0) {
-                D = d;
-                done = true;  // signal start: zero state
-                break;
-            }
-            uint64_t *pr = a.lb + (size_t)pred * (2 * K);
-            for (;;) {
-                // full record: every one of its K words carries the full flag
-                const uint64_t f0 = (lane < K) ? rec_load(pr + K + lane) : 0ull;
-                if (__ballot(lane < K && rec_flag(f0) != full_flag) == 0) {
-                    Xj = rec_val(f0);
-                    D = d;
-                    done = true;
-                    break;
-                }
-                // otherwise the K partial words (never mixed with full words);
-                // the last slot of an unbounded fold must be a full record
-                const uint64_t p0 = (lane < K) ? rec_load(pr + lane) : 0ull;
-                if (__ballot(lane < K && rec_flag(p0) != part_flag) == 0 && !(depth == 0 && d == maxd - 1)) {
-                    if (lane < K) lp[d * K + lane] = rec_val(p0);
-                    break;
-                }
-                if (++spins > a.spin_limit) {
-                    if (lane == 0) __hip_atomic_store(a.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    D = d;
-                    Xj = 0.0f;
-                    done = true;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            if (!done) D = d + 1;  // truncated fold reaches this far
-        }
-        // Horner fold, farthest partial first: X <- F X + p_d
-        float Frow[K];
-#pragma unroll
-        for (int m = 0; m < K; ++m) Frow[m] = (lane < K) ? tab[Y.kF + lane * K + m] : 0.0f;
-        for (int d = D - 1; d >= 0; --d) {
-            float acc = (lane < K) ? lp[d * K + lane] : 0.0f;
-#pragma unroll
-            for (int m = 0; m < K; ++m) acc = fmaf(Frow[m], readlane_f(Xj, m), acc);
-            Xj = acc;
-        }
+        const float Xj = lookbackK<K>(a.lb, 2 * K, chunk, part_flag, full_flag, tab + Y.kF, (int)ctab[Y.kDepth], lp,
+                                      a.spin_limit, a.err);
         if (lane < K) sv[lane] = Xj;
     }
     __syncthreads();

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 
 namespace iirx {
 
@@ -223,6 +225,208 @@ int plr_tables(const double *b, const double *a, int K, int CH, std::vector<floa
                 T[(size_t)r * K + c] = s;
             }
         P.swap(T);
+    }
+    out[Y.kDepth] = (float)depth;
+    return Y.stride;
+}
+
+}  // namespace iirx
+
+namespace iirx {
+
+namespace {
+
+using Mat = std::vector<double>;  // 16 x 16 row-major
+
+Mat mat_mul16(const Mat &a, const Mat &b)
+{
+    Mat c(256, 0.0);
+    for (int i = 0; i < 16; ++i)
+        for (int k = 0; k < 16; ++k) {
+            const double v = a[i * 16 + k];
+            if (v == 0.0) continue;
+            for (int j = 0; j < 16; ++j) c[i * 16 + j] += v * b[k * 16 + j];
+        }
+    return c;
+}
+
+// scale exponent s so that max|m| * 2^s < 2^14 (fp16 holds up to 65504)
+int pow2_scale(const double *m, size_t n)
+{
+    double mx = 0.0;
+    for (size_t i = 0; i < n; ++i) mx = std::max(mx, std::fabs(m[i]));
+    if (!(mx > 0.0) || !std::isfinite(mx)) return 0;
+    int e;
+    std::frexp(mx, &e);  // mx = f * 2^e, f in [0.5, 1)
+    return 14 - e;
+}
+
+uint16_t half_bits(_Float16 h)
+{
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+
+// 16x16 operand block (rows r0.., cols c0..) of matrix m (rows x cols, row
+// major, zero outside) in MFMA A-operand lane order, hi (hl = 0) or lo (1)
+void put_block(float *dst, const double *m, int rows, int cols, int r0, int c0, int s, int hl)
+{
+    uint16_t *h = reinterpret_cast<uint16_t *>(dst);
+    for (int l = 0; l < 64; ++l)
+        for (int j = 0; j < 4; ++j) {
+            const int r = r0 + (l % 16), c = c0 + 4 * (l / 16) + j;
+            const double v = (r < rows && c < cols) ? std::ldexp(m[(size_t)r * cols + c], s) : 0.0;
+            const _Float16 hi = (_Float16)(float)v;
+            const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
+            h[l * 4 + j] = half_bits(hl == 0 ? hi : lo);
+        }
+}
+
+}  // namespace
+
+int bs_tables(const float *sos, int ns, int L, std::vector<float> &out)
+{
+    const BsLayout Y = bs_layout(L);
+    out.assign((size_t)Y.stride, 0.0f);
+    const int p = 2 * ns;
+    // cascade of direct-form-II-transposed sections: z_s' = A_s z_s + B_s u_s,
+    // u_s = u_0 + sum_{r<s} z_r[0], y = u_0 + sum_r z_r[0]
+    Mat A(256, 0.0);
+    std::vector<double> B(16, 0.0), C(16, 0.0);
+    for (int s = 0; s < ns; ++s) {
+        const float *r = sos + 6 * s;
+        const double b1 = r[1], b2 = r[2], a1 = r[4], a2 = r[5];  // scipy sign
+        const double B0 = b1 - a1, B1 = b2 - a2;
+        A[(2 * s) * 16 + 2 * s] = -a1;
+        A[(2 * s) * 16 + 2 * s + 1] = 1.0;
+        A[(2 * s + 1) * 16 + 2 * s] = -a2;
+        for (int q = 0; q < s; ++q) {
+            A[(2 * s) * 16 + 2 * q] += B0;
+            A[(2 * s + 1) * 16 + 2 * q] += B1;
+        }
+        B[2 * s] = B0;
+        B[2 * s + 1] = B1;
+        C[2 * s] = 1.0;
+    }
+    // state balancing: scale state component c by d_c = sqrt(W_cc), W the
+    // controllability Gramian sum_k A^k B B^T A^kT (doubling, FP64), so that
+    // every state has the scale of the response to unit white input.  Without
+    // it the chained sections' states span ~14 decades at order 16 and the
+    // FP16 operands lose the small ones.
+    {
+        Mat W(256, 0.0), Ak(A);
+        for (int r = 0; r < 16; ++r)
+            for (int c = 0; c < 16; ++c) W[r * 16 + c] = B[r] * B[c];
+        for (int it = 0; it < 48; ++it) {
+            Mat AW = mat_mul16(Ak, W);
+            Mat AkT(256);
+            for (int r = 0; r < 16; ++r)
+                for (int c = 0; c < 16; ++c) AkT[r * 16 + c] = Ak[c * 16 + r];
+            const Mat add = mat_mul16(AW, AkT);
+            double delta = 0.0, size = 0.0;
+            for (int i = 0; i < 256; ++i) {
+                W[i] += add[i];
+                delta = std::max(delta, std::fabs(add[i]));
+                size = std::max(size, std::fabs(W[i]));
+            }
+            Ak = mat_mul16(Ak, Ak);
+            if (!(delta > 1e-18 * size)) break;
+        }
+        double dsc[16];
+        for (int c = 0; c < 16; ++c) {
+            const double w = W[c * 16 + c];
+            dsc[c] = (c < p && w > 0.0 && std::isfinite(w)) ? std::sqrt(w) : 1.0;
+        }
+        for (int r = 0; r < 16; ++r)
+            for (int c = 0; c < 16; ++c) A[r * 16 + c] *= dsc[c] / dsc[r];
+        for (int c = 0; c < 16; ++c) {
+            B[c] /= dsc[c];
+            C[c] *= dsc[c];
+        }
+    }
+    // impulse response, observability, controllability
+    std::vector<double> h((size_t)L), Om((size_t)L * 16), Gm((size_t)16 * L), Tm((size_t)L * L, 0.0);
+    {
+        std::vector<double> cA(C);  // C A^i
+        std::vector<double> AB(B);  // A^i B
+        std::vector<std::vector<double>> powB((size_t)L);
+        for (int i = 0; i < L; ++i) {
+            for (int c = 0; c < 16; ++c) Om[(size_t)i * 16 + c] = cA[c];
+            powB[i] = AB;
+            double hv = 0.0;
+            for (int c = 0; c < 16; ++c) hv += C[c] * AB[c];
+            if (i + 1 < L) h[i + 1] = hv;  // h[m] = C A^(m-1) B
+            std::vector<double> n1(16, 0.0), n2(16, 0.0);
+            for (int r = 0; r < 16; ++r)
+                for (int c = 0; c < 16; ++c) {
+                    n1[c] += cA[r] * A[r * 16 + c];
+                    n2[r] += A[r * 16 + c] * AB[c];
+                }
+            cA.swap(n1);
+            AB.swap(n2);
+        }
+        h[0] = 1.0;
+        for (int j = 0; j < L; ++j)
+            for (int c = 0; c < 16; ++c) Gm[(size_t)c * L + j] = powB[L - 1 - j][c];
+        for (int i = 0; i < L; ++i)
+            for (int j = 0; j <= i; ++j) Tm[(size_t)i * L + j] = h[i - j];
+    }
+    // Phi = A^L and its powers
+    Mat Phi(256, 0.0);
+    for (int i = 0; i < 16; ++i) Phi[i * 16 + i] = 1.0;
+    for (int i = 0; i < L; ++i) Phi = mat_mul16(Phi, A);
+    std::vector<Mat> P(6);
+    P[0] = Phi;
+    for (int j = 1; j < 6; ++j) P[j] = mat_mul16(P[j - 1], P[j - 1]);
+    Mat Phi16 = mat_mul16(P[3], P[3]);  // Phi^16
+    std::vector<Mat> Q(4);
+    Q[1] = Phi16;
+    Q[2] = mat_mul16(Phi16, Phi16);
+    Q[3] = mat_mul16(Q[2], Phi16);
+    const Mat Mc = mat_mul16(P[5], P[5]);  // Phi^64
+
+    float *sc = out.data() + Y.kScale;
+    const int sT = pow2_scale(Tm.data(), Tm.size()), sO = pow2_scale(Om.data(), Om.size());
+    const int sG = pow2_scale(Gm.data(), Gm.size());
+    sc[0] = (float)sT;
+    sc[1] = (float)sO;
+    sc[2] = (float)sG;
+    for (int hl = 0; hl < 2; ++hl) {
+        for (int mb = 0; mb < Y.MB; ++mb)
+            for (int kb = 0; kb < Y.MB; ++kb) put_block(out.data() + bs_T(Y, hl, mb, kb), Tm.data(), L, L, 16 * mb, 16 * kb, sT, hl);
+        for (int mb = 0; mb < Y.MB; ++mb) put_block(out.data() + bs_O(Y, hl, mb), Om.data(), L, 16, 16 * mb, 0, sO, hl);
+        for (int kb = 0; kb < Y.MB; ++kb) put_block(out.data() + bs_G(Y, hl, kb), Gm.data(), 16, L, 0, 16 * kb, sG, hl);
+        for (int j = 0; j < 6; ++j) {
+            const int s = pow2_scale(P[j].data(), 256);
+            sc[3 + j] = (float)s;
+            put_block(out.data() + bs_P(Y, hl, j), P[j].data(), 16, 16, 0, 0, s, hl);
+        }
+        for (int nb = 1; nb < 4; ++nb) {
+            const int s = pow2_scale(Q[nb].data(), 256);
+            sc[9 + nb - 1] = (float)s;
+            put_block(out.data() + bs_Q(Y, hl, nb), Q[nb].data(), 16, 16, 0, 0, s, hl);
+        }
+    }
+    for (int i = 0; i < 256; ++i) {
+        out[Y.kPhi + i] = (float)Phi[i];
+        out[Y.kMc + i] = (float)Mc[i];
+    }
+    // lookback depth of the chunk transition
+    Mat Pw = Mc;
+    int depth = 0;
+    for (int d = 1; d <= 64; ++d) {
+        double nrm = 0.0;
+        for (int r = 0; r < 16; ++r) {
+            double s = 0.0;
+            for (int c = 0; c < 16; ++c) s += std::fabs(Pw[r * 16 + c]);
+            nrm = std::max(nrm, s);
+        }
+        if (!(nrm > kLookbackTail)) {
+            depth = std::isfinite(nrm) ? d : 0;
+            break;
+        }
+        Pw = mat_mul16(Pw, Mc);
     }
     out[Y.kDepth] = (float)depth;
     return Y.stride;

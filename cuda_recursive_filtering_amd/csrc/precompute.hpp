@@ -46,4 +46,7 @@ int cr_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &ou
 // table size in floats.
 int plr_tables(const double *b, const double *a, int K, int CH, std::vector<float> &out);
 
+// Block-state tables (layout kernels.hpp:bs_layout) for tiles of L samples.
+int bs_tables(const float *sos, int ns, int L, std::vector<float> &out);
+
 }  // namespace iirx

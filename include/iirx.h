@@ -64,7 +64,12 @@ typedef enum {
     IIRX_KERNEL_PH = 0,    /* partial-LU factorisation, ph_kernels.cuh */
     IIRX_KERNEL_STCR = 1,  /* single-stage tridiagonal CR, stcr_kernels.cuh */
     IIRX_KERNEL_DTCR = 2,  /* decoupled two-sided CR, dtcr_kernels.cuh */
-    IIRX_KERNEL_PLR = 3    /* direct-form hierarchical merge, plr_kernels.cuh */
+    IIRX_KERNEL_PLR = 3,   /* direct-form hierarchical merge, plr_kernels.cuh */
+    IIRX_KERNEL_BLOCKSTATE = 4  /* no counterpart in the reference: the whole
+                                   cascade as a state-space block filter on the
+                                   matrix cores (FP16 hi/lo operands, FP32
+                                   accumulation); n_blocks = tile length L
+                                   (32 or 64), chunk = 64 tiles */
 } iirx_kernel;
 
 /*

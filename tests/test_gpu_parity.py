@@ -273,3 +273,56 @@ def test_plr_default_taps_from_sos(device):
     sos, g, xs = normal_workload(2, n, seed=2, wn=0.1)
     y, _ = run_gpu(device, "PLR", sos, xs, 0, 64)
     assert O.gate(y, O.golden(sos, xs))[0]
+
+
+# ---------------------------------------------------------------------------
+# MFMA block-state form (no reference counterpart): same filter, FP16 hi/lo
+# operands with FP32 accumulation; the bar is the FP64 gate and an error
+# (max|err| / RMS of the output) of the same order as the plain fp32 kernels'
+# ---------------------------------------------------------------------------
+def err_rms(y, ref):
+    ref = np.asarray(ref, np.float64)
+    return float(np.max(np.abs(np.asarray(y, np.float64) - ref)) / np.sqrt(np.mean(ref ** 2)))
+
+
+@pytest.mark.parametrize("order", ORDERS)
+@pytest.mark.parametrize("L", [32, 64])
+@pytest.mark.parametrize("wn", [0.1, 0.02])
+def test_blockstate_parity(device, order, L, wn):
+    print(f"case BS order={order} L={L} wn={wn}", flush=True)
+    sos, g, x = normal_workload(order, 1 << 20, seed=3, wn=wn)
+    y, info = run_gpu(device, "BS", sos, x, 64, L)
+    ref = O.golden(sos, x)
+    ok, err, mref, rel = O.gate(y, ref)
+    yp, _ = run_gpu(device, "PH", sos, x, 64, 32)
+    e_bs, e_ph = err_rms(y, ref), err_rms(yp, ref)
+    print(f"  rel={rel:.2e} err/rms BS={e_bs:.2e} PH={e_ph:.2e} vgprs={info['vgprs']} scratch={info['scratch_bytes']}")
+    assert ok, rel
+    assert e_bs <= max(8.0 * e_ph, 1e-6), (e_bs, e_ph)
+
+
+@pytest.mark.parametrize("schedule", ["persistent", "ticket"])
+@pytest.mark.parametrize("n_chunks", [1, 3, 70, 3000])
+def test_blockstate_grids_and_launches(device, n_chunks, schedule):
+    import torch
+    L = 32
+    n = 64 * L * n_chunks
+    print(f"case BS grid chunks={n_chunks} {schedule}", flush=True)
+    sos, g, xa = normal_workload(8, n, seed=21, wn=0.1)
+    _, _, xb = normal_workload(8, n, seed=22, wn=0.1)
+    ra, rb = O.golden(sos, xa), O.golden(sos, xb)
+    ta, tb = torch.from_numpy(xa).to(device), torch.from_numpy(xb).to(device)
+    out = torch.empty_like(ta)
+    with iirx.Plan("BS", sos, n, n_blocks=L, schedule=schedule) as p:
+        for i in range(4):
+            src, ref = (ta, ra) if i % 2 == 0 else (tb, rb)
+            p.execute(src, out)
+            p.check()
+            assert O.gate(out.cpu().numpy(), ref)[0], i
+
+
+def test_blockstate_reference_workload(device):
+    for order in ORDERS:
+        sos, g, x = reference_workload(order, 1 << 20)
+        y, _ = run_gpu(device, "BS", sos, x, 64, 32)
+        assert O.gate(y, O.golden(sos, x))[0], order

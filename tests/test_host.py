@@ -123,3 +123,68 @@ def test_lookback_depth(wn, N, lo, hi):
 def test_config_check(kernel, B, N, n, ok):
     st = iirx.config_check(kernel, 2, n, block_size=B, n_blocks=N)
     assert (st == 0) == ok, st
+
+
+def _bs_tables(sos, L):
+    """Dequantised block-state tables (layout of kernels.hpp:bs_layout)."""
+    tab, stride = iirx.plan_tables("BS", sos, 64 * L * 4, n_blocks=L)
+    MB = L // 16
+    kT = 0
+    kO = kT + 2 * MB * MB * 128
+    kG = kO + 2 * MB * 128
+    kP = kG + 2 * MB * 128
+    kQ = kP + 12 * 128
+    kS = kQ + 6 * 128
+    kPhi = kS + 16
+    kMc = kPhi + 256
+    halves = tab.view(np.uint32).view(np.float16)  # 2 halves per word
+
+    def block(word_off):
+        h = halves[2 * word_off: 2 * word_off + 256].astype(np.float64).reshape(64, 4)
+        m = np.zeros((16, 16))
+        for l in range(64):
+            for j in range(4):
+                m[l % 16, 4 * (l // 16) + j] = h[l, j]
+        return m
+
+    def mat(kind, rows, cols, sidx, offs):
+        s = tab[kS + sidx]
+        m = np.zeros((rows, cols))
+        for (r0, c0, off) in offs:
+            hi, lo = block(off), block(off + offs_lo[kind])
+            m[r0:r0 + 16, c0:c0 + 16] = hi + lo
+        return m * 2.0 ** (-s)
+
+    offs_lo = {"T": MB * MB * 128, "O": MB * 128, "G": MB * 128, "P": 6 * 128, "Q": 3 * 128}
+    T = mat("T", L, L, 0, [(16 * mb, 16 * kb, kT + (mb * MB + kb) * 128) for mb in range(MB) for kb in range(MB)])
+    Om = mat("O", L, 16, 1, [(16 * mb, 0, kO + mb * 128) for mb in range(MB)])
+    Gm = mat("G", 16, L, 2, [(0, 16 * kb, kG + kb * 128) for kb in range(MB)])
+    P0 = mat("P", 16, 16, 3, [(0, 0, kP)])
+    Phi = tab[kPhi:kPhi + 256].reshape(16, 16).astype(np.float64)
+    Mc = tab[kMc:kMc + 256].reshape(16, 16).astype(np.float64)
+    return T, Om, Gm, P0, Phi, Mc, int(tab[kMc + 256])
+
+
+@pytest.mark.parametrize("order", [2, 8, 16])
+@pytest.mark.parametrize("L", [32, 64])
+def test_blockstate_tables_reproduce_the_cascade(order, L):
+    """The block-state tables (FP64 state space of the cascade, FP16 hi/lo
+    operands) define the same filter: running y_t = T x_t + O s_t,
+    s_{t+1} = Phi s_t + Gam x_t in float64 from the dequantised tables
+    matches the FP64 cascade to the operand precision (2^-22 relative)."""
+    from oracle_lib import golden
+    sos, g = iirx.butterworth_sos(order, 0.1)
+    T, Om, Gm, P0, Phi, Mc, depth = _bs_tables(sos, L)
+    assert np.allclose(P0, Phi, rtol=1e-6, atol=1e-6 * np.abs(Phi).max())
+    assert np.allclose(np.linalg.matrix_power(Phi, 64), Mc, rtol=1e-4, atol=1e-30 + 1e-6 * np.abs(Mc).max())
+    n = 64 * L * 3
+    x = (g * iirx.fill_normal(4, n).astype(np.float64)).astype(np.float32)
+    ref = golden(sos, x)
+    y = np.zeros(n)
+    s = np.zeros(16)
+    for t in range(n // L):
+        xt = x[t * L:(t + 1) * L].astype(np.float64)
+        y[t * L:(t + 1) * L] = T @ xt + Om @ s
+        s = Phi @ s + Gm @ xt
+    assert np.max(np.abs(y - ref)) <= 1e-5 * np.max(np.abs(ref))
+    assert depth >= 1
