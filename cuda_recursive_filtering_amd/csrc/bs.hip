@@ -2,44 +2,66 @@
 //
 // No counterpart in the reference; it computes the same filter as
 // ph/stcr/dtcr (the whole cascade, zero initial state) in a form whose cost
-// does not grow with the order.  The cascade is one state-space system of
-// dimension p = 2 N_SECTIONS (padded to 16).  A chunk of G = 64 tiles of L
-// samples is one wave64:
-//   1. Y0 = T X and V = Gam X (T: L x L Toeplitz of the impulse response,
-//      Gam: 16 x L), as v_mfma_f32_16x16x16_f16 products of FP16 hi/lo
-//      operands (hi*hi + hi*lo + lo*hi) with FP32 accumulation; every operand
-//      is scaled by a power of two first (per matrix for the constant ones,
-//      per chunk for the data) so that FP16 holds it;
+// grows with the order only through the state dimension of small products.
+// The cascade is one state-space system of dimension p = 2 N_SECTIONS,
+// embedded in 16 rows.  A chunk of G = 64 tiles of L samples is one wave64:
+//   1. the zero-state parts Y0 = T X (T: L x L lower-triangular Toeplitz of
+//      the impulse response) and V = Gam X (Gam: 16 x L), as
+//      v_mfma_f32_16x16x32_f16 products of FP16 hi/lo operands
+//      (hi*hi + hi*lo + lo*hi, FP32 accumulation); T and Gam are scaled by
+//      a power of two on the host, X by one per chunk, so that FP16 holds
+//      them; blocks of T above the diagonal are skipped;
 //   2. the tile end states Z_k = sum_{j<=k} Phi^(k-j) V_j by a 6-level
-//      Kogge-Stone scan over the 64 tiles, each level one 16x16x64 product
-//      Phi^(2^j) * shift(Z) on the matrix cores;
+//      Kogge-Stone scan over the 64 tiles, each level Phi^(2^j) * shift(Z)
+//      on v_mfma_f32_16x16x4f32 (plain FP32 operands: no splitting, no data
+//      scaling), over ceil(p/4) K slabs only;
 //   3. the chunk's incoming state X_in from the lookback (16-word records,
 //      chunk transition Phi^64, bounded depth), injected as Phi^(k+1) X_in:
-//      a 4-level scan inside one 16-tile block and three Phi^(16 nb) products;
-//   4. Y = Y0 + O S, S_k = state entering tile k.
-// The MFMA D layout (row 4(l/16)+j, column l%16) equals the B layout, so every
-// product feeds the next one from registers; tiles are MFMA columns and the
-// scan's shifts are lane shuffles inside 16-lane rows.
+//      Phi X_in in tile 0, a 4-level scan inside one 16-tile block and three
+//      Phi^(16 nb) products;
+//   4. Y = Y0 + O S, S_k = state entering tile k, accumulated onto Y0 by the
+//      FP32 matrix core.
+// Both MFMA shapes put D[4 (l/16) + j][l%16] on lane l, register j: tiles are
+// MFMA columns, states rows 4 g + j, so every product feeds the next from
+// registers and the scan's tile shifts are DPP row rotations.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace iirx {
 
-typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ f4 mfma(h4 a, h4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma_h(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma_f(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
-__device__ __forceinline__ h4 load_blk(const float *tab, int word_off)
+__device__ __forceinline__ h8 load_h8(const float *tab, int word_off)
 {
-    const uint2 u = reinterpret_cast<const uint2 *>(tab + word_off)[threadIdx.x & 63];
-    return __builtin_bit_cast(h4, u);
+    return __builtin_bit_cast(h8, reinterpret_cast<const uint4 *>(tab + word_off)[threadIdx.x & 63]);
+}
+
+__device__ __forceinline__ f4 load_f4(const float *tab, int word_off)
+{
+    return reinterpret_cast<const f4 *>(tab + word_off)[threadIdx.x & 63];
+}
+
+// DPP moves (v_mov_dpp, no LDS round trip): row_ror:D rotates every 16-lane
+// row, lane l <- lane (l - D) mod 16 of its row; row_bcast15/31 pass lane 15 /
+// 31 on to the next row / upper half
+template <int D>
+__device__ __forceinline__ float row_ror(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + D, 0xF, 0xF, false));
 }
 
 __device__ __forceinline__ float wave_max(float v)
 {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
+    v = fmaxf(v, row_ror<1>(v));
+    v = fmaxf(v, row_ror<2>(v));
+    v = fmaxf(v, row_ror<4>(v));
+    v = fmaxf(v, row_ror<8>(v));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x142, 0xF, 0xF, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x143, 0xF, 0xF, false)));
+    return readlane_f(v, 63);
 }
 
 // power-of-two exponent s with max * 2^s < 2^14
@@ -48,274 +70,282 @@ __device__ __forceinline__ int scale_exp(float mx)
     return (mx > 0.0f) ? 14 - __builtin_amdgcn_frexp_expf(mx) : 0;
 }
 
-// hi/lo FP16 split of v * 2^s
-__device__ __forceinline__ void split(const f4 &v, int s, h4 &hi, h4 &lo)
+// hi/lo FP16 split of (a, b) * 2^s, the 8 K rows of one x32 B operand
+__device__ __forceinline__ void split8(const f4 &a, const f4 &b, int s, h8 &hi, h8 &lo)
 {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const float x = ldexpf(v[j], s);
+    for (int j = 0; j < 8; ++j) {
+        const float x = ldexpf(j < 4 ? a[j & 3] : b[j & 3], s);
         const _Float16 h = (_Float16)x;
         hi[j] = h;
         lo[j] = (_Float16)(x - (float)h);
     }
 }
 
-// 3-term compensated product into acc: A (hi/lo) * B (hi/lo)
-__device__ __forceinline__ f4 mfma3(h4 ah, h4 al, h4 bh, h4 bl, f4 acc)
+// acc + M S over the first NSL K slabs; M is this lane's float4 of an FP32 block
+template <int NSL>
+__device__ __forceinline__ f4 apply(const f4 &M, const f4 &S, f4 acc)
 {
-    acc = mfma(al, bh, acc);
-    acc = mfma(ah, bl, acc);
-    acc = mfma(ah, bh, acc);
+#pragma unroll
+    for (int m = 0; m < NSL; ++m) acc = mfma_f(M[m], S[m], acc);
     return acc;
 }
 
-// S(c, k) = Z(c, k - d) (0 for k < d) over the 64 tiles held as 4 column
-// blocks of 16; d < 16 moves values inside each 16-lane row
-__device__ __forceinline__ void shift_tiles(const f4 (&Z)[4], f4 (&S)[4], int d)
+// S(c, k) = Z(c, k - D) (0 for k < D) over NB column blocks of 16 tiles;
+// D < 16 rotates inside each 16-lane row and takes the wrapped lanes from the
+// previous block.  Only the NSL live state registers are moved.
+template <int D, int NSL, int NB>
+__device__ __forceinline__ void shift_tiles(const f4 (&Z)[NB], f4 (&S)[NB])
 {
-    const int lane = threadIdx.x & 63;
-    if (d >= 16) {
-        const int dn = d >> 4;
+    if constexpr (D >= 16) {
+        constexpr int dn = D >> 4;
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb) S[nb] = (nb >= dn) ? Z[nb - dn < 0 ? 0 : nb - dn] : f4{0.f, 0.f, 0.f, 0.f};
-        return;
-    }
-    const int src = (lane & 48) | ((lane - d) & 15);
-    const bool wrap = (lane & 15) < d;
+        for (int nb = 0; nb < NB; ++nb) S[nb] = (nb >= dn) ? Z[nb >= dn ? nb - dn : 0] : f4{0.f, 0.f, 0.f, 0.f};
+    } else {
+        const bool wrap = (threadIdx.x & 15) < D;
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
-        f4 r0, r1;
+        for (int nb = 0; nb < NB; ++nb) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            r0[j] = __shfl(Z[nb][j], src, 64);
-            r1[j] = (nb > 0) ? __shfl(Z[nb > 0 ? nb - 1 : 0][j], src, 64) : 0.0f;
+            for (int j = 0; j < 4; ++j) {
+                if (j < NSL) {
+                    const float r0 = row_ror<D>(Z[nb][j]);
+                    const float r1 = (nb > 0) ? row_ror<D>(Z[nb > 0 ? nb - 1 : 0][j]) : 0.0f;
+                    S[nb][j] = wrap ? r1 : r0;
+                } else {
+                    S[nb][j] = 0.0f;
+                }
+            }
         }
-        S[nb] = wrap ? r1 : r0;
     }
 }
 
-__device__ __forceinline__ float abs_max4(const f4 (&v)[4])
-{
-    float m = 0.0f;
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) m = fmaxf(m, fabsf(v[nb][j]));
-    return m;
-}
-
-// Z += M^(2^j)-style product: Z[nb] += Pm * S[nb], Pm given as hi/lo blocks with scale sP
-__device__ __forceinline__ void apply_blocks(f4 (&Z)[4], const f4 (&S)[4], h4 ph, h4 pl, int sP, int nblocks)
-{
-    const int eS = scale_exp(wave_max(abs_max4(S)));
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
-        if (nb >= nblocks) break;
-        h4 sh, sl;
-        split(S[nb], eS, sh, sl);
-        f4 t = {0.f, 0.f, 0.f, 0.f};
-        t = mfma3(ph, pl, sh, sl, t);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) Z[nb][j] = fmaf(t[j], ldexpf(1.0f, -(sP + eS)), Z[nb][j]);
-    }
-}
-
-template <int L>
+template <int L, int NSL>
 __global__ __launch_bounds__(64) void bs_kernel(LaunchArgs a)
 {
-    constexpr int MB = L / 16;  // row/column blocks of T
+    constexpr int MB = L / 16;  // output row blocks of T
+    constexpr int KQ = L / 32;  // K slabs of 32 samples
     constexpr int G = kBsTiles;
     constexpr int CH = L * G;
     constexpr BsLayout Y = bs_layout(L);
     __shared__ float lp[64 * 16];
-    __shared__ uint4 xop[MB * 4 * 64];  // FP16 hi/lo input operands, reused by the final product
 
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
     const float *tab = a.coef;
     const cfloat *ctab = as_const(a.coef);
-    const int sT = (int)ctab[Y.kScale + 0], sO = (int)ctab[Y.kScale + 1], sG = (int)ctab[Y.kScale + 2];
+    const int sT = (int)ctab[Y.kScale + 0], sG = (int)ctab[Y.kScale + 1];
     const int depth = (int)ctab[Y.kDepth];
 
-    for_each_chunk(a.ticket, a.epoch, a.num_chunks, a.persistent,
-                   [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
-    const long base = (long)chunk * CH;
-    const int col = lane & 15, rq = lane >> 4;
-
-    // 1. input tile blocks in B layout: (sample 16 kb + 4 rq + j, tile 16 nb + col)
-    f4 Z[4];
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) Z[nb] = f4{0.f, 0.f, 0.f, 0.f};
-    int eX;
-    {
-        f4 xv[MB][4];
-#pragma unroll
-        for (int kb = 0; kb < MB; ++kb)
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb)
-                xv[kb][nb] = __builtin_nontemporal_load(
-                    reinterpret_cast<const f4 *>(a.in + base + (long)(16 * nb + col) * L + 16 * kb + 4 * rq));
-        float mx = 0.0f;
-#pragma unroll
-        for (int kb = 0; kb < MB; ++kb)
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) mx = fmaxf(mx, fabsf(xv[kb][nb][j]));
-        eX = scale_exp(wave_max(mx));
-        // V = Gam X (zero chunk-incoming state), operands kept in LDS for Y0 = T X
-#pragma unroll
-        for (int kb = 0; kb < MB; ++kb) {
-            const h4 gh = load_blk(tab, bs_G(Y, 0, kb)), gl = load_blk(tab, bs_G(Y, 1, kb));
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                h4 xh, xl;
-                split(xv[kb][nb], eX, xh, xl);
-                Z[nb] = mfma3(gh, gl, xh, xl, Z[nb]);
-                uint4 w;
-                const uint2 uh = __builtin_bit_cast(uint2, xh), ul = __builtin_bit_cast(uint2, xl);
-                w.x = uh.x;
-                w.y = uh.y;
-                w.z = ul.x;
-                w.w = ul.y;
-                xop[(kb * 4 + nb) * 64 + lane] = w;
-            }
-        }
-    }
-    {
-        const float u = ldexpf(1.0f, -(sG + eX));
+    // X as x32 B operands: tile 16 nb + col, K slab q, K rows 8 g + j =
+    // samples 32 q + 4 g + j (xa) and 32 q + 16 + 4 g + j - 4 (xb)
+    struct Raw {
+        f4 a[4][KQ], b[4][KQ];
+    };
+    auto load_chunk = [&](uint32_t chunk, Raw &r) {
+        const long base = (long)chunk * CH;
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) Z[nb][j] *= u;
-    }
+            for (int q = 0; q < KQ; ++q) {
+                const f4 *p = reinterpret_cast<const f4 *>(a.in + base + (long)(16 * nb + col) * L + 32 * q + 4 * g);
+                r.a[nb][q] = __builtin_nontemporal_load(p);
+                r.b[nb][q] = __builtin_nontemporal_load(p + 4);
+            }
+    };
 
-    // 2. tile end states, 6-level scan
+    auto process = [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag, const Raw &xr) {
+    const long base = (long)chunk * CH;
+
+    // 1. per-chunk scale and FP16 hi/lo split of X
+    h8 xh[4][KQ], xl[4][KQ];
+    int eX;
+    {
+        const auto &xa = xr.a;
+        const auto &xb = xr.b;
+        float mx = 0.0f;
 #pragma unroll
-    for (int lev = 0; lev < 6; ++lev) {
-        f4 S[4];
-        shift_tiles(Z, S, 1 << lev);
-        apply_blocks(Z, S, load_blk(tab, bs_P(Y, 0, lev)), load_blk(tab, bs_P(Y, 1, lev)),
-                     (int)ctab[Y.kScale + 3 + lev], 4);
+        for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+            for (int q = 0; q < KQ; ++q)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) mx = fmaxf(mx, fmaxf(fabsf(xa[nb][q][j]), fabsf(xb[nb][q][j])));
+        eX = scale_exp(wave_max(mx));
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+            for (int q = 0; q < KQ; ++q) split8(xa[nb][q], xb[nb][q], eX, xh[nb][q], xl[nb][q]);
     }
 
-    // 3. chunk partial = Z(:, 63): lanes 15, 31, 47, 63 hold components 4 rq + j
+    // 2. V = Gam X (zero chunk-incoming state) and Y0 = T X
+    f4 Z[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) {
+            const h8 gh = load_h8(tab, bs_G(Y, 0, q)), gl = load_h8(tab, bs_G(Y, 1, q));
+            acc = mfma_h(gl, xh[nb][q], acc);
+            acc = mfma_h(gh, xl[nb][q], acc);
+            acc = mfma_h(gh, xh[nb][q], acc);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Z[nb][j] = ldexpf(acc[j], -(sG + eX));
+    }
+    f4 Y0[MB][4];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+        const int qmax = (16 * mb + 15) / 32;  // T(16 mb + r, 32 q + c) = 0 beyond
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int q = 0; q < KQ; ++q) {
+                if (q > qmax) break;
+                const h8 th = load_h8(tab, bs_T(Y, 0, mb, q)), tl = load_h8(tab, bs_T(Y, 1, mb, q));
+                acc = mfma_h(tl, xh[nb][q], acc);
+                acc = mfma_h(th, xl[nb][q], acc);
+                acc = mfma_h(th, xh[nb][q], acc);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Y0[mb][nb][j] = ldexpf(acc[j], -(sT + eX));
+        }
+    }
+
+    // 3. tile end states, 6-level scan (levels 4 and 5 skip the all-zero blocks)
+    static_for<0, 6, 1>([&](auto LV) {
+        constexpr int D = 1 << decltype(LV)::value;
+        f4 S[4];
+        shift_tiles<D, NSL, 4>(Z, S);
+        const f4 P = load_f4(tab, bs_P(Y, decltype(LV)::value));
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+            if (D < 16 || nb >= (D >> 4)) Z[nb] = apply<NSL>(P, S[nb], Z[nb]);
+    });
+
+    // 4. chunk partial = Z(:, 63): lanes 15, 31, 47, 63 hold components 4 g + j
     uint64_t *rec = a.lb + (size_t)chunk * 32;
     if (col == 15) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) rec_store(rec + 4 * rq + j, part_flag, Z[3][j]);
+        for (int j = 0; j < 4; ++j) rec_store(rec + 4 * g + j, part_flag, Z[3][j]);
     }
     const float Xj = lookbackK<16>(a.lb, 32, chunk, part_flag, full_flag, tab + Y.kMc, depth, lp, a.spin_limit,
                                    a.err);
+    // X_in in B layout on column 0 (component 4 g + m lives on lane 4 g + m)
+    f4 xin;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xin[j] = (j < NSL) ? __shfl(Xj, 4 * g + j, 64) : 0.0f;
 
-    // injection W_k = Phi^(k+1) X_in: column 0 of block 0 = Phi X_in, a
+    // injection W_k = Phi^(k+1) X_in: Phi X_in in column 0 of one block, a
     // 4-level scan inside the block, then Phi^(16 nb) for blocks 1..3
-    f4 U[4];
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) U[nb] = f4{0.f, 0.f, 0.f, 0.f};
+    f4 U[1];
     {
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        f4 b0;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const float xm = readlane_f(Xj, m);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[j] = fmaf(tab[Y.kPhi + (4 * rq + j) * 16 + m], xm, acc[j]);
-        }
-        if (col == 0) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) U[0][j] = acc[j];
-        }
+        for (int j = 0; j < 4; ++j) b0[j] = (col == 0) ? xin[j] : 0.0f;
+        U[0] = apply<NSL>(load_f4(tab, Y.kPhi), b0, f4{0.f, 0.f, 0.f, 0.f});
     }
+    static_for<0, 4, 1>([&](auto LV) {
+        constexpr int D = 1 << decltype(LV)::value;
+        f4 S[1];
+        shift_tiles<D, NSL, 1>(U, S);
+        U[0] = apply<NSL>(load_f4(tab, bs_P(Y, decltype(LV)::value)), S[0], U[0]);
+    });
 #pragma unroll
-    for (int lev = 0; lev < 4; ++lev) {
-        f4 S[4];
-        shift_tiles(U, S, 1 << lev);
-        apply_blocks(U, S, load_blk(tab, bs_P(Y, 0, lev)), load_blk(tab, bs_P(Y, 1, lev)),
-                     (int)ctab[Y.kScale + 3 + lev], 1);
-    }
-    {
-        const int eU = scale_exp(wave_max(fmaxf(fmaxf(fabsf(U[0][0]), fabsf(U[0][1])),
-                                                fmaxf(fabsf(U[0][2]), fabsf(U[0][3])))));
-        h4 uh, ul;
-        split(U[0], eU, uh, ul);
+    for (int nb = 1; nb < 4; ++nb) Z[nb] = apply<NSL>(load_f4(tab, bs_Q(Y, nb)), U[0], Z[nb]);
 #pragma unroll
-        for (int nb = 1; nb < 4; ++nb) {
-            f4 t = {0.f, 0.f, 0.f, 0.f};
-            t = mfma3(load_blk(tab, bs_Q(Y, 0, nb)), load_blk(tab, bs_Q(Y, 1, nb)), uh, ul, t);
-            const float u = ldexpf(1.0f, -((int)ctab[Y.kScale + 9 + nb - 1] + eU));
-#pragma unroll
-            for (int j = 0; j < 4; ++j) Z[nb][j] = fmaf(t[j], u, Z[nb][j]);
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) Z[0][j] += U[0][j];
-    }
+    for (int j = 0; j < 4; ++j) Z[0][j] += U[0][j];
 
     // chunk full = Z(:, 63)
     if (col == 15) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) rec_store(rec + 16 + 4 * rq + j, full_flag, Z[3][j]);
+        for (int j = 0; j < 4; ++j) rec_store(rec + 16 + 4 * g + j, full_flag, Z[3][j]);
     }
 
-    // 4. S_k = state entering tile k; Y = T X + O S
+    // 5. S_k = state entering tile k (X_in for tile 0); Y = Y0 + O S
     f4 S[4];
-    shift_tiles(Z, S, 1);
-    {
-        // column 0 enters with X_in: component 4 rq + j lives on lane 4 rq + j
-        f4 xin;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) xin[j] = __shfl(Xj, 4 * rq + j, 64);
-        if (col == 0) S[0] = xin;
-    }
-    const int eS = scale_exp(wave_max(abs_max4(S)));
-    h4 sh[4], sl[4];
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) split(S[nb], eS, sh[nb], sl[nb]);
-    const float uT = ldexpf(1.0f, -(sT + eX)), uO = ldexpf(1.0f, -(sO + eS));
+    shift_tiles<1, NSL, 4>(Z, S);
+    if (col == 0) S[0] = xin;
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
-        const h4 oh = load_blk(tab, bs_O(Y, 0, mb)), ol = load_blk(tab, bs_O(Y, 1, mb));
+        const f4 O = load_f4(tab, bs_O(Y, mb));
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb) {
-            f4 ty = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kb = 0; kb < MB; ++kb) {
-                const uint4 w = xop[(kb * 4 + nb) * 64 + lane];
-                const h4 xh = __builtin_bit_cast(h4, make_uint2(w.x, w.y));
-                const h4 xl = __builtin_bit_cast(h4, make_uint2(w.z, w.w));
-                ty = mfma3(load_blk(tab, bs_T(Y, 0, mb, kb)), load_blk(tab, bs_T(Y, 1, mb, kb)), xh, xl, ty);
-            }
-            f4 to = {0.f, 0.f, 0.f, 0.f};
-            to = mfma3(oh, ol, sh[nb], sl[nb], to);
-            f4 y;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) y[j] = fmaf(to[j], uO, ty[j] * uT);
+            const f4 y = apply<NSL>(O, S[nb], Y0[mb][nb]);
             __builtin_nontemporal_store(
-                y, reinterpret_cast<f4 *>(a.out + base + (long)(16 * nb + col) * L + 16 * mb + 4 * rq));
+                y, reinterpret_cast<f4 *>(a.out + base + (long)(16 * nb + col) * L + 16 * mb + 4 * g));
         }
     }
-    });
+    };
+
+    if (a.persistent) {
+        // chunks blockIdx.x + i * grid; the next chunk's input is in flight
+        // while this one is scanned, looked back and written
+        const uint32_t epoch = epoch_load(a.epoch);
+        const uint32_t pf = 2u * epoch + 1u, ff = pf + 1u;
+        uint32_t c = blockIdx.x;
+        Raw cur;
+        if (c < a.num_chunks) load_chunk(c, cur);
+        for (; c < a.num_chunks; c += gridDim.x) {
+            Raw nxt;
+            const bool more = c + gridDim.x < a.num_chunks;
+            if (more) load_chunk(c + gridDim.x, nxt);
+            process(c, pf, ff, cur);
+            if (more) cur = nxt;
+        }
+        // the owner of the last chunk advances the epoch after publishing it
+        if (threadIdx.x == 0 && (a.num_chunks - 1u) % gridDim.x == blockIdx.x) epoch_store(a.epoch, epoch + 1u);
+    } else {
+        const Ticket tk = take_ticket(a.ticket, a.num_chunks);
+        Raw cur;
+        load_chunk(tk.chunk, cur);
+        process(tk.chunk, tk.part_flag, tk.full_flag, cur);
+    }
 }
 
-template <int L>
-static hipError_t launch_l(const LaunchArgs &a, hipStream_t s)
+template <int L, int NSL>
+static hipError_t launch_ln(const LaunchArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(bs_kernel<L>, dim3(a.grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((bs_kernel<L, NSL>), dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_bs(int L, const LaunchArgs &a, hipStream_t s)
+template <int L>
+static hipError_t launch_l(int nsl, const LaunchArgs &a, hipStream_t s)
 {
-    switch (L) {
-    case 32: return launch_l<32>(a, s);
-    case 64: return launch_l<64>(a, s);
+    switch (nsl) {
+    case 1: return launch_ln<L, 1>(a, s);
+    case 2: return launch_ln<L, 2>(a, s);
+    case 3: return launch_ln<L, 3>(a, s);
+    case 4: return launch_ln<L, 4>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-const void *kernel_bs(int L)
+template <int L>
+static const void *fn_l(int nsl)
+{
+    switch (nsl) {
+    case 1: return (const void *)bs_kernel<L, 1>;
+    case 2: return (const void *)bs_kernel<L, 2>;
+    case 3: return (const void *)bs_kernel<L, 3>;
+    case 4: return (const void *)bs_kernel<L, 4>;
+    default: return nullptr;
+    }
+}
+
+hipError_t launch_bs(int L, int ns, const LaunchArgs &a, hipStream_t s)
 {
     switch (L) {
-    case 32: return (const void *)bs_kernel<32>;
-    case 64: return (const void *)bs_kernel<64>;
+    case 32: return launch_l<32>(bs_slabs(ns), a, s);
+    case 64: return launch_l<64>(bs_slabs(ns), a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+const void *kernel_bs(int L, int ns)
+{
+    switch (L) {
+    case 32: return fn_l<32>(bs_slabs(ns));
+    case 64: return fn_l<64>(bs_slabs(ns));
     default: return nullptr;
     }
 }

@@ -193,7 +193,7 @@ const void *kernel_fn(const iirx_config *cfg, const Shape &sh)
     case IIRX_KERNEL_STCR: return kernel_stcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_DTCR: return kernel_dtcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_PLR: return kernel_plr(2 * cfg->n_sections, sh.plr_block, sh.plr_x);
-    case IIRX_KERNEL_BLOCKSTATE: return kernel_bs(cfg->n_blocks);
+    case IIRX_KERNEL_BLOCKSTATE: return kernel_bs(cfg->n_blocks, cfg->n_sections);
     default: return nullptr;
     }
 }
@@ -353,7 +353,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
     case IIRX_KERNEL_STCR: e = launch_stcr(p->lanes, p->n_blocks, a, s); break;
     case IIRX_KERNEL_DTCR: e = launch_dtcr(p->lanes, p->n_blocks, a, s); break;
     case IIRX_KERNEL_PLR: e = launch_plr(2 * p->cfg.n_sections, p->shape.plr_block, p->shape.plr_x, a, s); break;
-    case IIRX_KERNEL_BLOCKSTATE: e = launch_bs(p->n_blocks, a, s); break;
+    case IIRX_KERNEL_BLOCKSTATE: e = launch_bs(p->n_blocks, p->cfg.n_sections, a, s); break;
     default: break;
     }
     if (e != hipSuccess) return IIRX_ERR_HIP;

@@ -158,49 +158,55 @@ namespace iirx {
 // chained), padded to 16.  Tiles of L samples, G = 64 tiles per chunk:
 //   y_tile = T x_tile + O s,   s_next = Phi s + Gam x_tile
 // T: L x L lower-triangular Toeplitz of the impulse response, O: L x 16
-// (rows C A^i), Gam: 16 x L (columns A^(L-1-j) B), Phi = A^L.  MFMA operand
-// blocks (16x16, FP16 hi/lo of the matrix scaled by a power of two) are
-// stored in lane order: block word 2*lane + {0,1} = the 4 halves lane `lane`
-// feeds to v_mfma_f32_16x16x16_f16, A[l%16][4(l/16)+j].
-//   [kT]  T   hi/lo x MB x KB blocks        [kO] O   hi/lo x MB
-//   [kG]  Gam hi/lo x KB                     [kP] Phi^(2^j) hi/lo, j < 6
-//   [kQ]  Phi^(16 nb) hi/lo, nb = 1..3       [kScale] exponents s (matrix * 2^s)
-//   [kPhi] Phi (fp32, 16x16)                 [kMc] chunk transition Phi^64
-//   [kDepth] lookback depth
+// (rows C A^i), Gam: 16 x L (columns A^(L-1-j) B), Phi = A^L.  The p = 2 NS
+// states sit on rows 4 (s / nsl) + s % nsl of 16, nsl = ceil(p / 4), so that
+// the FP32 products need only nsl of the four K = 4 slabs.
+// FP16 blocks (T, Gam; hi/lo of the matrix scaled by 2^s) are the A operand
+// of v_mfma_f32_16x16x32_f16 in lane order: block word 4 lane + {0..3} = the
+// 8 halves A[l%16][8 (l/16) + j] of K slab q (samples 32 q ..); K row
+// 8 g + j is sample 32 q + 4 g + j (j < 4) or 32 q + 16 + 4 g + j - 4 (j >= 4).
+// FP32 blocks are the A operands of v_mfma_f32_16x16x4f32: word 4 lane + m =
+// M[l%16][4 (l/16) + m], slab m.
+//   [kT]  T hi/lo x MB row blocks x KQ slabs   [kG] Gam hi/lo x KQ slabs
+//   [kP]  Phi^(2^j), j < 6      [kQ] Phi^(16 nb), nb = 1..3     [kPhi] Phi
+//   [kO]  O row blocks          [kScale] exponents of T and Gam
+//   [kMc] chunk transition Phi^64 (fp32 row major)   [kDepth] lookback depth
 // ---------------------------------------------------------------------------
 struct BsLayout {
-    int MB, kT, kO, kG, kP, kQ, kScale, kPhi, kMc, kDepth, stride;
+    int MB, KQ, kT, kG, kP, kQ, kPhi, kO, kScale, kMc, kDepth, stride;
 };
 
 constexpr int kBsTiles = 64;  // tiles per chunk (one per MFMA column over 4 blocks)
-constexpr int kBsBlockWords = 128;
+constexpr int kBsBlockWords = 256;
 
 constexpr BsLayout bs_layout(int L)
 {
     BsLayout p{};
     p.MB = L / 16;
+    p.KQ = L / 32;
     p.kT = 0;
-    p.kO = p.kT + 2 * p.MB * p.MB * kBsBlockWords;
-    p.kG = p.kO + 2 * p.MB * kBsBlockWords;
-    p.kP = p.kG + 2 * p.MB * kBsBlockWords;
-    p.kQ = p.kP + 12 * kBsBlockWords;
-    p.kScale = p.kQ + 6 * kBsBlockWords;
-    p.kPhi = p.kScale + 16;
-    p.kMc = p.kPhi + 256;
+    p.kG = p.kT + 2 * p.MB * p.KQ * kBsBlockWords;
+    p.kP = p.kG + 2 * p.KQ * kBsBlockWords;
+    p.kQ = p.kP + 6 * kBsBlockWords;
+    p.kPhi = p.kQ + 3 * kBsBlockWords;
+    p.kO = p.kPhi + kBsBlockWords;
+    p.kScale = p.kO + p.MB * kBsBlockWords;
+    p.kMc = p.kScale + 4;
     p.kDepth = p.kMc + 256;
     p.stride = align4_c(p.kDepth + 4);
     return p;
 }
 
 // word offsets of operand blocks
-constexpr int bs_T(const BsLayout &y, int hl, int mb, int kb) { return y.kT + ((hl * y.MB + mb) * y.MB + kb) * kBsBlockWords; }
-constexpr int bs_O(const BsLayout &y, int hl, int mb) { return y.kO + (hl * y.MB + mb) * kBsBlockWords; }
-constexpr int bs_G(const BsLayout &y, int hl, int kb) { return y.kG + (hl * y.MB + kb) * kBsBlockWords; }
-constexpr int bs_P(const BsLayout &y, int hl, int j) { return y.kP + (hl * 6 + j) * kBsBlockWords; }
-constexpr int bs_Q(const BsLayout &y, int hl, int nb) { return y.kQ + (hl * 3 + nb - 1) * kBsBlockWords; }
-// scale slots: 0 T, 1 O, 2 Gam, 3 + j Phi^(2^j), 9 + nb - 1 Phi^(16 nb)
+constexpr int bs_T(const BsLayout &y, int hl, int mb, int q) { return y.kT + ((hl * y.MB + mb) * y.KQ + q) * kBsBlockWords; }
+constexpr int bs_G(const BsLayout &y, int hl, int q) { return y.kG + (hl * y.KQ + q) * kBsBlockWords; }
+constexpr int bs_P(const BsLayout &y, int j) { return y.kP + j * kBsBlockWords; }
+constexpr int bs_Q(const BsLayout &y, int nb) { return y.kQ + (nb - 1) * kBsBlockWords; }
+constexpr int bs_O(const BsLayout &y, int mb) { return y.kO + mb * kBsBlockWords; }
+constexpr int bs_slabs(int ns) { return (ns + 1) / 2; }
+// scale slots: 0 T, 1 Gam
 
-hipError_t launch_bs(int L, const LaunchArgs &a, hipStream_t s);
-const void *kernel_bs(int L);
+hipError_t launch_bs(int L, int ns, const LaunchArgs &a, hipStream_t s);
+const void *kernel_bs(int L, int ns);
 
 }  // namespace iirx

@@ -268,19 +268,29 @@ uint16_t half_bits(_Float16 h)
     return u;
 }
 
-// 16x16 operand block (rows r0.., cols c0..) of matrix m (rows x cols, row
-// major, zero outside) in MFMA A-operand lane order, hi (hl = 0) or lo (1)
-void put_block(float *dst, const double *m, int rows, int cols, int r0, int c0, int s, int hl)
+// FP16 A operand of v_mfma_f32_16x16x32_f16 (kernels.hpp: bs_layout): rows
+// r0..r0+15 and K slab q of matrix m (rows x cols, row major, zero outside),
+// scaled by 2^s, hi (hl = 0) or lo (1) part
+void put_block32(float *dst, const double *m, int rows, int cols, int r0, int q, int s, int hl)
 {
     uint16_t *h = reinterpret_cast<uint16_t *>(dst);
     for (int l = 0; l < 64; ++l)
-        for (int j = 0; j < 4; ++j) {
-            const int r = r0 + (l % 16), c = c0 + 4 * (l / 16) + j;
+        for (int j = 0; j < 8; ++j) {
+            const int g = l / 16, r = r0 + (l % 16);
+            const int c = 32 * q + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
             const double v = (r < rows && c < cols) ? std::ldexp(m[(size_t)r * cols + c], s) : 0.0;
             const _Float16 hi = (_Float16)(float)v;
             const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
-            h[l * 4 + j] = half_bits(hl == 0 ? hi : lo);
+            h[l * 8 + j] = half_bits(hl == 0 ? hi : lo);
         }
+}
+
+// FP32 A operand of v_mfma_f32_16x16x4f32: word 4 l + m = M[l%16][4 (l/16) + m]
+// of the 16 x 16 block at row r0 of m (rows x 16)
+void put_f32(float *dst, const double *m, int r0)
+{
+    for (int l = 0; l < 64; ++l)
+        for (int k = 0; k < 4; ++k) dst[4 * l + k] = (float)m[(size_t)(r0 + l % 16) * 16 + 4 * (l / 16) + k];
 }
 
 }  // namespace
@@ -345,6 +355,22 @@ int bs_tables(const float *sos, int ns, int L, std::vector<float> &out)
             C[c] *= dsc[c];
         }
     }
+    // state s onto row 4 (s / nsl) + s % nsl (only nsl FP32 K slabs are nonzero)
+    {
+        const int nsl = bs_slabs(ns);
+        int rho[16];
+        for (int st = 0; st < 16; ++st) rho[st] = (st < p) ? 4 * (st / nsl) + st % nsl : -1;
+        Mat A2(256, 0.0);
+        std::vector<double> B2(16, 0.0), C2(16, 0.0);
+        for (int r = 0; r < p; ++r) {
+            B2[rho[r]] = B[r];
+            C2[rho[r]] = C[r];
+            for (int c = 0; c < p; ++c) A2[rho[r] * 16 + rho[c]] = A[r * 16 + c];
+        }
+        A.swap(A2);
+        B.swap(B2);
+        C.swap(C2);
+    }
     // impulse response, observability, controllability
     std::vector<double> h((size_t)L), Om((size_t)L * 16), Gm((size_t)16 * L), Tm((size_t)L * L, 0.0);
     {
@@ -387,31 +413,19 @@ int bs_tables(const float *sos, int ns, int L, std::vector<float> &out)
     const Mat Mc = mat_mul16(P[5], P[5]);  // Phi^64
 
     float *sc = out.data() + Y.kScale;
-    const int sT = pow2_scale(Tm.data(), Tm.size()), sO = pow2_scale(Om.data(), Om.size());
-    const int sG = pow2_scale(Gm.data(), Gm.size());
+    const int sT = pow2_scale(Tm.data(), Tm.size()), sG = pow2_scale(Gm.data(), Gm.size());
     sc[0] = (float)sT;
-    sc[1] = (float)sO;
-    sc[2] = (float)sG;
+    sc[1] = (float)sG;
     for (int hl = 0; hl < 2; ++hl) {
         for (int mb = 0; mb < Y.MB; ++mb)
-            for (int kb = 0; kb < Y.MB; ++kb) put_block(out.data() + bs_T(Y, hl, mb, kb), Tm.data(), L, L, 16 * mb, 16 * kb, sT, hl);
-        for (int mb = 0; mb < Y.MB; ++mb) put_block(out.data() + bs_O(Y, hl, mb), Om.data(), L, 16, 16 * mb, 0, sO, hl);
-        for (int kb = 0; kb < Y.MB; ++kb) put_block(out.data() + bs_G(Y, hl, kb), Gm.data(), 16, L, 0, 16 * kb, sG, hl);
-        for (int j = 0; j < 6; ++j) {
-            const int s = pow2_scale(P[j].data(), 256);
-            sc[3 + j] = (float)s;
-            put_block(out.data() + bs_P(Y, hl, j), P[j].data(), 16, 16, 0, 0, s, hl);
-        }
-        for (int nb = 1; nb < 4; ++nb) {
-            const int s = pow2_scale(Q[nb].data(), 256);
-            sc[9 + nb - 1] = (float)s;
-            put_block(out.data() + bs_Q(Y, hl, nb), Q[nb].data(), 16, 16, 0, 0, s, hl);
-        }
+            for (int q = 0; q < Y.KQ; ++q) put_block32(out.data() + bs_T(Y, hl, mb, q), Tm.data(), L, L, 16 * mb, q, sT, hl);
+        for (int q = 0; q < Y.KQ; ++q) put_block32(out.data() + bs_G(Y, hl, q), Gm.data(), 16, L, 0, q, sG, hl);
     }
-    for (int i = 0; i < 256; ++i) {
-        out[Y.kPhi + i] = (float)Phi[i];
-        out[Y.kMc + i] = (float)Mc[i];
-    }
+    for (int j = 0; j < 6; ++j) put_f32(out.data() + bs_P(Y, j), P[j].data(), 0);
+    for (int nb = 1; nb < 4; ++nb) put_f32(out.data() + bs_Q(Y, nb), Q[nb].data(), 0);
+    put_f32(out.data() + Y.kPhi, Phi.data(), 0);
+    for (int mb = 0; mb < Y.MB; ++mb) put_f32(out.data() + bs_O(Y, mb), Om.data(), 16 * mb);
+    for (int i = 0; i < 256; ++i) out[Y.kMc + i] = (float)Mc[i];
     // lookback depth of the chunk transition
     Mat Pw = Mc;
     int depth = 0;

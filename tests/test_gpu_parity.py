@@ -285,7 +285,7 @@ def err_rms(y, ref):
     return float(np.max(np.abs(np.asarray(y, np.float64) - ref)) / np.sqrt(np.mean(ref ** 2)))
 
 
-@pytest.mark.parametrize("order", ORDERS)
+@pytest.mark.parametrize("order", ORDERS + [6, 12])  # 1, 2, 3 and 4 FP32 K slabs
 @pytest.mark.parametrize("L", [32, 64])
 @pytest.mark.parametrize("wn", [0.1, 0.02])
 def test_blockstate_parity(device, order, L, wn):

@@ -128,39 +128,50 @@ def test_config_check(kernel, B, N, n, ok):
 def _bs_tables(sos, L):
     """Dequantised block-state tables (layout of kernels.hpp:bs_layout)."""
     tab, stride = iirx.plan_tables("BS", sos, 64 * L * 4, n_blocks=L)
-    MB = L // 16
+    MB, KQ = L // 16, L // 32
     kT = 0
-    kO = kT + 2 * MB * MB * 128
-    kG = kO + 2 * MB * 128
-    kP = kG + 2 * MB * 128
-    kQ = kP + 12 * 128
-    kS = kQ + 6 * 128
-    kPhi = kS + 16
-    kMc = kPhi + 256
+    kG = kT + 2 * MB * KQ * 256
+    kP = kG + 2 * KQ * 256
+    kQ = kP + 6 * 256
+    kPhi = kQ + 3 * 256
+    kO = kPhi + 256
+    kS = kO + MB * 256
+    kMc = kS + 4
     halves = tab.view(np.uint32).view(np.float16)  # 2 halves per word
 
-    def block(word_off):
-        h = halves[2 * word_off: 2 * word_off + 256].astype(np.float64).reshape(64, 4)
-        m = np.zeros((16, 16))
+    def block32(word_off):
+        """16 x 32 x32-MFMA A operand: lane l, half j = A[l%16][K row 8(l/16)+j],
+        K row -> sample 4g + j (j < 4) or 16 + 4g + j - 4 of the slab."""
+        h = halves[2 * word_off: 2 * word_off + 512].astype(np.float64).reshape(64, 8)
+        m = np.zeros((16, 32))
         for l in range(64):
-            for j in range(4):
-                m[l % 16, 4 * (l // 16) + j] = h[l, j]
+            g = l // 16
+            for j in range(8):
+                m[l % 16, 4 * g + j if j < 4 else 16 + 4 * g + j - 4] = h[l, j]
         return m
 
-    def mat(kind, rows, cols, sidx, offs):
-        s = tab[kS + sidx]
-        m = np.zeros((rows, cols))
-        for (r0, c0, off) in offs:
-            hi, lo = block(off), block(off + offs_lo[kind])
-            m[r0:r0 + 16, c0:c0 + 16] = hi + lo
-        return m * 2.0 ** (-s)
+    def blockf(word_off):
+        """16 x 16 FP32 A operand: word 4 l + m = M[l%16][4(l/16)+m]."""
+        w = tab[word_off: word_off + 256].astype(np.float64).reshape(64, 4)
+        m = np.zeros((16, 16))
+        for l in range(64):
+            m[l % 16, 4 * (l // 16): 4 * (l // 16) + 4] = w[l]
+        return m
 
-    offs_lo = {"T": MB * MB * 128, "O": MB * 128, "G": MB * 128, "P": 6 * 128, "Q": 3 * 128}
-    T = mat("T", L, L, 0, [(16 * mb, 16 * kb, kT + (mb * MB + kb) * 128) for mb in range(MB) for kb in range(MB)])
-    Om = mat("O", L, 16, 1, [(16 * mb, 0, kO + mb * 128) for mb in range(MB)])
-    Gm = mat("G", 16, L, 2, [(0, 16 * kb, kG + kb * 128) for kb in range(MB)])
-    P0 = mat("P", 16, 16, 3, [(0, 0, kP)])
-    Phi = tab[kPhi:kPhi + 256].reshape(16, 16).astype(np.float64)
+    T = np.zeros((L, L))
+    for mb in range(MB):
+        for q in range(KQ):
+            off = kT + (mb * KQ + q) * 256
+            T[16 * mb:16 * mb + 16, 32 * q:32 * q + 32] = block32(off) + block32(off + MB * KQ * 256)
+    T *= 2.0 ** (-tab[kS])
+    Gm = np.zeros((16, L))
+    for q in range(KQ):
+        off = kG + q * 256
+        Gm[:, 32 * q:32 * q + 32] = block32(off) + block32(off + KQ * 256)
+    Gm *= 2.0 ** (-tab[kS + 1])
+    Om = np.vstack([blockf(kO + mb * 256) for mb in range(MB)])
+    P0 = blockf(kP)
+    Phi = blockf(kPhi)
     Mc = tab[kMc:kMc + 256].reshape(16, 16).astype(np.float64)
     return T, Om, Gm, P0, Phi, Mc, int(tab[kMc + 256])
 
@@ -175,7 +186,12 @@ def test_blockstate_tables_reproduce_the_cascade(order, L):
     from oracle_lib import golden
     sos, g = iirx.butterworth_sos(order, 0.1)
     T, Om, Gm, P0, Phi, Mc, depth = _bs_tables(sos, L)
-    assert np.allclose(P0, Phi, rtol=1e-6, atol=1e-6 * np.abs(Phi).max())
+    assert np.array_equal(P0, Phi)
+    # state s sits on row 4 (s // nsl) + s % nsl: rows with r % 4 >= nsl are empty
+    nsl = (order // 2 + 1) // 2
+    dead = [r for r in range(16) if r % 4 >= nsl]
+    assert not Phi[dead, :].any() and not Phi[:, dead].any()
+    assert not Gm[dead, :].any() and not Om[:, dead].any()
     assert np.allclose(np.linalg.matrix_power(Phi, 64), Mc, rtol=1e-4, atol=1e-30 + 1e-6 * np.abs(Mc).max())
     n = 64 * L * 3
     x = (g * iirx.fill_normal(4, n).astype(np.float64)).astype(np.float32)
