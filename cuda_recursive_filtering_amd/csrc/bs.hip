@@ -119,15 +119,24 @@ __device__ __forceinline__ void shift_tiles(const f4 (&Z)[NB], f4 (&S)[NB])
     }
 }
 
+// waves per SIMD the register budget is set for (LDS: the chunk's input
+// buffer plus 16 lookback slots)
+template <int L>
+constexpr int bs_waves() { return L == 32 ? 4 : 2; }
+
 template <int L, int NSL>
-__global__ __launch_bounds__(64) void bs_kernel(LaunchArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(bs_waves<L>(), bs_waves<L>())))
+void bs_kernel(LaunchArgs a)
 {
     constexpr int MB = L / 16;  // output row blocks of T
     constexpr int KQ = L / 32;  // K slabs of 32 samples
     constexpr int G = kBsTiles;
     constexpr int CH = L * G;
+    constexpr int NLD = 4 * KQ * 2;  // 16-byte loads per lane and chunk
     constexpr BsLayout Y = bs_layout(L);
-    __shared__ float lp[64 * 16];
+    constexpr int kSlots = 16;
+    __shared__ float lp[kSlots * 16];
+    __shared__ f4 xbuf[NLD * 64];  // one chunk of X, load i of lane l at xbuf[64 i + l]
 
     const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
     const float *tab = a.coef;
@@ -136,20 +145,36 @@ __global__ __launch_bounds__(64) void bs_kernel(LaunchArgs a)
     const int depth = (int)ctab[Y.kDepth];
 
     // X as x32 B operands: tile 16 nb + col, K slab q, K rows 8 g + j =
-    // samples 32 q + 4 g + j (xa) and 32 q + 16 + 4 g + j - 4 (xb)
+    // samples 32 q + 4 g + j (a) and 32 q + 16 + 4 g + j - 4 (b).  The chunk
+    // is fetched HBM -> LDS by global_load_lds (no registers held while it
+    // is in flight); the caller waits for it before read_chunk.
     struct Raw {
         f4 a[4][KQ], b[4][KQ];
     };
-    auto load_chunk = [&](uint32_t chunk, Raw &r) {
+    auto fetch_chunk = [&](uint32_t chunk) {
         const long base = (long)chunk * CH;
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
+            for (int q = 0; q < KQ; ++q)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const float *p = a.in + base + (long)(16 * nb + col) * L + 32 * q + 16 * h + 4 * g;
+                    __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void *)(p),
+                                                     (__attribute__((address_space(3))) void *)(&xbuf[((nb * KQ + q) * 2 + h) * 64]),
+                                                     16, 0, 0);
+                }
+    };
+    auto read_chunk = [&](Raw &r) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the LDS fill has landed
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
             for (int q = 0; q < KQ; ++q) {
-                const f4 *p = reinterpret_cast<const f4 *>(a.in + base + (long)(16 * nb + col) * L + 32 * q + 4 * g);
-                r.a[nb][q] = __builtin_nontemporal_load(p);
-                r.b[nb][q] = __builtin_nontemporal_load(p + 4);
+                r.a[nb][q] = xbuf[((nb * KQ + q) * 2 + 0) * 64 + lane];
+                r.b[nb][q] = xbuf[((nb * KQ + q) * 2 + 1) * 64 + lane];
             }
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): reads done before the next fill
     };
 
     auto process = [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag, const Raw &xr) {
@@ -227,8 +252,8 @@ __global__ __launch_bounds__(64) void bs_kernel(LaunchArgs a)
 #pragma unroll
         for (int j = 0; j < 4; ++j) rec_store(rec + 4 * g + j, part_flag, Z[3][j]);
     }
-    const float Xj = lookbackK<16>(a.lb, 32, chunk, part_flag, full_flag, tab + Y.kMc, depth, lp, a.spin_limit,
-                                   a.err);
+    const float Xj = lookbackK<16, kSlots>(a.lb, 32, chunk, part_flag, full_flag, tab + Y.kMc, depth, lp,
+                                           a.spin_limit, a.err);
     // X_in in B layout on column 0 (component 4 g + m lives on lane 4 g + m)
     f4 xin;
 #pragma unroll
@@ -277,26 +302,25 @@ __global__ __launch_bounds__(64) void bs_kernel(LaunchArgs a)
     };
 
     if (a.persistent) {
-        // chunks blockIdx.x + i * grid; the next chunk's input is in flight
-        // while this one is scanned, looked back and written
+        // chunks blockIdx.x + i * grid; the next chunk's input streams into
+        // LDS while this one is scanned, looked back and written
         const uint32_t epoch = epoch_load(a.epoch);
         const uint32_t pf = 2u * epoch + 1u, ff = pf + 1u;
         uint32_t c = blockIdx.x;
-        Raw cur;
-        if (c < a.num_chunks) load_chunk(c, cur);
+        if (c < a.num_chunks) fetch_chunk(c);
         for (; c < a.num_chunks; c += gridDim.x) {
-            Raw nxt;
-            const bool more = c + gridDim.x < a.num_chunks;
-            if (more) load_chunk(c + gridDim.x, nxt);
+            Raw cur;
+            read_chunk(cur);
+            if (c + gridDim.x < a.num_chunks) fetch_chunk(c + gridDim.x);
             process(c, pf, ff, cur);
-            if (more) cur = nxt;
         }
         // the owner of the last chunk advances the epoch after publishing it
         if (threadIdx.x == 0 && (a.num_chunks - 1u) % gridDim.x == blockIdx.x) epoch_store(a.epoch, epoch + 1u);
     } else {
         const Ticket tk = take_ticket(a.ticket, a.num_chunks);
         Raw cur;
-        load_chunk(tk.chunk, cur);
+        fetch_chunk(tk.chunk);
+        read_chunk(cur);
         process(tk.chunk, tk.part_flag, tk.full_flag, cur);
     }
 }

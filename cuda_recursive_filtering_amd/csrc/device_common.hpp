@@ -260,14 +260,17 @@ __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int se
 // one at a time with word j on lane j; a record is used whole, all-full or
 // all-partial, never mixed.  The fold X <- F X + p runs farthest first, with
 // row j of the chunk transition F (K x K, row-major in `F`) on lane j.
-// `lp` is LDS scratch of 64*K floats.  Returns X_j on lane j < K (0 elsewhere).
-template <int K>
+// `lp` is LDS scratch of SLOTS*K floats: at most SLOTS partials are folded;
+// an unbounded lookback (depth 0) or one deeper than SLOTS takes a full
+// record in its last slot.  Returns X_j on lane j < K (0 elsewhere).
+template <int K, int SLOTS = kWave>
 __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint32_t chunk, uint32_t part_flag,
                                           uint32_t full_flag, const float *F, int depth, float *lp,
                                           uint32_t spin_limit, uint32_t *err)
 {
     const int lane = threadIdx.x & (kWave - 1);
-    const int maxd = depth > 0 ? depth : kWave;
+    const bool capped = depth <= 0 || depth > SLOTS;
+    const int maxd = capped ? SLOTS : depth;
     float Xj = 0.0f;
     int D = 0;
     bool done = false;
@@ -290,7 +293,7 @@ __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint
             }
             // the last slot of an unbounded fold must be a full record
             const uint64_t p0 = (lane < K) ? rec_load(pr + lane) : 0ull;
-            if (__ballot(lane < K && rec_flag(p0) != part_flag) == 0 && !(depth == 0 && d == maxd - 1)) {
+            if (__ballot(lane < K && rec_flag(p0) != part_flag) == 0 && !(capped && d == maxd - 1)) {
                 if (lane < K) lp[d * K + lane] = rec_val(p0);
                 break;
             }
