@@ -132,12 +132,13 @@ def _stream_ptr(stream):
 class Plan:
     """A runtime plan: one filter, one geometry, one length, one device.
 
-    kernel       'PH' | 'STCR' | 'DTCR' | 'PLR'             (KERNEL)
+    kernel       'PH' | 'STCR' | 'DTCR' | 'PLR' | 'BS'      (KERNEL; BS has no counterpart)
     sos          (n_sections, 6) b0-normalised sections     (FILTER_SOS)
     n_samples    signal length                              (2^N_SAMPLES_LOG2)
     block_size   lanes per chunk, 32 or 64; PLR: threads per
-                 block (0/None = per-order default)         (BLOCK_SIZE / PLR_BLOCK_SIZE)
-    n_blocks     samples per lane                           (N_BLOCKS)
+                 block (0/None = per-order default); BS:
+                 tiles per chunk, 16, 32 or 64              (BLOCK_SIZE / PLR_BLOCK_SIZE)
+    n_blocks     samples per lane; BS: tile length, 32/64   (N_BLOCKS)
     unrolled     literal-index back substitution            (HANDUNROLLED)
     plr_x, plr_x_max   PLR values per thread                (PLR_X / PLR_X_MAX)
     schedule     'persistent' (default) or 'ticket'         (no counterpart)

@@ -4,21 +4,22 @@
 // ph/stcr/dtcr (the whole cascade, zero initial state) in a form whose cost
 // grows with the order only through the state dimension of small products.
 // The cascade is one state-space system of dimension p = 2 N_SECTIONS,
-// embedded in 16 rows.  A chunk of G = 64 tiles of L samples is one wave64:
+// embedded in 16 rows.  A chunk of G (16, 32 or 64) tiles of L samples is one
+// wave64:
 //   1. the zero-state parts Y0 = T X (T: L x L lower-triangular Toeplitz of
 //      the impulse response) and V = Gam X (Gam: 16 x L), as
 //      v_mfma_f32_16x16x32_f16 products of FP16 hi/lo operands
 //      (hi*hi + hi*lo + lo*hi, FP32 accumulation); T and Gam are scaled by
 //      a power of two on the host, X by one per chunk, so that FP16 holds
 //      them; blocks of T above the diagonal are skipped;
-//   2. the tile end states Z_k = sum_{j<=k} Phi^(k-j) V_j by a 6-level
-//      Kogge-Stone scan over the 64 tiles, each level Phi^(2^j) * shift(Z)
+//   2. the tile end states Z_k = sum_{j<=k} Phi^(k-j) V_j by a log2(G)-level
+//      Kogge-Stone scan over the G tiles, each level Phi^(2^j) * shift(Z)
 //      on v_mfma_f32_16x16x4f32 (plain FP32 operands: no splitting, no data
 //      scaling), over ceil(p/4) K slabs only;
 //   3. the chunk's incoming state X_in from the lookback (16-word records,
-//      chunk transition Phi^64, bounded depth), injected as Phi^(k+1) X_in:
-//      Phi X_in in tile 0, a 4-level scan inside one 16-tile block and three
-//      Phi^(16 nb) products;
+//      chunk transition Phi^G, bounded depth), injected as Phi^(k+1) X_in:
+//      Phi X_in in tile 0, a 4-level scan inside one 16-tile block and
+//      Phi^(16 nb) products for the other blocks;
 //   4. Y = Y0 + O S, S_k = state entering tile k, accumulated onto Y0 by the
 //      FP32 matrix core.
 // Both MFMA shapes put D[4 (l/16) + j][l%16] on lane l, register j: tiles are
@@ -119,20 +120,26 @@ __device__ __forceinline__ void shift_tiles(const f4 (&Z)[NB], f4 (&S)[NB])
     }
 }
 
-// waves per SIMD the register budget is set for (LDS: the chunk's input
-// buffer plus 16 lookback slots)
-template <int L>
-constexpr int bs_waves() { return L == 32 ? 4 : 2; }
+// waves per SIMD the register budget is set for; the LDS (the chunk's input
+// buffer, G L floats, plus 16 lookback slots) allows as many
+template <int L, int G>
+constexpr int bs_waves()
+{
+    return G == 64 ? (L == 32 ? 3 : 2) : G == 32 ? (L == 32 ? 5 : 3) : (L == 32 ? 6 : 4);
+}
 
-template <int L, int NSL>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(bs_waves<L>(), bs_waves<L>())))
+template <int G>
+constexpr int bs_levels() { return G == 16 ? 4 : G == 32 ? 5 : 6; }
+
+template <int L, int G, int NSL>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(bs_waves<L, G>(), bs_waves<L, G>())))
 void bs_kernel(LaunchArgs a)
 {
     constexpr int MB = L / 16;  // output row blocks of T
     constexpr int KQ = L / 32;  // K slabs of 32 samples
-    constexpr int G = kBsTiles;
+    constexpr int NB = G / 16;  // column blocks of 16 tiles
     constexpr int CH = L * G;
-    constexpr int NLD = 4 * KQ * 2;  // 16-byte loads per lane and chunk
+    constexpr int NLD = NB * KQ * 2;  // 16-byte loads per lane and chunk
     constexpr BsLayout Y = bs_layout(L);
     constexpr int kSlots = 16;
     __shared__ float lp[kSlots * 16];
@@ -147,14 +154,14 @@ void bs_kernel(LaunchArgs a)
     // X as x32 B operands: tile 16 nb + col, K slab q, K rows 8 g + j =
     // samples 32 q + 4 g + j (a) and 32 q + 16 + 4 g + j - 4 (b).  The chunk
     // is fetched HBM -> LDS by global_load_lds (no registers held while it
-    // is in flight); the caller waits for it before read_chunk.
+    // is in flight); read_chunk waits for it.
     struct Raw {
-        f4 a[4][KQ], b[4][KQ];
+        f4 a[NB][KQ], b[NB][KQ];
     };
     auto fetch_chunk = [&](uint32_t chunk) {
         const long base = (long)chunk * CH;
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb)
+        for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
             for (int q = 0; q < KQ; ++q)
 #pragma unroll
@@ -165,45 +172,49 @@ void bs_kernel(LaunchArgs a)
                                                      16, 0, 0);
                 }
     };
-    auto read_chunk = [&](Raw &r) {
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the LDS fill has landed
+    // vmcnt counts loads and stores together and retires them in issue
+    // order, so after the first chunk the fill has landed once at most the
+    // MB * NB output stores issued after it are outstanding
+    auto read_chunk = [&](Raw &r, bool first) {
+        if (first)
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(%0)" ::"n"(MB * NB) : "memory");
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb)
+        for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
             for (int q = 0; q < KQ; ++q) {
                 r.a[nb][q] = xbuf[((nb * KQ + q) * 2 + 0) * 64 + lane];
                 r.b[nb][q] = xbuf[((nb * KQ + q) * 2 + 1) * 64 + lane];
             }
-        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): reads done before the next fill
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // reads done before the next fill
     };
 
     auto process = [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag, const Raw &xr) {
     const long base = (long)chunk * CH;
 
     // 1. per-chunk scale and FP16 hi/lo split of X
-    h8 xh[4][KQ], xl[4][KQ];
+    h8 xh[NB][KQ], xl[NB][KQ];
     int eX;
     {
-        const auto &xa = xr.a;
-        const auto &xb = xr.b;
         float mx = 0.0f;
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb)
+        for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
             for (int q = 0; q < KQ; ++q)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) mx = fmaxf(mx, fmaxf(fabsf(xa[nb][q][j]), fabsf(xb[nb][q][j])));
+                for (int j = 0; j < 4; ++j) mx = fmaxf(mx, fmaxf(fabsf(xr.a[nb][q][j]), fabsf(xr.b[nb][q][j])));
         eX = scale_exp(wave_max(mx));
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb)
+        for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-            for (int q = 0; q < KQ; ++q) split8(xa[nb][q], xb[nb][q], eX, xh[nb][q], xl[nb][q]);
+            for (int q = 0; q < KQ; ++q) split8(xr.a[nb][q], xr.b[nb][q], eX, xh[nb][q], xl[nb][q]);
     }
 
     // 2. V = Gam X (zero chunk-incoming state) and Y0 = T X
-    f4 Z[4];
+    f4 Z[NB];
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
+    for (int nb = 0; nb < NB; ++nb) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int q = 0; q < KQ; ++q) {
@@ -215,12 +226,12 @@ void bs_kernel(LaunchArgs a)
 #pragma unroll
         for (int j = 0; j < 4; ++j) Z[nb][j] = ldexpf(acc[j], -(sG + eX));
     }
-    f4 Y0[MB][4];
+    f4 Y0[MB][NB];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
         const int qmax = (16 * mb + 15) / 32;  // T(16 mb + r, 32 q + c) = 0 beyond
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
+        for (int nb = 0; nb < NB; ++nb) {
             f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int q = 0; q < KQ; ++q) {
@@ -235,32 +246,39 @@ void bs_kernel(LaunchArgs a)
         }
     }
 
-    // 3. tile end states, 6-level scan (levels 4 and 5 skip the all-zero blocks)
-    static_for<0, 6, 1>([&](auto LV) {
+    // 3. tile end states, log2(G)-level scan (levels D >= 16 skip the
+    //    all-zero blocks)
+    static_for<0, bs_levels<G>(), 1>([&](auto LV) {
         constexpr int D = 1 << decltype(LV)::value;
-        f4 S[4];
-        shift_tiles<D, NSL, 4>(Z, S);
+        f4 S[NB];
+        shift_tiles<D, NSL, NB>(Z, S);
         const f4 P = load_f4(tab, bs_P(Y, decltype(LV)::value));
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb)
+        for (int nb = 0; nb < NB; ++nb)
             if (D < 16 || nb >= (D >> 4)) Z[nb] = apply<NSL>(P, S[nb], Z[nb]);
     });
 
-    // 4. chunk partial = Z(:, 63): lanes 15, 31, 47, 63 hold components 4 g + j
+    // 4. chunk partial = Z(:, G-1): lanes 15, 31, 47, 63 hold components 4 g + j
     uint64_t *rec = a.lb + (size_t)chunk * 32;
     if (col == 15) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) rec_store(rec + 4 * g + j, part_flag, Z[3][j]);
+        for (int j = 0; j < 4; ++j) rec_store(rec + 4 * g + j, part_flag, Z[NB - 1][j]);
     }
     const float Xj = lookbackK<16, kSlots>(a.lb, 32, chunk, part_flag, full_flag, tab + Y.kMc, depth, lp,
                                            a.spin_limit, a.err);
+    // the output stage's O blocks, fetched before any store of this chunk:
+    // vmcnt retires in issue order, so a load issued behind stores waits for them
+    f4 Ob[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) Ob[mb] = load_f4(tab, bs_O(Y, mb));
+    asm volatile("" ::: "memory");
     // X_in in B layout on column 0 (component 4 g + m lives on lane 4 g + m)
     f4 xin;
 #pragma unroll
     for (int j = 0; j < 4; ++j) xin[j] = (j < NSL) ? __shfl(Xj, 4 * g + j, 64) : 0.0f;
 
     // injection W_k = Phi^(k+1) X_in: Phi X_in in column 0 of one block, a
-    // 4-level scan inside the block, then Phi^(16 nb) for blocks 1..3
+    // 4-level scan inside the block, then Phi^(16 nb) for blocks 1..NB-1
     f4 U[1];
     {
         f4 b0;
@@ -275,26 +293,25 @@ void bs_kernel(LaunchArgs a)
         U[0] = apply<NSL>(load_f4(tab, bs_P(Y, decltype(LV)::value)), S[0], U[0]);
     });
 #pragma unroll
-    for (int nb = 1; nb < 4; ++nb) Z[nb] = apply<NSL>(load_f4(tab, bs_Q(Y, nb)), U[0], Z[nb]);
+    for (int nb = 1; nb < NB; ++nb) Z[nb] = apply<NSL>(load_f4(tab, bs_Q(Y, nb)), U[0], Z[nb]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) Z[0][j] += U[0][j];
 
-    // chunk full = Z(:, 63)
+    // chunk full = Z(:, G-1)
     if (col == 15) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) rec_store(rec + 16 + 4 * g + j, full_flag, Z[3][j]);
+        for (int j = 0; j < 4; ++j) rec_store(rec + 16 + 4 * g + j, full_flag, Z[NB - 1][j]);
     }
 
     // 5. S_k = state entering tile k (X_in for tile 0); Y = Y0 + O S
-    f4 S[4];
-    shift_tiles<1, NSL, 4>(Z, S);
+    f4 S[NB];
+    shift_tiles<1, NSL, NB>(Z, S);
     if (col == 0) S[0] = xin;
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
-        const f4 O = load_f4(tab, bs_O(Y, mb));
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            const f4 y = apply<NSL>(O, S[nb], Y0[mb][nb]);
+        for (int nb = 0; nb < NB; ++nb) {
+            const f4 y = apply<NSL>(Ob[mb], S[nb], Y0[mb][nb]);
             __builtin_nontemporal_store(
                 y, reinterpret_cast<f4 *>(a.out + base + (long)(16 * nb + col) * L + 16 * mb + 4 * g));
         }
@@ -305,71 +322,103 @@ void bs_kernel(LaunchArgs a)
         // chunks blockIdx.x + i * grid; the next chunk's input streams into
         // LDS while this one is scanned, looked back and written
         const uint32_t epoch = epoch_load(a.epoch);
+        epoch_arrive(a.arrive, epoch);
         const uint32_t pf = 2u * epoch + 1u, ff = pf + 1u;
         uint32_t c = blockIdx.x;
-        if (c < a.num_chunks) fetch_chunk(c);
-        for (; c < a.num_chunks; c += gridDim.x) {
+        if (c < a.num_chunks) {
+            // first chunk peeled: every entry into the loop below has this
+            // chunk's output stores behind the last fill
+            fetch_chunk(c);
             Raw cur;
-            read_chunk(cur);
+            read_chunk(cur, true);
+            if (c + gridDim.x < a.num_chunks) fetch_chunk(c + gridDim.x);
+            process(c, pf, ff, cur);
+        }
+        for (c += gridDim.x; c < a.num_chunks; c += gridDim.x) {
+            Raw cur;
+            read_chunk(cur, false);
             if (c + gridDim.x < a.num_chunks) fetch_chunk(c + gridDim.x);
             process(c, pf, ff, cur);
         }
         // the owner of the last chunk advances the epoch after publishing it
-        if (threadIdx.x == 0 && (a.num_chunks - 1u) % gridDim.x == blockIdx.x) epoch_store(a.epoch, epoch + 1u);
+        if ((a.num_chunks - 1u) % gridDim.x == blockIdx.x)
+            epoch_advance(a.epoch, a.arrive, epoch, gridDim.x, a.spin_limit, a.err);
     } else {
         const Ticket tk = take_ticket(a.ticket, a.num_chunks);
         Raw cur;
         fetch_chunk(tk.chunk);
-        read_chunk(cur);
+        read_chunk(cur, true);
         process(tk.chunk, tk.part_flag, tk.full_flag, cur);
     }
 }
 
-template <int L, int NSL>
-static hipError_t launch_ln(const LaunchArgs &a, hipStream_t s)
+template <int L, int G, int NSL>
+static hipError_t launch_lgn(const LaunchArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((bs_kernel<L, NSL>), dim3(a.grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((bs_kernel<L, G, NSL>), dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
-template <int L>
-static hipError_t launch_l(int nsl, const LaunchArgs &a, hipStream_t s)
+template <int L, int G>
+static hipError_t launch_lg(int nsl, const LaunchArgs &a, hipStream_t s)
 {
     switch (nsl) {
-    case 1: return launch_ln<L, 1>(a, s);
-    case 2: return launch_ln<L, 2>(a, s);
-    case 3: return launch_ln<L, 3>(a, s);
-    case 4: return launch_ln<L, 4>(a, s);
+    case 1: return launch_lgn<L, G, 1>(a, s);
+    case 2: return launch_lgn<L, G, 2>(a, s);
+    case 3: return launch_lgn<L, G, 3>(a, s);
+    case 4: return launch_lgn<L, G, 4>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template <int L>
-static const void *fn_l(int nsl)
+template <int L, int G>
+static const void *fn_lg(int nsl)
 {
     switch (nsl) {
-    case 1: return (const void *)bs_kernel<L, 1>;
-    case 2: return (const void *)bs_kernel<L, 2>;
-    case 3: return (const void *)bs_kernel<L, 3>;
-    case 4: return (const void *)bs_kernel<L, 4>;
+    case 1: return (const void *)bs_kernel<L, G, 1>;
+    case 2: return (const void *)bs_kernel<L, G, 2>;
+    case 3: return (const void *)bs_kernel<L, G, 3>;
+    case 4: return (const void *)bs_kernel<L, G, 4>;
     default: return nullptr;
     }
 }
 
-hipError_t launch_bs(int L, int ns, const LaunchArgs &a, hipStream_t s)
+template <int L>
+static hipError_t launch_l(int G, int nsl, const LaunchArgs &a, hipStream_t s)
 {
-    switch (L) {
-    case 32: return launch_l<32>(bs_slabs(ns), a, s);
-    case 64: return launch_l<64>(bs_slabs(ns), a, s);
+    switch (G) {
+    case 16: return launch_lg<L, 16>(nsl, a, s);
+    case 32: return launch_lg<L, 32>(nsl, a, s);
+    case 64: return launch_lg<L, 64>(nsl, a, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-const void *kernel_bs(int L, int ns)
+template <int L>
+static const void *fn_l(int G, int nsl)
+{
+    switch (G) {
+    case 16: return fn_lg<L, 16>(nsl);
+    case 32: return fn_lg<L, 32>(nsl);
+    case 64: return fn_lg<L, 64>(nsl);
+    default: return nullptr;
+    }
+}
+
+hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s)
 {
     switch (L) {
-    case 32: return fn_l<32>(bs_slabs(ns));
-    case 64: return fn_l<64>(bs_slabs(ns));
+    case 32: return launch_l<32>(G, bs_slabs(ns), a, s);
+    case 64: return launch_l<64>(G, bs_slabs(ns), a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+const void *kernel_bs(int L, int G, int ns)
+{
+    switch (L) {
+    case 32: return fn_l<32>(G, bs_slabs(ns));
+    case 64: return fn_l<64>(G, bs_slabs(ns));
     default: return nullptr;
     }
 }

@@ -118,6 +118,49 @@ __device__ __forceinline__ void epoch_store(uint32_t *p, uint32_t v)
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Persistent schedule: every wave announces, after reading the launch
+// epoch, that it has read it; the owner of the last chunk advances the epoch
+// only once every wave of the grid has arrived.  Without the arrival count a
+// workgroup that starts late (another XCD's dispatcher behind) could read the
+// next launch's epoch and wait on flags nobody writes.  The count is spread
+// over kArriveSlots words, 128 bytes apart, so that the adds do not queue on
+// one address.
+constexpr int kArriveSlots = 64;
+constexpr int kArriveStride = 32;  // words between slots
+
+__device__ __forceinline__ void epoch_arrive(uint32_t *arrive, uint32_t epoch)
+{
+    // the epoch value has returned before the add is issued (the asm reads
+    // it, so the load is waited for here, and memory operations stay behind
+    // it); a release would write back the whole L2 on every wave
+    asm volatile("" ::"v"(epoch) : "memory");
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        __hip_atomic_fetch_add(arrive + kArriveStride * (blockIdx.x % kArriveSlots), 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// called by one whole wave of the owner of the last chunk
+__device__ __forceinline__ void epoch_advance(uint32_t *epoch_word, uint32_t *arrive, uint32_t epoch, uint32_t waves,
+                                              uint32_t spin_limit, uint32_t *err)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    uint32_t *slot = arrive + kArriveStride * lane;
+    uint32_t spins = 0;
+    for (;;) {
+        uint32_t n = (lane < kArriveSlots) ? __hip_atomic_load(slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+#pragma unroll
+        for (int o = kWave / 2; o >= 1; o >>= 1) n += __shfl_xor(n, o, kWave);
+        if (n >= waves) break;
+        if (++spins > spin_limit) {
+            if (lane == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (lane < kArriveSlots) __hip_atomic_store(slot, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0) epoch_store(epoch_word, epoch + 1u);
+}
+
 __device__ __forceinline__ Ticket take_ticket(unsigned long long *ctr, uint32_t num_chunks)
 {
     __shared__ unsigned long long s_tk;
@@ -203,14 +246,14 @@ __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int se
     uint32_t spins = 0;
     for (;;) {
         if (st < 2) {
+            // the whole record in one round trip: partial words 0-1, full 2-3
             uint64_t *r = rec2(lb, (uint32_t)pred, sec, ns);
-            const uint64_t f0 = rec_load(r + 2), f1 = rec_load(r + 3);
+            const uint64_t q0 = rec_load(r), q1 = rec_load(r + 1), f0 = rec_load(r + 2), f1 = rec_load(r + 3);
             if (rec_flag(f0) == full_flag && rec_flag(f1) == full_flag) {
                 st = 2;
                 v2 = rec_val(f0);
                 v1 = rec_val(f1);
             } else if (st == 0) {
-                const uint64_t q0 = rec_load(r), q1 = rec_load(r + 1);
                 if (rec_flag(q0) == part_flag && rec_flag(q1) == part_flag) {
                     st = 1;
                     v2 = rec_val(q0);
@@ -257,8 +300,9 @@ __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int se
 // Lookback over K-word records ([K partial words][K full words] per chunk,
 // `rec_stride` words apart), for carries of K floats (PLR's direct-form
 // history, the block-state form's state vector).  Predecessors are scanned
-// one at a time with word j on lane j; a record is used whole, all-full or
-// all-partial, never mixed.  The fold X <- F X + p runs farthest first, with
+// one at a time, a whole record per load (partial word j on lane j, full word
+// j on lane K + j); a record is used whole, all-full or all-partial, never
+// mixed.  The fold X <- F X + p runs farthest first, with
 // row j of the chunk transition F (K x K, row-major in `F`) on lane j.
 // `lp` is LDS scratch of SLOTS*K floats: at most SLOTS partials are folded;
 // an unbounded lookback (depth 0) or one deeper than SLOTS takes a full
@@ -268,6 +312,7 @@ __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint
                                           uint32_t full_flag, const float *F, int depth, float *lp,
                                           uint32_t spin_limit, uint32_t *err)
 {
+    static_assert(2 * K <= kWave, "a record's two halves must fit on one wave");
     const int lane = threadIdx.x & (kWave - 1);
     const bool capped = depth <= 0 || depth > SLOTS;
     const int maxd = capped ? SLOTS : depth;
@@ -284,17 +329,19 @@ __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint
         }
         uint64_t *pr = lb + (size_t)pred * rec_stride;
         for (;;) {
-            const uint64_t f0 = (lane < K) ? rec_load(pr + K + lane) : 0ull;
-            if (__ballot(lane < K && rec_flag(f0) != full_flag) == 0) {
-                Xj = rec_val(f0);
+            // one round trip for the whole record: partial words on lanes
+            // [0, K), full words on lanes [K, 2K)
+            const uint64_t w = (lane < 2 * K) ? rec_load(pr + lane) : 0ull;
+            const float v = rec_val(w);
+            if (__ballot(lane >= K && lane < 2 * K && rec_flag(w) != full_flag) == 0) {
+                Xj = __shfl_down(v, K, kWave);  // full word j from lane j + K
                 D = d;
                 done = true;
                 break;
             }
             // the last slot of an unbounded fold must be a full record
-            const uint64_t p0 = (lane < K) ? rec_load(pr + lane) : 0ull;
-            if (__ballot(lane < K && rec_flag(p0) != part_flag) == 0 && !(capped && d == maxd - 1)) {
-                if (lane < K) lp[d * K + lane] = rec_val(p0);
+            if (__ballot(lane < K && rec_flag(w) != part_flag) == 0 && !(capped && d == maxd - 1)) {
+                if (lane < K) lp[d * K + lane] = v;
                 break;
             }
             if (++spins > spin_limit) {
@@ -326,16 +373,19 @@ __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint
 // workgroup owns under the plan's schedule (see the protocol notes above)
 // ------------------------------------------------------------------------
 template <typename Body>
-__device__ __forceinline__ void for_each_chunk(unsigned long long *ticket, uint32_t *epoch_word,
-                                               uint32_t num_chunks, int persistent, Body &&body)
+__device__ __forceinline__ void for_each_chunk(unsigned long long *ticket, uint32_t *epoch_word, uint32_t *arrive,
+                                               uint32_t num_chunks, int persistent, uint32_t spin_limit,
+                                               uint32_t *err, Body &&body)
 {
     if (persistent) {
         const uint32_t epoch = epoch_load(epoch_word);
+        epoch_arrive(arrive, epoch);
         const uint32_t pf = 2u * epoch + 1u, ff = pf + 1u;
         uint32_t c = blockIdx.x;
         for (; c < num_chunks; c += gridDim.x) body(c, pf, ff);
         // the owner of the last chunk advances the epoch after publishing it
-        if (threadIdx.x == 0 && (num_chunks - 1u) % gridDim.x == blockIdx.x) epoch_store(epoch_word, epoch + 1u);
+        if (threadIdx.x < kWave && (num_chunks - 1u) % gridDim.x == blockIdx.x)
+            epoch_advance(epoch_word, arrive, epoch, gridDim.x * ((blockDim.x + kWave - 1) / kWave), spin_limit, err);
     } else {
         const Ticket tk = take_ticket(ticket, num_chunks);
         body(tk.chunk, tk.part_flag, tk.full_flag);

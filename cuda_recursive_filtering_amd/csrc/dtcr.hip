@@ -36,7 +36,7 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
     const int tx = threadIdx.x & 63;
     const int ty = threadIdx.x >> 6;  // wave-uniform parity side
 
-    for_each_chunk(a.ticket, a.epoch, a.num_chunks, a.persistent,
+    for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err,
                    [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
     const long base = (long)chunk * (L * N);
     __syncthreads();  // previous chunk's LDS consumed

@@ -49,9 +49,11 @@ struct iirx_plan {
     uint64_t *d_lb;
     Shape shape;
     std::vector<double> plr_b, plr_a;  // direct-form taps supplied by the caller
-    unsigned long long *d_ticket;  // control block: [ticket u64][epoch u32][err u32]
+    unsigned long long *d_ticket;  // control block: [ticket u64][epoch u32][err u32], from byte 128
+                                   // the persistent schedule's arrival slots
     uint32_t *d_epoch;
     uint32_t *d_err;
+    uint32_t *d_arrive;
     uint32_t spin_limit;
     uint32_t grid;
     const void *fn;
@@ -81,9 +83,9 @@ int validate(const iirx_config *cfg, int64_t n)
         return IIRX_OK;
     }
     case IIRX_KERNEL_BLOCKSTATE: {
-        const int L = cfg->n_blocks;
-        if (!(L == 32 || L == 64)) return IIRX_ERR_GEOMETRY;
-        const int64_t chunk = 64LL * L;
+        const int L = cfg->n_blocks, G = cfg->block_size;
+        if (!(L == 32 || L == 64) || !(G == 16 || G == 32 || G == 64)) return IIRX_ERR_GEOMETRY;
+        const int64_t chunk = (int64_t)G * L;
         if (n % chunk != 0 || n / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
         return IIRX_OK;
     }
@@ -141,8 +143,8 @@ int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double
         if (sh.coef_stride < 0) return IIRX_ERR_CONDITIONING;
         break;
     case IIRX_KERNEL_BLOCKSTATE:
-        sh.coef_stride = bs_tables(sos, ns, cfg->n_blocks, tables);
-        sh.chunk = (int64_t)kBsTiles * cfg->n_blocks;
+        sh.coef_stride = bs_tables(sos, ns, cfg->n_blocks, cfg->block_size, tables);
+        sh.chunk = (int64_t)cfg->block_size * cfg->n_blocks;
         sh.num_chunks = (uint32_t)(n / sh.chunk);
         sh.lb_count = (size_t)sh.num_chunks * 32u;  // 16 partial + 16 full state words
         return IIRX_OK;
@@ -193,7 +195,7 @@ const void *kernel_fn(const iirx_config *cfg, const Shape &sh)
     case IIRX_KERNEL_STCR: return kernel_stcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_DTCR: return kernel_dtcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_PLR: return kernel_plr(2 * cfg->n_sections, sh.plr_block, sh.plr_x);
-    case IIRX_KERNEL_BLOCKSTATE: return kernel_bs(cfg->n_blocks, cfg->n_sections);
+    case IIRX_KERNEL_BLOCKSTATE: return kernel_bs(cfg->n_blocks, cfg->block_size, cfg->n_sections);
     default: return nullptr;
     }
 }
@@ -240,6 +242,7 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     p->d_ticket = nullptr;
     p->d_epoch = nullptr;
     p->d_err = nullptr;
+    p->d_arrive = nullptr;
     p->spin_limit = 1u << 22;
     p->launches = 0;
     hipGetDevice(&p->device);
@@ -268,14 +271,15 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     };
     if (hipMalloc((void **)&p->d_coef, p->host_coef.size() * sizeof(float)) != hipSuccess) return fail(IIRX_ERR_OOM);
     if (hipMalloc((void **)&p->d_lb, lb_count * sizeof(uint64_t)) != hipSuccess) return fail(IIRX_ERR_OOM);
-    if (hipMalloc((void **)&p->d_ticket, 16) != hipSuccess) return fail(IIRX_ERR_OOM);
+    if (hipMalloc((void **)&p->d_ticket, kControlBytes) != hipSuccess) return fail(IIRX_ERR_OOM);
     p->d_epoch = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(p->d_ticket) + 8);
     p->d_err = p->d_epoch + 1;
+    p->d_arrive = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(p->d_ticket) + 128);
     if (hipMemcpy(p->d_coef, p->host_coef.data(), p->host_coef.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
         return fail(IIRX_ERR_HIP);
     // zeroed once; launch-versioned flags never need a reset afterwards
     if (hipMemset(p->d_lb, 0, lb_count * sizeof(uint64_t)) != hipSuccess) return fail(IIRX_ERR_HIP);
-    if (hipMemset(p->d_ticket, 0, 16) != hipSuccess) return fail(IIRX_ERR_HIP);
+    if (hipMemset(p->d_ticket, 0, kControlBytes) != hipSuccess) return fail(IIRX_ERR_HIP);
     // grid: one workgroup per chunk (ticket) or per resident slot (persistent)
     p->fn = kernel_fn(cfg, sh);
     if (!p->fn) return fail(IIRX_ERR_GEOMETRY);
@@ -339,6 +343,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
     a.ticket = p->d_ticket;
     a.epoch = p->d_epoch;
     a.err = p->d_err;
+    a.arrive = p->d_arrive;
     a.n_samples = p->n_samples;
     a.num_chunks = p->num_chunks;
     a.spin_limit = p->spin_limit;
@@ -353,7 +358,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
     case IIRX_KERNEL_STCR: e = launch_stcr(p->lanes, p->n_blocks, a, s); break;
     case IIRX_KERNEL_DTCR: e = launch_dtcr(p->lanes, p->n_blocks, a, s); break;
     case IIRX_KERNEL_PLR: e = launch_plr(2 * p->cfg.n_sections, p->shape.plr_block, p->shape.plr_x, a, s); break;
-    case IIRX_KERNEL_BLOCKSTATE: e = launch_bs(p->n_blocks, p->cfg.n_sections, a, s); break;
+    case IIRX_KERNEL_BLOCKSTATE: e = launch_bs(p->n_blocks, p->cfg.block_size, p->cfg.n_sections, a, s); break;
     default: break;
     }
     if (e != hipSuccess) return IIRX_ERR_HIP;

@@ -16,6 +16,7 @@ struct LaunchArgs {
     unsigned long long *ticket;  // 64-bit workgroup ticket counter (ticket schedule)
     uint32_t *epoch;             // launch epoch (persistent schedule)
     uint32_t *err;               // watchdog flag
+    uint32_t *arrive;            // arrival slots: waves that have read the epoch (persistent schedule)
     int64_t n_samples;
     uint32_t num_chunks;         // chunks per launch
     uint32_t spin_limit;         // lookback polls before the watchdog trips
@@ -25,6 +26,10 @@ struct LaunchArgs {
     int32_t coef_stride;
     int32_t unrolled;
 };
+
+// plan control block: ticket, epoch, watchdog flag, then 64 arrival slots
+// 128 bytes apart
+constexpr size_t kControlBytes = 128 + 64 * 128;
 
 constexpr int ilog2_c(int v) { return v <= 1 ? 0 : 1 + ilog2_c(v >> 1); }
 constexpr int align4_c(int v) { return (v + 3) & ~3; }
@@ -170,13 +175,12 @@ namespace iirx {
 //   [kT]  T hi/lo x MB row blocks x KQ slabs   [kG] Gam hi/lo x KQ slabs
 //   [kP]  Phi^(2^j), j < 6      [kQ] Phi^(16 nb), nb = 1..3     [kPhi] Phi
 //   [kO]  O row blocks          [kScale] exponents of T and Gam
-//   [kMc] chunk transition Phi^64 (fp32 row major)   [kDepth] lookback depth
+//   [kMc] chunk transition Phi^G (fp32 row major)    [kDepth] lookback depth
 // ---------------------------------------------------------------------------
 struct BsLayout {
     int MB, KQ, kT, kG, kP, kQ, kPhi, kO, kScale, kMc, kDepth, stride;
 };
 
-constexpr int kBsTiles = 64;  // tiles per chunk (one per MFMA column over 4 blocks)
 constexpr int kBsBlockWords = 256;
 
 constexpr BsLayout bs_layout(int L)
@@ -206,7 +210,7 @@ constexpr int bs_O(const BsLayout &y, int mb) { return y.kO + mb * kBsBlockWords
 constexpr int bs_slabs(int ns) { return (ns + 1) / 2; }
 // scale slots: 0 T, 1 Gam
 
-hipError_t launch_bs(int L, int ns, const LaunchArgs &a, hipStream_t s);
-const void *kernel_bs(int L, int ns);
+hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s);
+const void *kernel_bs(int L, int G, int ns);
 
 }  // namespace iirx

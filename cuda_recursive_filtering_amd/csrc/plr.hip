@@ -40,7 +40,7 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
     const cfloat *ctab = as_const(a.coef);
     const float *fac = tab + Y.kFac;
 
-    for_each_chunk(a.ticket, a.epoch, a.num_chunks, a.persistent,
+    for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err,
                    [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
     const long cs = (long)chunk * CH;
     __syncthreads();

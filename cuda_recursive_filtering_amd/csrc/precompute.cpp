@@ -295,7 +295,7 @@ void put_f32(float *dst, const double *m, int r0)
 
 }  // namespace
 
-int bs_tables(const float *sos, int ns, int L, std::vector<float> &out)
+int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out)
 {
     const BsLayout Y = bs_layout(L);
     out.assign((size_t)Y.stride, 0.0f);
@@ -410,7 +410,7 @@ int bs_tables(const float *sos, int ns, int L, std::vector<float> &out)
     Q[1] = Phi16;
     Q[2] = mat_mul16(Phi16, Phi16);
     Q[3] = mat_mul16(Q[2], Phi16);
-    const Mat Mc = mat_mul16(P[5], P[5]);  // Phi^64
+    const Mat Mc = G == 16 ? mat_mul16(P[3], P[3]) : (G == 32 ? mat_mul16(P[4], P[4]) : mat_mul16(P[5], P[5]));  // Phi^G
 
     float *sc = out.data() + Y.kScale;
     const int sT = pow2_scale(Tm.data(), Tm.size()), sG = pow2_scale(Gm.data(), Gm.size());

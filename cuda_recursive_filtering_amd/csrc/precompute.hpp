@@ -47,6 +47,6 @@ int cr_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &ou
 int plr_tables(const double *b, const double *a, int K, int CH, std::vector<float> &out);
 
 // Block-state tables (layout kernels.hpp:bs_layout) for tiles of L samples.
-int bs_tables(const float *sos, int ns, int L, std::vector<float> &out);
+int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out);
 
 }  // namespace iirx

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(64) void stcr_kernel(LaunchArgs a)
     __shared__ float lds[L * (N + 1)];
 
     const int lane = threadIdx.x;
-    for_each_chunk(a.ticket, a.epoch, a.num_chunks, a.persistent,
+    for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err,
                    [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
     const long base = (long)chunk * (L * N);
 

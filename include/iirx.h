@@ -67,9 +67,11 @@ typedef enum {
     IIRX_KERNEL_PLR = 3,   /* direct-form hierarchical merge, plr_kernels.cuh */
     IIRX_KERNEL_BLOCKSTATE = 4  /* no counterpart in the reference: the whole
                                    cascade as a state-space block filter on the
-                                   matrix cores (FP16 hi/lo operands, FP32
-                                   accumulation); n_blocks = tile length L
-                                   (32 or 64), chunk = 64 tiles */
+                                   matrix cores (FP16 hi/lo operands for the
+                                   Toeplitz and input products, FP32 for the
+                                   tile-state scan); n_blocks = tile length L
+                                   (32 or 64), block_size = tiles per chunk G
+                                   (16, 32 or 64), chunk = G * L samples */
 } iirx_kernel;
 
 /*

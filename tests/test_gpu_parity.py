@@ -286,12 +286,13 @@ def err_rms(y, ref):
 
 
 @pytest.mark.parametrize("order", ORDERS + [6, 12])  # 1, 2, 3 and 4 FP32 K slabs
+@pytest.mark.parametrize("G", [16, 32, 64])
 @pytest.mark.parametrize("L", [32, 64])
 @pytest.mark.parametrize("wn", [0.1, 0.02])
-def test_blockstate_parity(device, order, L, wn):
-    print(f"case BS order={order} L={L} wn={wn}", flush=True)
+def test_blockstate_parity(device, order, G, L, wn):
+    print(f"case BS order={order} G={G} L={L} wn={wn}", flush=True)
     sos, g, x = normal_workload(order, 1 << 20, seed=3, wn=wn)
-    y, info = run_gpu(device, "BS", sos, x, 64, L)
+    y, info = run_gpu(device, "BS", sos, x, G, L)
     ref = O.golden(sos, x)
     ok, err, mref, rel = O.gate(y, ref)
     yp, _ = run_gpu(device, "PH", sos, x, 64, 32)
@@ -303,17 +304,18 @@ def test_blockstate_parity(device, order, L, wn):
 
 @pytest.mark.parametrize("schedule", ["persistent", "ticket"])
 @pytest.mark.parametrize("n_chunks", [1, 3, 70, 3000])
-def test_blockstate_grids_and_launches(device, n_chunks, schedule):
+@pytest.mark.parametrize("G", [16, 64])
+def test_blockstate_grids_and_launches(device, n_chunks, schedule, G):
     import torch
     L = 32
-    n = 64 * L * n_chunks
-    print(f"case BS grid chunks={n_chunks} {schedule}", flush=True)
+    n = G * L * n_chunks
+    print(f"case BS grid chunks={n_chunks} G={G} {schedule}", flush=True)
     sos, g, xa = normal_workload(8, n, seed=21, wn=0.1)
     _, _, xb = normal_workload(8, n, seed=22, wn=0.1)
     ra, rb = O.golden(sos, xa), O.golden(sos, xb)
     ta, tb = torch.from_numpy(xa).to(device), torch.from_numpy(xb).to(device)
     out = torch.empty_like(ta)
-    with iirx.Plan("BS", sos, n, n_blocks=L, schedule=schedule) as p:
+    with iirx.Plan("BS", sos, n, block_size=G, n_blocks=L, schedule=schedule) as p:
         for i in range(4):
             src, ref = (ta, ra) if i % 2 == 0 else (tb, rb)
             p.execute(src, out)

@@ -125,9 +125,9 @@ def test_config_check(kernel, B, N, n, ok):
     assert (st == 0) == ok, st
 
 
-def _bs_tables(sos, L):
+def _bs_tables(sos, L, G=64):
     """Dequantised block-state tables (layout of kernels.hpp:bs_layout)."""
-    tab, stride = iirx.plan_tables("BS", sos, 64 * L * 4, n_blocks=L)
+    tab, stride = iirx.plan_tables("BS", sos, G * L * 4, block_size=G, n_blocks=L)
     MB, KQ = L // 16, L // 32
     kT = 0
     kG = kT + 2 * MB * KQ * 256
@@ -178,21 +178,22 @@ def _bs_tables(sos, L):
 
 @pytest.mark.parametrize("order", [2, 8, 16])
 @pytest.mark.parametrize("L", [32, 64])
-def test_blockstate_tables_reproduce_the_cascade(order, L):
+@pytest.mark.parametrize("G", [16, 64])
+def test_blockstate_tables_reproduce_the_cascade(order, L, G):
     """The block-state tables (FP64 state space of the cascade, FP16 hi/lo
     operands) define the same filter: running y_t = T x_t + O s_t,
     s_{t+1} = Phi s_t + Gam x_t in float64 from the dequantised tables
     matches the FP64 cascade to the operand precision (2^-22 relative)."""
     from oracle_lib import golden
     sos, g = iirx.butterworth_sos(order, 0.1)
-    T, Om, Gm, P0, Phi, Mc, depth = _bs_tables(sos, L)
+    T, Om, Gm, P0, Phi, Mc, depth = _bs_tables(sos, L, G)
     assert np.array_equal(P0, Phi)
     # state s sits on row 4 (s // nsl) + s % nsl: rows with r % 4 >= nsl are empty
     nsl = (order // 2 + 1) // 2
     dead = [r for r in range(16) if r % 4 >= nsl]
     assert not Phi[dead, :].any() and not Phi[:, dead].any()
     assert not Gm[dead, :].any() and not Om[:, dead].any()
-    assert np.allclose(np.linalg.matrix_power(Phi, 64), Mc, rtol=1e-4, atol=1e-30 + 1e-6 * np.abs(Mc).max())
+    assert np.allclose(np.linalg.matrix_power(Phi, G), Mc, rtol=1e-4, atol=1e-30 + 1e-6 * np.abs(Mc).max())
     n = 64 * L * 3
     x = (g * iirx.fill_normal(4, n).astype(np.float64)).astype(np.float32)
     ref = golden(sos, x)
