@@ -24,7 +24,7 @@ from . import _lib
 from ._lib import IirxError, KERNELS, check, lib
 
 __all__ = [
-    "Plan", "IirxError", "KERNELS", "butterworth_sos", "direct_form_taps", "fill_normal",
+    "Plan", "Batch", "IirxError", "KERNELS", "butterworth_sos", "direct_form_taps", "fill_normal",
     "plan_tables", "config_check", "version",
 ]
 
@@ -196,6 +196,88 @@ class Plan:
     def close(self):
         if getattr(self, "_h", None) is not None:
             lib().iirx_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """Ragged batch: many channels of uneven length, each with its own order
+    and sections, filtered by one persistent launch (no counterpart in the
+    reference, which filters one channel per binary).
+
+    offsets      n_channels + 1 CSR offsets into the packed input/output
+    sections     list of (n_sections_c, 6) arrays, one per channel, or one
+                 (total_sections, 6) array with `n_sections` given
+    n_blocks     samples per lane (16, 32 or 64): chunks of 64 * n_blocks
+    schedule     'persistent' (default) or 'ticket'
+    """
+
+    def __init__(self, offsets, sections, n_sections=None, n_blocks=16, schedule="persistent"):
+        off = np.ascontiguousarray(np.asarray(offsets, np.int64))
+        if n_sections is None:
+            secs = [_sos_array(s) for s in sections]
+            ns = np.array([s.shape[0] for s in secs], np.int32)
+            sos = np.ascontiguousarray(np.concatenate(secs, axis=0), np.float32)
+        else:
+            ns = np.ascontiguousarray(np.asarray(n_sections, np.int32))
+            sos = np.ascontiguousarray(np.asarray(sections, np.float32).reshape(-1, 6))
+        if off.size != ns.size + 1:
+            raise ValueError("offsets must have one entry more than there are channels")
+        self.offsets = off
+        self.n_sections = ns
+        self.n_channels = int(ns.size)
+        self.n_samples = int(off[-1])
+        self._cfg = _lib.BatchConfig()
+        self._cfg.n_blocks = int(n_blocks)
+        self._cfg.schedule = _lib.SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
+        h = ctypes.c_void_p()
+        check(lib().iirx_batch_create(ctypes.byref(self._cfg), self.n_channels, off.ctypes.data, ns.ctypes.data,
+                                      sos.ctypes.data, ctypes.byref(h)), "batch_create")
+        self._h = h
+
+    def _check_len(self, *ts):
+        for t in ts:
+            if hasattr(t, "numel") and t.numel() < self.n_samples:
+                raise ValueError(f"tensor has {t.numel()} samples, the batch needs {self.n_samples}")
+
+    def execute(self, d_in, d_out, stream=None):
+        """One persistent launch over every channel."""
+        self._check_len(d_in, d_out)
+        check(lib().iirx_batch_execute(self._h, _ptr(d_in), _ptr(d_out), _stream_ptr(stream)), "batch_execute")
+
+    __call__ = execute
+
+    def capture(self, d_in, d_out, stream):
+        """Capture the launch for (d_in, d_out) into a hipGraph on `stream`
+        (a torch.cuda.Stream other than the null stream)."""
+        self._check_len(d_in, d_out)
+        check(lib().iirx_batch_capture(self._h, _ptr(d_in), _ptr(d_out), _stream_ptr(stream)), "batch_capture")
+
+    def replay(self, stream=None):
+        check(lib().iirx_batch_replay(self._h, _stream_ptr(stream)), "batch_replay")
+
+    def check(self):
+        check(lib().iirx_batch_check(self._h), "batch_check")
+
+    def info(self):
+        i = _lib.BatchInfo()
+        check(lib().iirx_batch_info_get(self._h, ctypes.byref(i)), "batch_info")
+        return {f: getattr(i, f) for f, _ in _lib.BatchInfo._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().iirx_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

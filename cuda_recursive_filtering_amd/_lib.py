@@ -54,6 +54,29 @@ class PlanInfo(ctypes.Structure):
     ]
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("n_blocks", ctypes.c_int32),
+        ("schedule", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
+    ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_channels", ctypes.c_int64),
+        ("n_samples", ctypes.c_int64),
+        ("num_chunks", ctypes.c_int64),
+        ("chunk_samples", ctypes.c_int64),
+        ("grid", ctypes.c_int64),
+        ("launches", ctypes.c_int64),
+        ("vgprs", ctypes.c_int32),
+        ("waves_per_simd", ctypes.c_int32),
+        ("graph_ready", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
 # every symbol include/iirx.h declares, with its ctypes signature
 _P = ctypes.c_void_p
 _SIGS = {
@@ -73,6 +96,14 @@ _SIGS = {
     "iirx_fill_normal": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, _P]),
     "iirx_plan_tables": (ctypes.c_int, [ctypes.POINTER(Config), _P, ctypes.c_int64, _P, ctypes.c_int64,
                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
+    "iirx_batch_create": (ctypes.c_int, [ctypes.POINTER(BatchConfig), ctypes.c_int64, _P, _P, _P,
+                                         ctypes.POINTER(_P)]),
+    "iirx_batch_execute": (ctypes.c_int, [_P, _P, _P, _P]),
+    "iirx_batch_capture": (ctypes.c_int, [_P, _P, _P, _P]),
+    "iirx_batch_replay": (ctypes.c_int, [_P, _P]),
+    "iirx_batch_check": (ctypes.c_int, [_P]),
+    "iirx_batch_info_get": (ctypes.c_int, [_P, ctypes.POINTER(BatchInfo)]),
+    "iirx_batch_destroy": (ctypes.c_int, [_P]),
 }
 
 _lib = None

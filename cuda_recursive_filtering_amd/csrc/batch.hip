@@ -1,0 +1,199 @@
+// batch.hip -- ragged batch: many independent channels of uneven length, each
+// with its own order and coefficients, in one persistent launch.
+//
+// No counterpart in the reference (one channel per binary there).  Every
+// channel is cut into chunks of 64 lanes x N samples, numbered channel after
+// channel; each chunk runs the PH algorithm of ph.hip (include/ph_kernels.cuh
+// PH_64, 501-820) on its channel's tables, with the lookback chain stopping at
+// the channel's first chunk (zero initial state per channel).  A channel's
+// last chunk is partial: samples past the channel end are read as zero (the
+// filter is causal, so they cannot reach earlier outputs) and not written.
+// Channel offsets are arbitrary (CSR packing), so HBM is accessed with
+// 4-byte coalesced loads and stores: 64 consecutive floats per instruction.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace iirx {
+
+typedef const __attribute__((address_space(4))) BatchChannel cchannel;
+typedef const __attribute__((address_space(4))) uint32_t cu32;
+
+template <int N>
+__global__ __launch_bounds__(64) void batch_ph_kernel(BatchArgs a)
+{
+    constexpr int L = 64;
+    constexpr int CH = L * N;
+    constexpr PhLayout Y = ph_layout(L, N);
+    __shared__ float lds[L * (N + 1)];
+
+    const int lane = threadIdx.x;
+    const cchannel *chan = (const cchannel *)a.chan;
+    const cu32 *chunk_chan = (const cu32 *)a.chunk_chan;
+
+    for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err,
+                   [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
+    // channel of this chunk (uniform: scalar loads)
+    const uint32_t ci = chunk_chan[chunk];
+    const long ch_off = chan[ci].offset, ch_len = chan[ci].length;
+    const int ns = chan[ci].ns;
+    const uint32_t first = chan[ci].first_chunk;
+    const float *coef = a.coef + chan[ci].coef_off;
+    const long j = (long)(chunk - first);
+    const long base = ch_off + j * CH;
+    const long rem = ch_len - j * CH;
+    const int r = rem < CH ? (int)rem : CH;  // valid samples in this chunk
+
+    // stage in: chunk element e = 64 i + lane (coalesced), lane t owns
+    // [t N, t N + N); past the channel end the address is clamped to its last
+    // sample (always in bounds, no branch) and the value replaced by zero
+    __syncthreads();  // LDS of the previous chunk fully consumed
+    {
+        const float *src = a.in + base;
+        float v[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const int e = i * L + lane;
+            v[i] = src[e < r ? e : r - 1];
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const int e = i * L + lane;
+            lds[(e / N) * (N + 1) + e % N] = (e < r) ? v[i] : 0.0f;
+        }
+    }
+    __syncthreads();
+
+    float x[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) x[n] = lds[lane * (N + 1) + n];
+
+    // inputs of the first section just before this lane's samples
+    float xi2, xi1;
+    if (lane > 0) {
+        xi2 = lds[(lane - 1) * (N + 1) + N - 2];
+        xi1 = lds[(lane - 1) * (N + 1) + N - 1];
+    } else if (j > 0) {
+        xi2 = a.in[base - 2];
+        xi1 = a.in[base - 1];
+    } else {
+        xi2 = 0.0f;
+        xi1 = 0.0f;
+    }
+
+    for (int sec = 0; sec < ns; ++sec) {
+        const float *cv = coef + (size_t)sec * a.coef_stride;
+        const cfloat *c = as_const(cv);
+        const float b1 = c[Y.kB + 0], b2 = c[Y.kB + 1], a1 = c[Y.kB + 2], a2 = c[Y.kB + 3];
+
+        // zero-state biquad over the lane's samples
+        {
+            float x2 = xi2, x1 = xi1, y2 = 0.0f, y1 = 0.0f;
+#pragma unroll
+            for (int n = 0; n < N; ++n) {
+                const float xc = x[n];
+                float yc = fmaf(b1, x1, fmaf(b2, x2, xc));
+                if (n == 1) yc = fmaf(a1, y1, yc);
+                else if (n > 1) yc = fmaf(a1, y1, fmaf(a2, y2, yc));
+                x[n] = yc;
+                x2 = x1;
+                x1 = xc;
+                y2 = y1;
+                y1 = yc;
+            }
+        }
+
+        // lane scan of the end states with C^(2^k)
+        float s2 = x[N - 2], s1 = x[N - 1];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int d = 1 << k;
+            const cfloat *P = c + Y.kCpow + 4 * k;
+            const float t2 = lane_up(s2, d), t1 = lane_up(s1, d);
+            if (lane >= d) {
+                s2 = fmaf(P[1], t1, fmaf(P[0], t2, s2));
+                s1 = fmaf(P[3], t1, fmaf(P[2], t2, s1));
+            }
+        }
+
+        // chunk partial, lookback within the channel, chunk full
+        const float m0 = c[Y.kM + 0], m1 = c[Y.kM + 1], m2 = c[Y.kM + 2], m3 = c[Y.kM + 3];
+        const float p2 = readlane_f(s2, L - 1), p1 = readlane_f(s1, L - 1);
+        uint64_t *rec = rec2(a.lb, chunk, sec, kBatchMaxSections);
+        if (lane == 0) publish2(rec, false, part_flag, p2, p1);
+        const int depth = (int)c[Y.kDepth];
+        const Carry2 X = lookback2(a.lb, chunk, sec, kBatchMaxSections, part_flag, full_flag, m0, m1, m2, m3, depth,
+                                   a.spin_limit, a.err, first);
+        if (lane == 0) {
+            const float f2 = fmaf(m1, X.y1, fmaf(m0, X.y2, p2));
+            const float f1 = fmaf(m3, X.y1, fmaf(m2, X.y2, p1));
+            publish2(rec, true, full_flag, f2, f1);
+        }
+
+        // per-lane end state with the chunk carry injected, C^(lane+1) X
+        const float4 cf = *reinterpret_cast<const float4 *>(cv + Y.kCfin + 4 * lane);
+        const float e2 = fmaf(cf.y, X.y1, fmaf(cf.x, X.y2, s2));
+        const float e1 = fmaf(cf.w, X.y1, fmaf(cf.z, X.y2, s1));
+        float u2 = lane_up(e2, 1), u1 = lane_up(e1, 1);
+        if (lane == 0) {
+            u2 = X.y2;
+            u1 = X.y1;
+        }
+
+        // back-correction by the zero-input response to the lane's incoming state
+#pragma unroll
+        for (int n = 0; n < N - 2; ++n) x[n] = fmaf(c[Y.kH1 + n], u1, fmaf(c[Y.kH2 + n], u2, x[n]));
+        x[N - 2] = e2;
+        x[N - 1] = e1;
+        xi2 = u2;
+        xi1 = u1;
+    }
+
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < N; ++n) lds[lane * (N + 1) + n] = x[n];
+    __syncthreads();
+    float *dst = a.out + base;
+    if (r == CH) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const int e = i * L + lane;
+            dst[e] = lds[(e / N) * (N + 1) + e % N];
+        }
+    } else {
+        // the channel's last chunk
+        for (int i = 0; i < N; ++i) {
+            const int e = i * L + lane;
+            if (e < r) dst[e] = lds[(e / N) * (N + 1) + e % N];
+        }
+    }
+    });
+}
+
+template <int N>
+static hipError_t launch_n(const BatchArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_ph_kernel<N>, dim3(a.grid), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(int n_blocks, const BatchArgs &a, hipStream_t s)
+{
+    switch (n_blocks) {
+    case 16: return launch_n<16>(a, s);
+    case 32: return launch_n<32>(a, s);
+    case 64: return launch_n<64>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+const void *kernel_batch(int n_blocks)
+{
+    switch (n_blocks) {
+    case 16: return (const void *)batch_ph_kernel<16>;
+    case 32: return (const void *)batch_ph_kernel<32>;
+    case 64: return (const void *)batch_ph_kernel<64>;
+    default: return nullptr;
+    }
+}
+
+}  // namespace iirx

@@ -231,7 +231,7 @@ __device__ __forceinline__ void publish2(uint64_t *r, bool full, uint32_t flag, 
 __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int sec, int ns,
                                             uint32_t part_flag, uint32_t full_flag,
                                             float m0, float m1, float m2, float m3, int depth,
-                                            uint32_t spin_limit, uint32_t *err)
+                                            uint32_t spin_limit, uint32_t *err, uint32_t first = 0)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const long pred = (long)chunk - 1 - lane;
@@ -240,7 +240,7 @@ __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int se
     const uint64_t below_depth = (depth >= kWave) ? ~0ull : ((1ull << depth) - 1ull);
     float v2 = 0.0f, v1 = 0.0f;
     // 0 not ready, 1 partial, 2 full, 3 before the signal start, 4 outside the window
-    int st = !active ? 4 : (pred < 0 ? 3 : 0);
+    int st = !active ? 4 : (pred < (long)first ? 3 : 0);
     int D = 0;
     bool from_full = false;
     uint32_t spins = 0;

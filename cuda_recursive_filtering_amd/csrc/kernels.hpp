@@ -210,6 +210,46 @@ constexpr int bs_O(const BsLayout &y, int mb) { return y.kO + mb * kBsBlockWords
 constexpr int bs_slabs(int ns) { return (ns + 1) / 2; }
 // scale slots: 0 T, 1 Gam
 
+// ---------------------------------------------------------------------------
+// Ragged batch (batch.hip): channel c holds samples [offset, offset + length)
+// of the packed input and output, chunks first_chunk .. first_chunk +
+// ceil(length / chunk) - 1 of the global chunk list, its PH tables at
+// coef + coef_off (ns sections, coef_stride floats each).  Lookback records
+// are laid out for kBatchMaxSections sections per chunk.
+// ---------------------------------------------------------------------------
+constexpr int kBatchMaxSections = 8;
+
+struct BatchChannel {
+    int64_t offset;
+    int64_t length;
+    int32_t ns;
+    int32_t coef_off;
+    uint32_t first_chunk;
+    uint32_t pad;
+};
+
+struct BatchArgs {
+    const float *in;
+    float *out;
+    const float *coef;
+    uint64_t *lb;
+    const BatchChannel *chan;
+    const uint32_t *chunk_chan;  // channel of every chunk
+    unsigned long long *ticket;
+    uint32_t *epoch;
+    uint32_t *err;
+    uint32_t *arrive;
+    uint32_t num_chunks;
+    uint32_t spin_limit;
+    int32_t persistent;
+    uint32_t grid;
+    int32_t coef_stride;
+    int32_t pad;
+};
+
+hipError_t launch_batch(int n_blocks, const BatchArgs &a, hipStream_t s);
+const void *kernel_batch(int n_blocks);
+
 hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s);
 const void *kernel_bs(int L, int G, int ns);
 

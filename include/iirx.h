@@ -232,6 +232,59 @@ int iirx_plan_tables(const iirx_config *cfg, const float *sos,
                      int64_t n_samples, float *out, int64_t cap,
                      int64_t *size, int32_t *stride);
 
+/* ------------------------------------------------------------------------
+ * Ragged batch: no counterpart in the reference, which filters one channel
+ * per binary (test_accuracy.cu:84-205).  Many independent channels, each
+ * with its own length, order and coefficients, packed back to back (CSR):
+ * channel c is samples [offsets[c], offsets[c+1]) of the input and of the
+ * output, filtered with sections sos[sec_off[c] .. sec_off[c] + n_sections[c])
+ * where sec_off is the running sum of n_sections (rows of 6 floats, the same
+ * contract as iirx_plan_create), zero initial state per channel.  One
+ * persistent launch filters every channel (PH per chunk of 64 x n_blocks
+ * samples, lookback chains per channel); a channel's last chunk may be
+ * partial.  The launch can be captured into a hipGraph and replayed: the
+ * lookback epoch lives on the device, so replays stay correct.
+ * ------------------------------------------------------------------------ */
+typedef struct iirx_batch iirx_batch;
+
+typedef struct {
+    int32_t n_blocks;   /* samples per lane: 16, 32 or 64 (0 = 16)          */
+    int32_t schedule;   /* iirx_schedule                                    */
+    int32_t reserved[2];
+} iirx_batch_config;
+
+typedef struct {
+    int64_t n_channels;
+    int64_t n_samples;      /* offsets[n_channels] - offsets[0] */
+    int64_t num_chunks;
+    int64_t chunk_samples;
+    int64_t grid;
+    int64_t launches;
+    int32_t vgprs, waves_per_simd;
+    int32_t graph_ready;    /* a captured graph can be replayed */
+    int32_t pad;
+} iirx_batch_info;
+
+/* Validate the channels and build the per-channel FP64 tables on the host,
+ * then upload them.  offsets: n_channels + 1 non-decreasing values, offsets[0]
+ * >= 0; n_sections: 1..8 per channel. */
+int iirx_batch_create(const iirx_batch_config *cfg, int64_t n_channels,
+                      const int64_t *offsets, const int32_t *n_sections,
+                      const float *sos, iirx_batch **out);
+/* One launch over every channel; d_in/d_out hold at least offsets[n_channels]
+ * floats and must not overlap. */
+int iirx_batch_execute(iirx_batch *b, const float *d_in, float *d_out,
+                       void *stream);
+/* Capture the launch for (d_in, d_out) into a hipGraph on `stream` (not the
+ * null stream) and instantiate it; iirx_batch_replay launches it. */
+int iirx_batch_capture(iirx_batch *b, const float *d_in, float *d_out,
+                       void *stream);
+int iirx_batch_replay(iirx_batch *b, void *stream);
+/* Synchronise and report the lookback watchdog, as iirx_plan_check. */
+int iirx_batch_check(iirx_batch *b);
+int iirx_batch_info_get(const iirx_batch *b, iirx_batch_info *info);
+int iirx_batch_destroy(iirx_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,158 @@
+"""Ragged batch (iirx_batch_*): many channels of uneven length, order and
+coefficients in one persistent launch.  No counterpart in the reference; every
+channel is checked against the FP64 cascade with the reference's accuracy
+gate (ref_generate.py golden + test_accuracy.cu tolerance), as a single
+channel is."""
+import numpy as np
+import pytest
+
+import cuda_recursive_filtering_amd as iirx
+import oracle_lib as O
+from workloads import ragged_channels, ragged_input
+
+
+def _explicit_channels():
+    """Edge lengths around the chunk sizes (64 x 16/32/64) and mixed orders."""
+    lens = [1, 2, 3, 63, 64, 65, 1023, 1024, 1025, 2047, 2048, 2049, 4095, 4096, 4097, 5000, 70001, 0, 12345]
+    orders = [2, 4, 8, 16]
+    wns = [0.02, 0.1, 0.3, 0.45]
+    offsets = np.zeros(len(lens) + 1, np.int64)
+    offsets[1:] = np.cumsum(lens)
+    sections, gains = [], []
+    for i in range(len(lens)):
+        s, g = iirx.butterworth_sos(orders[i % 4], wns[(i // 4) % 4])
+        sections.append(s)
+        gains.append(g)
+    return offsets, sections, np.asarray(gains)
+
+
+def _check_channels(y, x, offsets, sections, chans=None):
+    chans = range(len(sections)) if chans is None else chans
+    for c in chans:
+        a, b = int(offsets[c]), int(offsets[c + 1])
+        if b == a:
+            continue
+        ref = O.golden(sections[c], x[a:b])
+        ok, err, mref, rel = O.gate(y[a:b], ref)
+        assert ok, (c, b - a, sections[c].shape[0], rel)
+
+
+# --- host-side validation (no device needed: it is checked before any HIP call)
+
+def _create_status(offsets, ns, sos, n_blocks=16):
+    import ctypes
+    from cuda_recursive_filtering_amd import _lib
+    cfg = _lib.BatchConfig()
+    cfg.n_blocks = n_blocks
+    off = np.ascontiguousarray(offsets, np.int64)
+    nsa = np.ascontiguousarray(ns, np.int32)
+    s = np.ascontiguousarray(sos, np.float32)
+    h = ctypes.c_void_p()
+    st = _lib.lib().iirx_batch_create(ctypes.byref(cfg), nsa.size, off.ctypes.data, nsa.ctypes.data, s.ctypes.data,
+                                      ctypes.byref(h))
+    if st == 0:
+        _lib.lib().iirx_batch_destroy(h)
+    return st
+
+
+def test_batch_rejects_bad_channels():
+    sos, _ = iirx.butterworth_sos(4, 0.1)
+    good = np.concatenate([sos, sos])
+    assert _create_status([0, 10, 5], [2, 2], good) == 1          # decreasing offsets
+    assert _create_status([-1, 10, 20], [2, 2], good) == 1        # negative start
+    assert _create_status([0, 10, 20], [2, 9], np.zeros((11, 6))) == 1  # 9 sections
+    bad = good.copy()
+    bad[1, 4] = np.nan
+    assert _create_status([0, 10, 20], [2, 2], bad) == 1          # non-finite coefficient
+    assert _create_status([0, 10, 20], [2, 2], good, n_blocks=8) == 2  # no instance for N=8
+    assert _create_status([0, 0, 0], [2, 2], good) == 1           # no samples at all
+
+
+# --- GPU parity
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n_blocks", [16, 32, 64])
+@pytest.mark.parametrize("schedule", ["persistent", "ticket"])
+def test_batch_edge_lengths(device, n_blocks, schedule):
+    import torch
+    print(f"case batch edge lengths N={n_blocks} {schedule}", flush=True)
+    offsets, sections, gains = _explicit_channels()
+    x = ragged_input(offsets, gains, seed=40)
+    xin = torch.from_numpy(x).to(device)
+    out = torch.full_like(xin, float("nan"))
+    with iirx.Batch(offsets, sections, n_blocks=n_blocks, schedule=schedule) as b:
+        for _ in range(3):  # launch-versioned lookback: repeated launches on one batch
+            b.execute(xin, out)
+            b.check()
+        y = out.cpu().numpy()
+    _check_channels(y, x, offsets, sections)
+
+
+@pytest.mark.gpu
+def test_batch_ragged_workload(device):
+    """256 channels of the ragged workload's distribution (lengths 2^10..2^16)."""
+    import torch
+    print("case batch ragged 256 channels", flush=True)
+    offsets, sections, gains = ragged_channels(256, seed=4, hi_log2=16)
+    x = ragged_input(offsets, gains, seed=400)
+    xin = torch.from_numpy(x).to(device)
+    out = torch.empty_like(xin)
+    with iirx.Batch(offsets, sections) as b:
+        b.execute(xin, out)
+        b.check()
+        info = b.info()
+    assert info["n_channels"] == 256 and info["num_chunks"] > 256
+    _check_channels(out.cpu().numpy(), x, offsets, sections)
+
+
+@pytest.mark.gpu
+def test_batch_graph_replay(device):
+    """One captured launch replayed with new input contents each time; every
+    replay is checked, with eager work and synchronisation between replays."""
+    import torch
+    print("case batch graph replay", flush=True)
+    offsets, sections, gains = ragged_channels(64, seed=5, hi_log2=15)
+    xin = torch.empty(int(offsets[-1]), dtype=torch.float32, device=device)
+    out = torch.empty_like(xin)
+    s = torch.cuda.Stream()
+    with iirx.Batch(offsets, sections) as b:
+        b.capture(xin, out, s)
+        assert b.info()["graph_ready"] == 1
+        for r in range(4):
+            x = ragged_input(offsets, gains, seed=500 + 97 * r)
+            xin.copy_(torch.from_numpy(x))
+            torch.cuda.synchronize()
+            b.replay(s)
+            s.synchronize()
+            b.check()
+            _check_channels(out.cpu().numpy(), x, offsets, sections)
+            # eager launch between replays shares the lookback state
+            b.execute(xin, out)
+            b.check()
+            _check_channels(out.cpu().numpy(), x, offsets, sections, chans=range(0, 64, 7))
+
+
+@pytest.mark.gpu
+@pytest.mark.slow
+def test_batch_full_ragged_workload(device):
+    """The full ragged workload: 4096 channels, lengths 2^10..2^20 (~6.2e8
+    samples); 64 channels spread over the batch are checked against FP64."""
+    import torch
+    print("case batch full ragged workload", flush=True)
+    offsets, sections, gains = ragged_channels(4096, seed=4)
+    n = int(offsets[-1])
+    chans = list(range(0, 4096, 64))
+    xin = torch.randn(n, dtype=torch.float32, device=device)
+    g = torch.from_numpy(np.repeat(gains, np.diff(offsets)).astype(np.float32)).to(device)
+    xin.mul_(g)
+    del g
+    out = torch.empty_like(xin)
+    with iirx.Batch(offsets, sections) as b:
+        b.execute(xin, out)
+        b.check()
+    x_cpu = {c: xin[int(offsets[c]):int(offsets[c + 1])].cpu().numpy() for c in chans}
+    y_cpu = {c: out[int(offsets[c]):int(offsets[c + 1])].cpu().numpy() for c in chans}
+    for c in chans:
+        ref = O.golden(sections[c], x_cpu[c])
+        ok, err, mref, rel = O.gate(y_cpu[c], ref)
+        assert ok, (c, rel)

@@ -34,3 +34,30 @@ def ulp_scaled(a, b, ref):
     b = np.asarray(b, np.float64)
     m = float(np.max(np.abs(ref)))
     return float(np.max(np.abs(a - b))) / (np.finfo(np.float32).eps * m)
+
+
+def ragged_channels(n_channels, seed, lo_log2=10, hi_log2=20, orders=(2, 4, 8, 16), wn=(0.02, 0.45)):
+    """The ragged-batch workload's channel set: lengths log-uniform in
+    [2^lo, 2^hi], order uniform over `orders`, Butterworth Wn uniform in `wn`
+    (numpy default_rng(seed)).  Returns (offsets, sections, gains)."""
+    rng = np.random.default_rng(seed)
+    lens = np.floor(2.0 ** rng.uniform(lo_log2, hi_log2, n_channels)).astype(np.int64)
+    ords = rng.choice(np.asarray(orders), n_channels)
+    wns = rng.uniform(wn[0], wn[1], n_channels)
+    offsets = np.zeros(n_channels + 1, np.int64)
+    offsets[1:] = np.cumsum(lens)
+    sections, gains = [], []
+    for o, w in zip(ords, wns):
+        s, g = iirx.butterworth_sos(int(o), float(w))
+        sections.append(s)
+        gains.append(g)
+    return offsets, sections, np.asarray(gains)
+
+
+def ragged_input(offsets, gains, seed):
+    """Packed float32 input: channel c is gain_c * N(0,1) (generator seed + c)."""
+    x = np.empty(int(offsets[-1]), np.float32)
+    for c in range(len(gains)):
+        a, b = int(offsets[c]), int(offsets[c + 1])
+        x[a:b] = (gains[c] * iirx.fill_normal(seed + c, b - a).astype(np.float64)).astype(np.float32)
+    return x
