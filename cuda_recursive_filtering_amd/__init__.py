@@ -25,7 +25,7 @@ from ._lib import IirxError, KERNELS, check, lib
 
 __all__ = [
     "Plan", "Batch", "IirxError", "KERNELS", "butterworth_sos", "direct_form_taps", "fill_normal",
-    "plan_tables", "config_check", "version",
+    "plan_tables", "config_check", "version", "tune_candidates", "autotune",
 ]
 
 
@@ -103,6 +103,44 @@ def plan_tables(kernel, sos, n_samples, **kw):
     return out, stride.value
 
 
+_KERNEL_NAMES = {v: k for k, v in KERNELS.items()}
+
+
+def _cfg_dict(c):
+    return {"kernel": _KERNEL_NAMES.get(c.kernel, c.kernel), "block_size": c.block_size, "n_blocks": c.n_blocks,
+            "unrolled": bool(c.unrolled), "schedule": "ticket" if c.schedule == 1 else "persistent"}
+
+
+def tune_candidates(sos, n_samples, flags=0):
+    """Host-only candidate list (the autotuner's search space) with each
+    candidate's status: 0 eligible, 2 length does not fit the chunk, 3 CR
+    factors ill-conditioned for this filter."""
+    s = _sos_array(sos)
+    n = ctypes.c_int32()
+    check(lib().iirx_tune_candidates(s.ctypes.data, s.shape[0], int(n_samples), int(flags), None, 0,
+                                     ctypes.byref(n)), "tune_candidates")
+    arr = (_lib.TuneEntry * n.value)()
+    check(lib().iirx_tune_candidates(s.ctypes.data, s.shape[0], int(n_samples), int(flags), arr, n.value,
+                                     ctypes.byref(n)), "tune_candidates")
+    return [dict(_cfg_dict(e.cfg), status=e.status) for e in arr]
+
+
+def autotune(sos, n_samples, flags=0, reps=10):
+    """Time every eligible candidate on the device; returns (best config dict,
+    best ms per launch, table of every candidate with its status and ms)."""
+    s = _sos_array(sos)
+    n = ctypes.c_int32()
+    check(lib().iirx_tune_candidates(s.ctypes.data, s.shape[0], int(n_samples), int(flags), None, 0,
+                                     ctypes.byref(n)), "tune_candidates")
+    arr = (_lib.TuneEntry * n.value)()
+    best = _lib.Config()
+    ms = ctypes.c_float()
+    check(lib().iirx_autotune(s.ctypes.data, s.shape[0], int(n_samples), int(flags), int(reps), ctypes.byref(best),
+                              ctypes.byref(ms), arr, n.value, ctypes.byref(n)), "autotune")
+    table = [dict(_cfg_dict(e.cfg), status=e.status, ms=e.ms) for e in arr]
+    return _cfg_dict(best), ms.value, table
+
+
 def _ptr(t):
     if isinstance(t, int):
         return t
@@ -160,6 +198,13 @@ class Plan:
         if plr_taps is not None:
             b, a = (np.ascontiguousarray(np.asarray(v, np.float64)) for v in plr_taps)
             check(lib().iirx_plan_set_plr_taps(self._h, b.ctypes.data, a.ctypes.data, b.size), "set_plr_taps")
+
+    @classmethod
+    def autotuned(cls, sos, n_samples, flags=0, reps=10):
+        """A plan with the configuration the autotuner measures fastest."""
+        best, _, _ = autotune(sos, n_samples, flags, reps)
+        return cls(best["kernel"], sos, n_samples, block_size=best["block_size"], n_blocks=best["n_blocks"],
+                   unrolled=best["unrolled"], schedule=best["schedule"])
 
     def execute(self, d_in, d_out, stream=None):
         """One grid-wide launch: d_out = filter(d_in).  Tensors or raw device pointers."""

@@ -54,6 +54,20 @@ class PlanInfo(ctypes.Structure):
     ]
 
 
+TUNE_NO_BLOCKSTATE = 1
+TUNE_NO_CR = 2
+TUNE_PLR_ALL_ORDERS = 4
+TUNE_TICKET = 8
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [
+        ("cfg", Config),
+        ("status", ctypes.c_int32),
+        ("ms", ctypes.c_float),
+    ]
+
+
 class BatchConfig(ctypes.Structure):
     _fields_ = [
         ("n_blocks", ctypes.c_int32),
@@ -96,6 +110,11 @@ _SIGS = {
     "iirx_fill_normal": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, _P]),
     "iirx_plan_tables": (ctypes.c_int, [ctypes.POINTER(Config), _P, ctypes.c_int64, _P, ctypes.c_int64,
                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
+    "iirx_tune_candidates": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, _P, ctypes.c_int32,
+                                            ctypes.POINTER(ctypes.c_int32)]),
+    "iirx_autotune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32, ctypes.c_int32,
+                                     ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_float), _P, ctypes.c_int32,
+                                     ctypes.POINTER(ctypes.c_int32)]),
     "iirx_batch_create": (ctypes.c_int, [ctypes.POINTER(BatchConfig), ctypes.c_int64, _P, _P, _P,
                                          ctypes.POINTER(_P)]),
     "iirx_batch_execute": (ctypes.c_int, [_P, _P, _P, _P]),

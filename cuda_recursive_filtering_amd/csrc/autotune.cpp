@@ -1,0 +1,139 @@
+// autotune.cpp -- runtime algorithm and geometry selection.
+//
+// Replaces the reference's build-time search: run_accuracy.sh and
+// run_performance.sh sweep KERNEL x (BLOCK_SIZE, N_BLOCKS) per order and
+// calibrate PLR_X_MAX from ptxas spill reports (run_accuracy.sh:95-151).  Here
+// every candidate configuration is classified on the host (length fits the
+// chunk, CR factors well conditioned: the same checks a plan makes), then the
+// eligible ones are timed on the device and the fastest is returned.
+#include "../../include/iirx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+std::vector<iirx_config> enumerate(int32_t ns, uint32_t flags)
+{
+    std::vector<iirx_config> c;
+    const int32_t sched = (flags & IIRX_TUNE_TICKET) ? IIRX_SCHEDULE_TICKET : IIRX_SCHEDULE_PERSISTENT;
+    auto add = [&](int32_t kernel, int32_t B, int32_t N, int32_t unrolled) {
+        iirx_config x;
+        std::memset(&x, 0, sizeof x);
+        x.kernel = kernel;
+        x.block_size = B;
+        x.n_blocks = N;
+        x.n_sections = ns;
+        x.unrolled = unrolled;
+        x.schedule = sched;
+        c.push_back(x);
+    };
+    for (int N = 8; N <= 128; N *= 2) add(IIRX_KERNEL_PH, 64, N, 0);
+    if (!(flags & IIRX_TUNE_NO_CR))
+        for (int k : {IIRX_KERNEL_STCR, IIRX_KERNEL_DTCR})
+            for (int N = 8; N <= 128; N *= 2)
+                for (int u = 0; u < 2; ++u) add(k, 64, N, u);
+    if (!(flags & IIRX_TUNE_NO_BLOCKSTATE))
+        for (int G : {16, 32, 64})
+            for (int L : {32, 64}) add(IIRX_KERNEL_BLOCKSTATE, G, L, 0);
+    if (ns <= 2 || (flags & IIRX_TUNE_PLR_ALL_ORDERS)) add(IIRX_KERNEL_PLR, 0, 0, 0);
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int iirx_tune_candidates(const float *sos, int32_t n_sections, int64_t n_samples, uint32_t flags,
+                         iirx_tune_entry *entries, int32_t cap, int32_t *count)
+{
+    if (!sos || n_sections < 1 || n_sections > 8 || n_samples < 1) return IIRX_ERR_INVALID;
+    const std::vector<iirx_config> c = enumerate(n_sections, flags);
+    for (size_t i = 0; i < c.size(); ++i) {
+        if ((int32_t)i >= cap || !entries) break;
+        iirx_tune_entry &e = entries[i];
+        e.cfg = c[i];
+        int64_t size = 0;
+        int32_t stride = 0;
+        e.status = iirx_plan_tables(&c[i], sos, n_samples, nullptr, 0, &size, &stride);
+        e.ms = -1.0f;
+    }
+    if (count) *count = (int32_t)c.size();
+    return IIRX_OK;
+}
+
+int iirx_autotune(const float *sos, int32_t n_sections, int64_t n_samples, uint32_t flags, int32_t reps,
+                  iirx_config *best, float *best_ms, iirx_tune_entry *entries, int32_t cap, int32_t *count)
+{
+    if (!sos || !best || reps < 1) return IIRX_ERR_INVALID;
+    int32_t n = 0;
+    int st = iirx_tune_candidates(sos, n_sections, n_samples, flags, nullptr, 0, &n);
+    if (st != IIRX_OK) return st;
+    std::vector<iirx_tune_entry> t((size_t)n);
+    iirx_tune_candidates(sos, n_sections, n_samples, flags, t.data(), n, &n);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IIRX_ERR_NO_DEVICE;
+
+    // inputs rotate over enough buffers that a long signal streams from HBM
+    // instead of the 256 MB Infinity Cache (as in real use), at most 8
+    const size_t bytes = (size_t)n_samples * sizeof(float);
+    const size_t nbuf = std::min<size_t>(8, std::max<size_t>(1, (512u << 20) / bytes + 1));
+    float *d_in = nullptr, *d_out = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        if (s) hipStreamDestroy(s);
+        if (d_in) hipFree(d_in);
+        if (d_out) hipFree(d_out);
+    };
+    if (hipMalloc((void **)&d_in, nbuf * bytes) != hipSuccess || hipMalloc((void **)&d_out, bytes) != hipSuccess) {
+        cleanup();
+        return IIRX_ERR_OOM;
+    }
+    if (hipMemset(d_in, 0, nbuf * bytes) != hipSuccess || hipStreamCreate(&s) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        cleanup();
+        return IIRX_ERR_HIP;
+    }
+    int besti = -1;
+    for (int32_t i = 0; i < n; ++i) {
+        if (t[(size_t)i].status != IIRX_OK) continue;
+        iirx_plan *p = nullptr;
+        const int cs = iirx_plan_create(&t[(size_t)i].cfg, sos, n_samples, &p);
+        if (cs != IIRX_OK) {
+            t[(size_t)i].status = cs;
+            continue;
+        }
+        auto src = [&](int k) { return d_in + (size_t)(k % (int)nbuf) * (size_t)n_samples; };
+        int ok = IIRX_OK;
+        for (int w = 0; w < 3 && ok == IIRX_OK; ++w) ok = iirx_execute(p, src(w), d_out, s);
+        if (ok == IIRX_OK) ok = hipEventRecord(e0, s) == hipSuccess ? IIRX_OK : IIRX_ERR_HIP;
+        for (int r = 0; r < reps && ok == IIRX_OK; ++r) ok = iirx_execute(p, src(r), d_out, s);
+        if (ok == IIRX_OK) ok = hipEventRecord(e1, s) == hipSuccess ? IIRX_OK : IIRX_ERR_HIP;
+        if (ok == IIRX_OK) ok = hipEventSynchronize(e1) == hipSuccess ? IIRX_OK : IIRX_ERR_HIP;
+        if (ok == IIRX_OK) ok = iirx_plan_check(p);
+        float ms = 0.0f;
+        if (ok == IIRX_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+            t[(size_t)i].ms = ms / (float)reps;
+            if (besti < 0 || t[(size_t)i].ms < t[(size_t)besti].ms) besti = i;
+        } else {
+            t[(size_t)i].status = ok == IIRX_OK ? IIRX_ERR_HIP : ok;
+        }
+        iirx_plan_destroy(p);
+    }
+    cleanup();
+    if (entries)
+        for (int32_t i = 0; i < n && i < cap; ++i) entries[i] = t[(size_t)i];
+    if (count) *count = n;
+    if (besti < 0) return IIRX_ERR_GEOMETRY;
+    *best = t[(size_t)besti].cfg;
+    if (best_ms) *best_ms = t[(size_t)besti].ms;
+    return IIRX_OK;
+}
+
+}  // extern "C"

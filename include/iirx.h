@@ -233,6 +233,45 @@ int iirx_plan_tables(const iirx_config *cfg, const float *sos,
                      int64_t *size, int32_t *stride);
 
 /* ------------------------------------------------------------------------
+ * Autotuner: replaces the reference's fixed per-order geometry sweep and
+ * PLR_X_MAX calibration (run_accuracy.sh:95-151, run_performance.sh) with a
+ * runtime search.  Candidates: PH (N_BLOCKS 8..128), STCR and DTCR (loop and
+ * hand-unrolled, N_BLOCKS 8..128), the block-state form (G x L) and PLR
+ * (automatic geometry, spill-calibrated X; orders <= 4 unless
+ * IIRX_TUNE_PLR_ALL_ORDERS).  A candidate is excluded when the length does
+ * not fit its chunk (IIRX_ERR_GEOMETRY) or when its CR factors are
+ * ill-conditioned for the filter (IIRX_ERR_CONDITIONING: STCR/DTCR fall back
+ * to PH/BS).
+ * ------------------------------------------------------------------------ */
+enum {
+    IIRX_TUNE_NO_BLOCKSTATE = 1,   /* plain fp32 kernels only */
+    IIRX_TUNE_NO_CR = 2,           /* no STCR/DTCR candidates */
+    IIRX_TUNE_PLR_ALL_ORDERS = 4,  /* PLR also above order 4 (it fails the
+                                      accuracy gate at order 16) */
+    IIRX_TUNE_TICKET = 8           /* ticket schedule instead of persistent */
+};
+
+typedef struct {
+    iirx_config cfg;
+    int32_t status;   /* iirx_status of the candidate's tables; OK = eligible */
+    float ms;         /* mean ms per launch (iirx_autotune), -1 if not timed */
+} iirx_tune_entry;
+
+/* Host only: the candidate list for (sos, n_samples) with each candidate's
+ * eligibility.  Writes up to cap entries, *count = the full number. */
+int iirx_tune_candidates(const float *sos, int32_t n_sections, int64_t n_samples,
+                         uint32_t flags, iirx_tune_entry *entries, int32_t cap,
+                         int32_t *count);
+/* Time every eligible candidate (reps launches after 3 warm-up launches on
+ * zeroed device buffers of n_samples floats, rotating over up to 8 inputs so
+ * that long signals stream from HBM) and return the fastest in *best /
+ * *best_ms; entries (optional) receive the full table. */
+int iirx_autotune(const float *sos, int32_t n_sections, int64_t n_samples,
+                  uint32_t flags, int32_t reps, iirx_config *best,
+                  float *best_ms, iirx_tune_entry *entries, int32_t cap,
+                  int32_t *count);
+
+/* ------------------------------------------------------------------------
  * Ragged batch: no counterpart in the reference, which filters one channel
  * per binary (test_accuracy.cu:84-205).  Many independent channels, each
  * with its own length, order and coefficients, packed back to back (CSR):

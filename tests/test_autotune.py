@@ -1,0 +1,82 @@
+"""Autotuner: the runtime replacement of the reference's geometry sweep and
+PLR_X_MAX calibration (run_accuracy.sh:95-151).  Candidate classification is
+host logic (CPU tests); the timed search runs on the GPU."""
+import collections
+
+import numpy as np
+import pytest
+
+import cuda_recursive_filtering_amd as iirx
+import oracle_lib as O
+from workloads import normal_workload
+
+# a biquad with poles at +-0.9j: a1 = 0 makes the CR elimination divide by
+# e_0 = a1 (stcr_kernels.cuh setup), so STCR/DTCR must be excluded
+QUADRATURE = np.array([[1.0, 0.5, 0.25, 1.0, 0.0, 0.81]], np.float32)
+
+
+def _by_kernel(entries):
+    out = collections.defaultdict(list)
+    for e in entries:
+        out[e["kernel"]].append(e)
+    return out
+
+
+def test_candidates_cover_every_algorithm():
+    sos, _ = iirx.butterworth_sos(8, 0.1)
+    k = _by_kernel(iirx.tune_candidates(sos, 1 << 22))
+    assert len(k["PH"]) == 5 and len(k["STCR"]) == 10 and len(k["DTCR"]) == 10 and len(k["BS"]) == 6
+    assert "PLR" not in k  # the direct form is offered at orders <= 4 only
+    assert all(e["status"] == 0 for es in k.values() for e in es)
+    k2 = _by_kernel(iirx.tune_candidates(iirx.butterworth_sos(4, 0.1)[0], 1 << 22))
+    assert len(k2["PLR"]) == 1 and k2["PLR"][0]["status"] == 0
+
+
+def test_candidates_flags():
+    sos, _ = iirx.butterworth_sos(8, 0.1)
+    k = _by_kernel(iirx.tune_candidates(sos, 1 << 22, flags=iirx._lib.TUNE_NO_BLOCKSTATE | iirx._lib.TUNE_NO_CR))
+    assert set(k) == {"PH"}
+    k = _by_kernel(iirx.tune_candidates(sos, 1 << 22, flags=iirx._lib.TUNE_PLR_ALL_ORDERS))
+    assert "PLR" in k
+    k = _by_kernel(iirx.tune_candidates(sos, 1 << 22, flags=iirx._lib.TUNE_TICKET))
+    assert all(e["schedule"] == "ticket" for es in k.values() for e in es)
+
+
+def test_candidates_length_must_fit_the_chunk():
+    sos, _ = iirx.butterworth_sos(4, 0.1)
+    n = 64 * 8 * 3  # a multiple of the 64 x 8 chunk only
+    k = _by_kernel(iirx.tune_candidates(sos, n))
+    ph = {e["n_blocks"]: e["status"] for e in k["PH"]}
+    assert ph[8] == 0 and all(ph[N] == 2 for N in (16, 32, 64, 128))
+    assert k["PLR"][0]["status"] == 0  # PLR is tail-safe
+
+
+def test_candidates_exclude_ill_conditioned_cr():
+    k = _by_kernel(iirx.tune_candidates(QUADRATURE, 1 << 20))
+    assert all(e["status"] == 3 for e in k["STCR"] + k["DTCR"])
+    assert all(e["status"] == 0 for e in k["PH"] + k["BS"])
+
+
+@pytest.mark.gpu
+def test_autotune_picks_the_fastest_valid(device):
+    import torch
+    print("case autotune order 8", flush=True)
+    sos, g, x = normal_workload(8, 1 << 22, seed=7, wn=0.1)
+    best, ms, table = iirx.autotune(sos, x.size, reps=5)
+    timed = [e for e in table if e["status"] == 0 and e["ms"] > 0]
+    assert len(timed) >= 20
+    assert ms <= min(e["ms"] for e in timed) + 1e-6
+    with iirx.Plan.autotuned(sos, x.size, reps=5) as p:
+        xin = torch.from_numpy(x).to(device)
+        out = torch.empty_like(xin)
+        p.execute(xin, out)
+        p.check()
+        assert O.gate(out.cpu().numpy(), O.golden(sos, x))[0]
+
+
+@pytest.mark.gpu
+def test_autotune_falls_back_from_cr(device):
+    print("case autotune ill-conditioned CR", flush=True)
+    best, ms, table = iirx.autotune(QUADRATURE, 1 << 20, reps=3)
+    assert best["kernel"] in ("PH", "BS")
+    assert all(e["ms"] < 0 for e in table if e["kernel"] in ("STCR", "DTCR"))
