@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--check", action="store_true", help="gate the last output against the FP64 oracle")
+    ap.add_argument("--bs-geom", default="64x64",
+                    help="block-state form geometry GxL for the MFMA vs plain fp32 comparison (orders 8 and 16)")
+    ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA vs plain fp32 comparison")
     ap.add_argument("--workload", default="single", choices=["single", "ragged"],
                     help="single: one 2^25-sample channel per order; ragged: the 4096-channel batch")
     ap.add_argument("--batch-n", type=int, default=16, help="ragged batch: samples per lane (16, 32 or 64)")
@@ -48,13 +51,14 @@ def parse():
     return ap.parse_args()
 
 
-def time_order(iirx, torch, order, args, bufs, out, stream):
-    B, N = (int(v) for v in args.geom.lower().split("x"))
+def time_order(iirx, torch, order, args, bufs, out, stream, kernel=None, geom=None):
+    kernel = kernel or args.kernel
+    B, N = (int(v) for v in (geom or args.geom).lower().split("x"))
     sos, g = iirx.butterworth_sos(order, args.wn)
-    if args.kernel.upper() == "PLR":  # --geom BLOCKxX, 0 = automatic
+    if kernel.upper() == "PLR":  # --geom BLOCKxX, 0 = automatic
         plan = iirx.Plan("PLR", sos, N_SAMPLES, block_size=B, plr_x=N, schedule=args.schedule)
     else:
-        plan = iirx.Plan(args.kernel, sos, N_SAMPLES, block_size=B, n_blocks=N, schedule=args.schedule)
+        plan = iirx.Plan(kernel, sos, N_SAMPLES, block_size=B, n_blocks=N, schedule=args.schedule)
     for i in range(args.warmup):
         plan.execute(bufs[i % len(bufs)], out, stream)
     torch.cuda.synchronize()
@@ -216,7 +220,7 @@ def main():
     nbuf = max(4, math.ceil(2 * MALL_BYTES / (4 * N_SAMPLES)) + 1)
     base = iirx.fill_normal(1, N_SAMPLES)
     orders = [int(o) for o in args.orders.split(",")]
-    per = {}
+    per, mfma = {}, {}
     for order in orders:
         sos, g = iirx.butterworth_sos(order, args.wn)
         host = (g * base.astype(np.float64)).astype(np.float32)
@@ -237,6 +241,28 @@ def main():
             ok, err, mref, rel = O.gate(y, O.golden(sos, host))
             entry["gate_rel"] = float(rel)
             entry["gate_pass"] = bool(ok)
+        if not args.no_mfma and order in (8, 16):
+            # MFMA block-state form beside the plain fp32 kernel, same input, error vs FP64
+            sys.path.insert(0, os.path.join(HERE, "tests"))
+            import oracle_lib as O
+            ref = O.golden(sos, host)
+            rms = float(np.sqrt(np.mean(ref ** 2)))
+            last = bufs[r["last_input"]]
+            def errs(plan):
+                plan.execute(last, out, stream)
+                torch.cuda.synchronize()
+                d = out.cpu().numpy().astype(np.float64) - ref
+                return float(np.max(np.abs(d))) / rms, float(np.sqrt(np.mean(d ** 2))) / rms
+            e_fp32, r_fp32 = errs(r["plan"])
+            rb = time_order(iirx, torch, order, args, bufs, out, stream, kernel="BS", geom=args.bs_geom)
+            e_bs, r_bs = errs(rb["plan"])
+            rb["plan"].close()
+            mfma[str(order)] = {
+                "bs_gsamples_per_s": round(N_SAMPLES / (rb["ms"] * 1e-3) / 1e9, 3), "bs_geometry": args.bs_geom,
+                "fp32_gsamples_per_s": entry["gsamples_per_s"], "fp32_kernel": f"{args.kernel} {args.geom}",
+                "bs_max_err_over_rms": e_bs, "fp32_max_err_over_rms": e_fp32,
+                "bs_rms_err_over_rms": r_bs, "fp32_rms_err_over_rms": r_fp32}
+            print(f"# mfma order {order}: {mfma[str(order)]}", file=sys.stderr, flush=True)
         per[str(order)] = entry
         r["plan"].close()
         del bufs, src, out
@@ -268,6 +294,7 @@ def main():
                      "frac": o2["roof_frac"], "traffic": measured_traffic(args),
                      "algorithmic_bytes": BYTES_PER_SAMPLE * N_SAMPLES},
         "copy_ceiling": ceiling,
+        "mfma_vs_fp32": mfma or None,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(iirx, args)
