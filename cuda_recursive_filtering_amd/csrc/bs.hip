@@ -35,11 +35,6 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ f4 mfma_h(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ f4 mfma_f(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
-__device__ __forceinline__ h8 load_h8(const float *tab, int word_off)
-{
-    return __builtin_bit_cast(h8, reinterpret_cast<const uint4 *>(tab + word_off)[threadIdx.x & 63]);
-}
-
 __device__ __forceinline__ f4 load_f4(const float *tab, int word_off)
 {
     return reinterpret_cast<const f4 *>(tab + word_off)[threadIdx.x & 63];
@@ -120,36 +115,54 @@ __device__ __forceinline__ void shift_tiles(const f4 (&Z)[NB], f4 (&S)[NB])
     }
 }
 
-// waves per SIMD the register budget is set for; the LDS (the chunk's input
-// buffer, G L floats, plus 16 lookback slots) allows as many
-template <int L, int G>
-constexpr int bs_waves()
-{
-    return G == 64 ? (L == 32 ? 3 : 2) : G == 32 ? (L == 32 ? 5 : 3) : (L == 32 ? 6 : 4);
-}
-
 template <int G>
 constexpr int bs_levels() { return G == 16 ? 4 : G == 32 ? 5 : 6; }
 
 template <int L, int G, int NSL>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(bs_waves<L, G>(), bs_waves<L, G>())))
-void bs_kernel(LaunchArgs a)
+__global__ __launch_bounds__(64 * bs_wg_waves(L, G))
+__attribute__((amdgpu_waves_per_eu(bs_waves(L, G), bs_waves(L, G)))) void bs_kernel(LaunchArgs a)
 {
     constexpr int MB = L / 16;  // output row blocks of T
     constexpr int KQ = L / 32;  // K slabs of 32 samples
     constexpr int NB = G / 16;  // column blocks of 16 tiles
     constexpr int CH = L * G;
     constexpr int NLD = NB * KQ * 2;  // 16-byte loads per lane and chunk
+    constexpr int W = bs_wg_waves(L, G);
+    constexpr int NT = bs_tidx(MB);   // nonzero blocks of T
     constexpr BsLayout Y = bs_layout(L);
     constexpr int kSlots = 16;
-    __shared__ float lp[kSlots * 16];
-    __shared__ f4 xbuf[NLD * 64];  // one chunk of X, load i of lane l at xbuf[64 i + l]
+    // shared by the workgroup: the operands consumed before the lookback.  A
+    // VMEM table load issued after the next chunk's fill would wait for the
+    // whole fill (vmcnt retires in issue order); LDS reads do not.
+    __shared__ uint4 lT[2 * NT * 64];  // T hi/lo, nonzero blocks (mb, q <= bs_tq(mb))
+    __shared__ uint4 lG[2 * KQ * 64];  // Gam hi/lo
+    __shared__ f4 lP[6 * 64];          // Phi^(2^j), FP32 operand blocks
+    // per wave: one chunk of X (load i of lane l at [64 i + l]) and the lookback slots
+    __shared__ f4 xbuf_w[W][NLD * 64];
+    __shared__ float lp_w[W][kSlots * 16];
 
-    const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
     const float *tab = a.coef;
     const cfloat *ctab = as_const(a.coef);
     const int sT = (int)ctab[Y.kScale + 0], sG = (int)ctab[Y.kScale + 1];
     const int depth = (int)ctab[Y.kDepth];
+    f4 *xbuf = xbuf_w[wave];
+    float *lp = lp_w[wave];
+
+    for (int i = threadIdx.x; i < 2 * NT * 64; i += 64 * W) {
+        const int t = (i >> 6) % NT, hl = (i >> 6) / NT;
+        int mb = 0;
+        while (bs_tidx(mb + 1) <= t) ++mb;
+        lT[i] = reinterpret_cast<const uint4 *>(tab + bs_T(Y, hl, mb, t - bs_tidx(mb)))[i & 63];
+    }
+    for (int i = threadIdx.x; i < 2 * KQ * 64; i += 64 * W)
+        lG[i] = reinterpret_cast<const uint4 *>(tab + bs_G(Y, (i >> 6) / KQ, (i >> 6) % KQ))[i & 63];
+    for (int i = threadIdx.x; i < 6 * 64; i += 64 * W)
+        lP[i] = reinterpret_cast<const f4 *>(tab + bs_P(Y, i >> 6))[i & 63];
+    __syncthreads();
+    auto opT = [&](int hl, int mb, int q) { return __builtin_bit_cast(h8, lT[(hl * NT + bs_tidx(mb) + q) * 64 + lane]); };
+    auto opG = [&](int hl, int q) { return __builtin_bit_cast(h8, lG[(hl * KQ + q) * 64 + lane]); };
+    auto opP = [&](int j) { return lP[j * 64 + lane]; };
 
     // X as x32 B operands: tile 16 nb + col, K slab q, K rows 8 g + j =
     // samples 32 q + 4 g + j (a) and 32 q + 16 + 4 g + j - 4 (b).  The chunk
@@ -218,7 +231,7 @@ void bs_kernel(LaunchArgs a)
         f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int q = 0; q < KQ; ++q) {
-            const h8 gh = load_h8(tab, bs_G(Y, 0, q)), gl = load_h8(tab, bs_G(Y, 1, q));
+            const h8 gh = opG(0, q), gl = opG(1, q);
             acc = mfma_h(gl, xh[nb][q], acc);
             acc = mfma_h(gh, xl[nb][q], acc);
             acc = mfma_h(gh, xh[nb][q], acc);
@@ -229,14 +242,14 @@ void bs_kernel(LaunchArgs a)
     f4 Y0[MB][NB];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
-        const int qmax = (16 * mb + 15) / 32;  // T(16 mb + r, 32 q + c) = 0 beyond
+        const int qmax = bs_tq(mb);  // T(16 mb + r, 32 q + c) = 0 beyond
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int q = 0; q < KQ; ++q) {
                 if (q > qmax) break;
-                const h8 th = load_h8(tab, bs_T(Y, 0, mb, q)), tl = load_h8(tab, bs_T(Y, 1, mb, q));
+                const h8 th = opT(0, mb, q), tl = opT(1, mb, q);
                 acc = mfma_h(tl, xh[nb][q], acc);
                 acc = mfma_h(th, xl[nb][q], acc);
                 acc = mfma_h(th, xh[nb][q], acc);
@@ -252,7 +265,7 @@ void bs_kernel(LaunchArgs a)
         constexpr int D = 1 << decltype(LV)::value;
         f4 S[NB];
         shift_tiles<D, NSL, NB>(Z, S);
-        const f4 P = load_f4(tab, bs_P(Y, decltype(LV)::value));
+        const f4 P = opP(decltype(LV)::value);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
             if (D < 16 || nb >= (D >> 4)) Z[nb] = apply<NSL>(P, S[nb], Z[nb]);
@@ -290,7 +303,7 @@ void bs_kernel(LaunchArgs a)
         constexpr int D = 1 << decltype(LV)::value;
         f4 S[1];
         shift_tiles<D, NSL, 1>(U, S);
-        U[0] = apply<NSL>(load_f4(tab, bs_P(Y, decltype(LV)::value)), S[0], U[0]);
+        U[0] = apply<NSL>(opP(decltype(LV)::value), S[0], U[0]);
     });
 #pragma unroll
     for (int nb = 1; nb < NB; ++nb) Z[nb] = apply<NSL>(load_f4(tab, bs_Q(Y, nb)), U[0], Z[nb]);
@@ -321,44 +334,52 @@ void bs_kernel(LaunchArgs a)
     }
     };
 
+    const uint32_t GW = gridDim.x * W, gw = blockIdx.x * W + wave;  // this wave among the grid's
     if (a.persistent) {
-        // chunks blockIdx.x + i * grid; the next chunk's input streams into
-        // LDS while this one is scanned, looked back and written
+        // chunks gw + i * GW; the next chunk's input streams into LDS while
+        // this one is scanned, looked back and written
         const uint32_t epoch = epoch_load(a.epoch);
         epoch_arrive(a.arrive, epoch);
         const uint32_t pf = 2u * epoch + 1u, ff = pf + 1u;
-        uint32_t c = blockIdx.x;
+        uint32_t c = gw;
         if (c < a.num_chunks) {
             // first chunk peeled: every entry into the loop below has this
             // chunk's output stores behind the last fill
             fetch_chunk(c);
             Raw cur;
             read_chunk(cur, true);
-            if (c + gridDim.x < a.num_chunks) fetch_chunk(c + gridDim.x);
+            if (c + GW < a.num_chunks) fetch_chunk(c + GW);
             process(c, pf, ff, cur);
         }
-        for (c += gridDim.x; c < a.num_chunks; c += gridDim.x) {
+        for (c += GW; c < a.num_chunks; c += GW) {
             Raw cur;
             read_chunk(cur, false);
-            if (c + gridDim.x < a.num_chunks) fetch_chunk(c + gridDim.x);
+            if (c + GW < a.num_chunks) fetch_chunk(c + GW);
             process(c, pf, ff, cur);
         }
         // the owner of the last chunk advances the epoch after publishing it
-        if ((a.num_chunks - 1u) % gridDim.x == blockIdx.x)
-            epoch_advance(a.epoch, a.arrive, epoch, gridDim.x, a.spin_limit, a.err);
-    } else {
-        const Ticket tk = take_ticket(a.ticket, a.num_chunks);
+        if ((a.num_chunks - 1u) % GW == gw) epoch_advance(a.epoch, a.arrive, epoch, GW, a.spin_limit, a.err);
+    } else if (gw < a.num_chunks) {
+        // one ticket per wave, exactly num_chunks per launch (grid = ceil(chunks / W))
+        unsigned long long tk = 0;
+        if (lane == 0) tk = atomicAdd(a.ticket, 1ull);
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)tk);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(tk >> 32));
+        tk = ((unsigned long long)hi << 32) | lo;
+        const unsigned long long ep = tk / a.num_chunks;
+        const uint32_t chunk = (uint32_t)(tk - ep * a.num_chunks);
+        const uint32_t pf = (uint32_t)(2ull * ep + 1ull);
         Raw cur;
-        fetch_chunk(tk.chunk);
+        fetch_chunk(chunk);
         read_chunk(cur, true);
-        process(tk.chunk, tk.part_flag, tk.full_flag, cur);
+        process(chunk, pf, pf + 1u, cur);
     }
 }
 
 template <int L, int G, int NSL>
 static hipError_t launch_lgn(const LaunchArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((bs_kernel<L, G, NSL>), dim3(a.grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((bs_kernel<L, G, NSL>), dim3(a.grid), dim3(64 * bs_wg_waves(L, G)), 0, s, a);
     return hipGetLastError();
 }
 

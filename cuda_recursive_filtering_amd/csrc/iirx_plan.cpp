@@ -25,6 +25,7 @@ namespace {
 // per-kernel launch shape and tables of a configuration
 struct Shape {
     int lanes, n_blocks, threads, plr_block, plr_x;  // plr_*: threads per block, values per thread
+    int chunks_per_block;                             // waves with their own chunk stream (block-state form)
     int64_t chunk;
     uint32_t num_chunks;
     size_t lb_count;
@@ -131,6 +132,7 @@ int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double
     sh.lanes = 64;
     sh.n_blocks = cfg->n_blocks;
     sh.threads = 64;
+    sh.chunks_per_block = 1;
     sh.plr_block = sh.plr_x = 0;
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH:
@@ -144,6 +146,8 @@ int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double
         break;
     case IIRX_KERNEL_BLOCKSTATE:
         sh.coef_stride = bs_tables(sos, ns, cfg->n_blocks, cfg->block_size, tables);
+        sh.chunks_per_block = bs_wg_waves(cfg->n_blocks, cfg->block_size);
+        sh.threads = 64 * sh.chunks_per_block;
         sh.chunk = (int64_t)cfg->block_size * cfg->n_blocks;
         sh.num_chunks = (uint32_t)(n / sh.chunk);
         sh.lb_count = (size_t)sh.num_chunks * 32u;  // 16 partial + 16 full state words
@@ -283,7 +287,10 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     // grid: one workgroup per chunk (ticket) or per resident slot (persistent)
     p->fn = kernel_fn(cfg, sh);
     if (!p->fn) return fail(IIRX_ERR_GEOMETRY);
-    p->grid = p->num_chunks;
+    // workgroups: one per chunk (ticket) or one per resident slot (persistent);
+    // a block-state workgroup runs chunks_per_block waves, each its own chunks
+    const int64_t blocks_needed = ((int64_t)p->num_chunks + sh.chunks_per_block - 1) / sh.chunks_per_block;
+    p->grid = (uint32_t)blocks_needed;
     if (cfg->schedule == IIRX_SCHEDULE_PERSISTENT) {
         int per_cu = 0, cus = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->fn, p->threads, 0) != hipSuccess)
@@ -291,7 +298,7 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess)
             return fail(IIRX_ERR_HIP);
         const int64_t slots = (int64_t)std::max(per_cu, 1) * std::max(cus, 1);
-        p->grid = (uint32_t)std::min<int64_t>(slots, p->num_chunks);
+        p->grid = (uint32_t)std::min<int64_t>(slots, blocks_needed);
     }
     if (hipDeviceSynchronize() != hipSuccess) return fail(IIRX_ERR_HIP);
     *out = p;

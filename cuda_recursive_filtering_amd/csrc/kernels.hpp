@@ -208,6 +208,26 @@ constexpr int bs_P(const BsLayout &y, int j) { return y.kP + j * kBsBlockWords; 
 constexpr int bs_Q(const BsLayout &y, int nb) { return y.kQ + (nb - 1) * kBsBlockWords; }
 constexpr int bs_O(const BsLayout &y, int mb) { return y.kO + mb * kBsBlockWords; }
 constexpr int bs_slabs(int ns) { return (ns + 1) / 2; }
+// last nonzero K slab of T's row block mb, and the compact index of (mb, 0)
+// among T's nonzero blocks (blocks above the diagonal are zero)
+constexpr int bs_tq(int mb) { return (16 * mb + 15) / 32; }
+constexpr int bs_tidx(int mb)
+{
+    int s = 0;
+    for (int m = 0; m < mb; ++m) s += bs_tq(m) + 1;
+    return s;
+}
+// waves per SIMD the register budget is set for, and waves per workgroup
+// (the workgroup shares T, Gam and Phi^(2^j) in LDS; each wave runs its own
+// chunks with its own input buffer of G L floats and 16 lookback slots)
+constexpr int bs_waves(int L, int G)
+{
+    return G == 64 ? (L == 32 ? 3 : 2) : G == 32 ? (L == 32 ? 5 : 3) : (L == 32 ? 6 : 4);
+}
+constexpr int bs_wg_waves(int L, int G)
+{
+    return G == 64 ? (L == 32 ? 12 : 8) : G == 32 ? (L == 32 ? 10 : 12) : (L == 32 ? 12 : 16);
+}
 // scale slots: 0 T, 1 Gam
 
 // ---------------------------------------------------------------------------
