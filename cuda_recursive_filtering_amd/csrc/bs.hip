@@ -224,7 +224,7 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G), bs_waves(L, G)))) void bs_ker
             for (int q = 0; q < KQ; ++q) split8(xr.a[nb][q], xr.b[nb][q], eX, xh[nb][q], xl[nb][q]);
     }
 
-    // 2. V = Gam X (zero chunk-incoming state) and Y0 = T X
+    // 2. V = Gam X (zero chunk-incoming state); Y0 = T X follows the partial
     f4 Z[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
@@ -239,6 +239,26 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G), bs_waves(L, G)))) void bs_ker
 #pragma unroll
         for (int j = 0; j < 4; ++j) Z[nb][j] = ldexpf(acc[j], -(sG + eX));
     }
+    // 3. tile end states, log2(G)-level scan (levels D >= 16 skip the
+    //    all-zero blocks)
+    static_for<0, bs_levels<G>(), 1>([&](auto LV) {
+        constexpr int D = 1 << decltype(LV)::value;
+        f4 S[NB];
+        shift_tiles<D, NSL, NB>(Z, S);
+        const f4 P = opP(decltype(LV)::value);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+            if (D < 16 || nb >= (D >> 4)) Z[nb] = apply<NSL>(P, S[nb], Z[nb]);
+    });
+
+    // 4. chunk partial = Z(:, G-1): lanes 15, 31, 47, 63 hold components 4 g + j
+    uint64_t *rec = a.lb + (size_t)chunk * 32;
+    if (col == 15) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) rec_store(rec + 4 * g + j, part_flag, Z[NB - 1][j]);
+    }
+    // the predecessor's record is in flight while Y0 = T X runs on the matrix cores
+    const uint64_t poll0 = lookback_first_poll<16>(a.lb, 32, chunk);
     f4 Y0[MB][NB];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
@@ -259,26 +279,8 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G), bs_waves(L, G)))) void bs_ker
         }
     }
 
-    // 3. tile end states, log2(G)-level scan (levels D >= 16 skip the
-    //    all-zero blocks)
-    static_for<0, bs_levels<G>(), 1>([&](auto LV) {
-        constexpr int D = 1 << decltype(LV)::value;
-        f4 S[NB];
-        shift_tiles<D, NSL, NB>(Z, S);
-        const f4 P = opP(decltype(LV)::value);
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-            if (D < 16 || nb >= (D >> 4)) Z[nb] = apply<NSL>(P, S[nb], Z[nb]);
-    });
-
-    // 4. chunk partial = Z(:, G-1): lanes 15, 31, 47, 63 hold components 4 g + j
-    uint64_t *rec = a.lb + (size_t)chunk * 32;
-    if (col == 15) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) rec_store(rec + 4 * g + j, part_flag, Z[NB - 1][j]);
-    }
     const float Xj = lookbackK<16, kSlots>(a.lb, 32, chunk, part_flag, full_flag, tab + Y.kMc, depth, lp,
-                                           a.spin_limit, a.err);
+                                           a.spin_limit, a.err, &poll0);
     // the output stage's O blocks, fetched before any store of this chunk:
     // vmcnt retires in issue order, so a load issued behind stores waits for them
     f4 Ob[MB];

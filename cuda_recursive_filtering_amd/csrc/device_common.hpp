@@ -307,10 +307,19 @@ __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int se
 // `lp` is LDS scratch of SLOTS*K floats: at most SLOTS partials are folded;
 // an unbounded lookback (depth 0) or one deeper than SLOTS takes a full
 // record in its last slot.  Returns X_j on lane j < K (0 elsewhere).
+// The predecessor record's first poll, issued early so that its round trip
+// overlaps independent work (lanes [0, 2K), 0 elsewhere and for chunk 0).
+template <int K>
+__device__ __forceinline__ uint64_t lookback_first_poll(uint64_t *lb, size_t rec_stride, uint32_t chunk)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    return (chunk > 0 && lane < 2 * K) ? rec_load(lb + (size_t)(chunk - 1) * rec_stride + lane) : 0ull;
+}
+
 template <int K, int SLOTS = kWave>
 __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint32_t chunk, uint32_t part_flag,
                                           uint32_t full_flag, const float *F, int depth, float *lp,
-                                          uint32_t spin_limit, uint32_t *err)
+                                          uint32_t spin_limit, uint32_t *err, const uint64_t *first_poll = nullptr)
 {
     static_assert(2 * K <= kWave, "a record's two halves must fit on one wave");
     const int lane = threadIdx.x & (kWave - 1);
@@ -328,10 +337,12 @@ __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint
             break;
         }
         uint64_t *pr = lb + (size_t)pred * rec_stride;
+        bool use_first = first_poll && d == 0;
         for (;;) {
             // one round trip for the whole record: partial words on lanes
             // [0, K), full words on lanes [K, 2K)
-            const uint64_t w = (lane < 2 * K) ? rec_load(pr + lane) : 0ull;
+            const uint64_t w = use_first ? *first_poll : ((lane < 2 * K) ? rec_load(pr + lane) : 0ull);
+            use_first = false;
             const float v = rec_val(w);
             if (__ballot(lane >= K && lane < 2 * K && rec_flag(w) != full_flag) == 0) {
                 Xj = __shfl_down(v, K, kWave);  // full word j from lane j + K
