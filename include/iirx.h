@@ -263,7 +263,7 @@ int iirx_tune_candidates(const float *sos, int32_t n_sections, int64_t n_samples
                          uint32_t flags, iirx_tune_entry *entries, int32_t cap,
                          int32_t *count);
 /* Time every eligible candidate (reps launches after 3 warm-up launches on
- * zeroed device buffers of n_samples floats, rotating over up to 8 inputs so
+ * N(0,1) device buffers of n_samples floats, rotating over up to 8 inputs so
  * that long signals stream from HBM) and return the fastest in *best /
  * *best_ms; entries (optional) receive the full table. */
 int iirx_autotune(const float *sos, int32_t n_sections, int64_t n_samples,
