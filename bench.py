@@ -42,8 +42,9 @@ def parse():
     ap.add_argument("--bs-geom", default="64x64",
                     help="block-state form geometry GxL for the MFMA vs plain fp32 comparison (orders 8 and 16)")
     ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA vs plain fp32 comparison")
-    ap.add_argument("--workload", default="single", choices=["single", "ragged"],
-                    help="single: one 2^25-sample channel per order; ragged: the 4096-channel batch")
+    ap.add_argument("--workload", default="single", choices=["single", "ragged", "sweep"],
+                    help="single: one 2^25-sample channel per order; ragged: the 4096-channel batch; "
+                         "sweep: order 8 at Wn 0.02/0.1/0.4 over lengths 2^16..2^25")
     ap.add_argument("--batch-n", type=int, default=16, help="ragged batch: samples per lane (16, 32 or 64)")
     ap.add_argument("--copy-steps", type=int, default=50,
                     help="device-to-device copies of one 2^25-sample buffer, timed as the measured HBM ceiling "
@@ -201,6 +202,50 @@ def bench_ragged(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_sweep(args):
+    """Length / conditioning sweep: order-8 Butterworth at Wn in {0.02, 0.1, 0.4},
+    lengths 2^16..2^25, N(0,1) + 0.5 sin(2 pi 0.05 n) (seed 2); GSamples/s of
+    --kernel/--geom per (Wn, length), inputs resident in HBM (one buffer: the
+    short lengths are cache-resident, as they are in use)."""
+    import numpy as np
+    import torch
+    import cuda_recursive_filtering_amd as iirx
+    B, N = (int(v) for v in args.geom.lower().split("x"))
+    dev = torch.device("cuda:0")
+    rows = []
+    for wn in (0.02, 0.1, 0.4):
+        sos, g = iirx.butterworth_sos(8, wn)
+        for log2n in range(16, 26):
+            n = 1 << log2n
+            x = torch.from_numpy((g * (iirx.fill_normal(2, n).astype(np.float64)
+                                       + 0.5 * np.sin(2 * np.pi * 0.05 * np.arange(n)))).astype(np.float32)).to(dev)
+            out = torch.empty_like(x)
+            with iirx.Plan(args.kernel, sos, n, block_size=B, n_blocks=N, schedule=args.schedule) as p:
+                for _ in range(args.warmup):
+                    p.execute(x, out)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(args.steps):
+                    p.execute(x, out)
+                e1.record()
+                torch.cuda.synchronize()
+                p.check()
+                ms = e0.elapsed_time(e1) / args.steps
+                info = p.info()
+            rows.append({"wn": wn, "log2n": log2n, "ms": round(ms, 5), "gsamples_per_s": round(n / (ms * 1e-3) / 1e9, 2),
+                         "chunks": info["num_chunks"], "grid": info["grid"]})
+            print(f"# wn={wn} n=2^{log2n}: {rows[-1]}", file=sys.stderr, flush=True)
+    big = [r for r in rows if r["log2n"] == 25 and r["wn"] == 0.1][0]
+    print(json.dumps({"metric": "GSamples/s, order-8 length/conditioning sweep", "value": big["gsamples_per_s"],
+                      "unit": "GSamples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": big["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "order-8 Butterworth, Wn 0.02/0.1/0.4, 2^16..2^25 samples of N(0,1) + "
+                                             "0.5 sin(2 pi 0.05 n) (seed 2); value = Wn 0.1 at 2^25",
+                                 "kernel": args.kernel, "geometry": args.geom}, "sweep": rows}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -209,6 +254,9 @@ def main():
         sys.exit(2)
     if args.workload == "ragged":
         bench_ragged(args)
+        return
+    if args.workload == "sweep":
+        bench_sweep(args)
         return
     import numpy as np
     import torch
