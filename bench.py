@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--workload", default="single", choices=["single", "ragged", "sweep"],
                     help="single: one 2^25-sample channel per order; ragged: the 4096-channel batch; "
                          "sweep: order 8 at Wn 0.02/0.1/0.4 over lengths 2^16..2^25")
-    ap.add_argument("--batch-n", type=int, default=16, help="ragged batch: samples per lane (16, 32 or 64)")
+    ap.add_argument("--batch-n", type=int, default=32, help="ragged batch: samples per lane (16, 32 or 64)")
     ap.add_argument("--copy-steps", type=int, default=50,
                     help="device-to-device copies of one 2^25-sample buffer, timed as the measured HBM ceiling "
                          "(also the known-size kernel that scales rocprof FETCH_SIZE/WRITE_SIZE)")

@@ -268,7 +268,7 @@ class Batch:
     schedule     'persistent' (default) or 'ticket'
     """
 
-    def __init__(self, offsets, sections, n_sections=None, n_blocks=16, schedule="persistent"):
+    def __init__(self, offsets, sections, n_sections=None, n_blocks=32, schedule="persistent"):
         off = np.ascontiguousarray(np.asarray(offsets, np.int64))
         if n_sections is None:
             secs = [_sos_array(s) for s in sections]

@@ -78,7 +78,7 @@ int iirx_batch_create(const iirx_batch_config *cfg, int64_t n_channels, const in
 {
     if (!cfg || !offsets || !n_sections || !sos || !out || n_channels < 1) return IIRX_ERR_INVALID;
     *out = nullptr;
-    const int N = cfg->n_blocks == 0 ? 16 : cfg->n_blocks;
+    const int N = cfg->n_blocks == 0 ? 32 : cfg->n_blocks;
     if (!(N == 16 || N == 32 || N == 64)) return IIRX_ERR_GEOMETRY;
     if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && cfg->schedule != IIRX_SCHEDULE_TICKET) return IIRX_ERR_INVALID;
     if (offsets[0] < 0) return IIRX_ERR_INVALID;

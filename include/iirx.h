@@ -287,7 +287,7 @@ int iirx_autotune(const float *sos, int32_t n_sections, int64_t n_samples,
 typedef struct iirx_batch iirx_batch;
 
 typedef struct {
-    int32_t n_blocks;   /* samples per lane: 16, 32 or 64 (0 = 16)          */
+    int32_t n_blocks;   /* samples per lane: 16, 32 or 64 (0 = 32)          */
     int32_t schedule;   /* iirx_schedule                                    */
     int32_t reserved[2];
 } iirx_batch_config;
