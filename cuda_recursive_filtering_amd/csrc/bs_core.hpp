@@ -1,0 +1,248 @@
+// bs_core.hpp -- one chunk of the block-state form (used by bs.hip for one
+// long signal and by batch.hip for the block-state channels of a ragged
+// batch).  The algorithm is described at the top of bs.hip.
+//
+// A chunk is G tiles of L samples held by one wave64.  `xr` is the chunk's
+// input in the x32 B-operand layout: lane (g, col), tile 16 nb + col, K slab
+// q: a[nb][q] = samples 32 q + 4 g + (0..3), b[nb][q] = samples
+// 32 q + 16 + 4 g + (0..3).  `ops` supplies the operand blocks (tmat(hl, mb,
+// q) of T, gam(hl, q) of Gam, pow2(j) = Phi^(2^j), phi() = Phi, pow16(nb) =
+// Phi^(16 nb), obs(mb) of O) and `store(nb, mb, y)` writes
+// output row block mb of tile block nb (lane (g, col): samples 16 mb + 4 g +
+// (0..3) of tile 16 nb + col).  Lookback records are 32 words per chunk; the
+// chain starts at chunk `first`.
+#pragma once
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace iirx {
+
+constexpr int kBsSlots = 16;  // lookback partials folded at most (LDS: 16 x 16 floats)
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f4 mfma_h(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma_f(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+// DPP moves (v_mov_dpp, no LDS round trip): row_ror:D rotates every 16-lane
+// row, lane l <- lane (l - D) mod 16 of its row; row_bcast15/31 pass lane 15 /
+// 31 on to the next row / upper half
+template <int D>
+__device__ __forceinline__ float row_ror(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + D, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+    v = fmaxf(v, row_ror<1>(v));
+    v = fmaxf(v, row_ror<2>(v));
+    v = fmaxf(v, row_ror<4>(v));
+    v = fmaxf(v, row_ror<8>(v));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x142, 0xF, 0xF, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x143, 0xF, 0xF, false)));
+    return readlane_f(v, 63);
+}
+
+// power-of-two exponent s with max * 2^s < 2^14
+__device__ __forceinline__ int scale_exp(float mx)
+{
+    return (mx > 0.0f) ? 14 - __builtin_amdgcn_frexp_expf(mx) : 0;
+}
+
+// hi/lo FP16 split of (a, b) * 2^s, the 8 K rows of one x32 B operand
+__device__ __forceinline__ void split8(const f4 &a, const f4 &b, int s, h8 &hi, h8 &lo)
+{
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float x = ldexpf(j < 4 ? a[j & 3] : b[j & 3], s);
+        const _Float16 h = (_Float16)x;
+        hi[j] = h;
+        lo[j] = (_Float16)(x - (float)h);
+    }
+}
+
+// acc + M S over the first NSL K slabs; M is this lane's float4 of an FP32 block
+template <int NSL>
+__device__ __forceinline__ f4 apply(const f4 &M, const f4 &S, f4 acc)
+{
+#pragma unroll
+    for (int m = 0; m < NSL; ++m) acc = mfma_f(M[m], S[m], acc);
+    return acc;
+}
+
+// S(c, k) = Z(c, k - D) (0 for k < D) over NB column blocks of 16 tiles;
+// D < 16 rotates inside each 16-lane row and takes the wrapped lanes from the
+// previous block.  Only the NSL live state registers are moved.
+template <int D, int NSL, int NB>
+__device__ __forceinline__ void shift_tiles(const f4 (&Z)[NB], f4 (&S)[NB])
+{
+    if constexpr (D >= 16) {
+        constexpr int dn = D >> 4;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) S[nb] = (nb >= dn) ? Z[nb >= dn ? nb - dn : 0] : f4{0.f, 0.f, 0.f, 0.f};
+    } else {
+        const bool wrap = (threadIdx.x & 15) < D;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j < NSL) {
+                    const float r0 = row_ror<D>(Z[nb][j]);
+                    const float r1 = (nb > 0) ? row_ror<D>(Z[nb > 0 ? nb - 1 : 0][j]) : 0.0f;
+                    S[nb][j] = wrap ? r1 : r0;
+                } else {
+                    S[nb][j] = 0.0f;
+                }
+            }
+        }
+    }
+}
+
+template <int G>
+constexpr int bs_levels() { return G == 16 ? 4 : G == 32 ? 5 : 6; }
+
+template <int NB, int KQ>
+struct BsRaw {
+    f4 a[NB][KQ], b[NB][KQ];
+};
+
+template <int L, int G, int NSL, class Ops, class Store>
+__device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32_t chunk, uint32_t part_flag,
+                                         uint32_t full_flag, uint64_t *lb, const float *Mc, int depth, float *lp,
+                                         uint32_t spin_limit, uint32_t *err, uint32_t first, int sT, int sG,
+                                         const Ops &ops, const Store &store)
+{
+    constexpr int MB = L / 16;  // output row blocks of T
+    constexpr int KQ = L / 32;  // K slabs of 32 samples
+    constexpr int NB = G / 16;  // column blocks of 16 tiles
+    const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
+
+    // 1. per-chunk scale and FP16 hi/lo split of X
+    h8 xh[NB][KQ], xl[NB][KQ];
+    int eX;
+    {
+        float mx = 0.0f;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int q = 0; q < KQ; ++q)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) mx = fmaxf(mx, fmaxf(fabsf(xr.a[nb][q][j]), fabsf(xr.b[nb][q][j])));
+        eX = scale_exp(wave_max(mx));
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int q = 0; q < KQ; ++q) split8(xr.a[nb][q], xr.b[nb][q], eX, xh[nb][q], xl[nb][q]);
+    }
+
+    // 2. V = Gam X (zero chunk-incoming state); Y0 = T X follows the partial
+    f4 Z[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) {
+            const h8 gh = ops.gam(0, q), gl = ops.gam(1, q);
+            acc = mfma_h(gl, xh[nb][q], acc);
+            acc = mfma_h(gh, xl[nb][q], acc);
+            acc = mfma_h(gh, xh[nb][q], acc);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Z[nb][j] = ldexpf(acc[j], -(sG + eX));
+    }
+    // 3. tile end states, log2(G)-level scan (levels D >= 16 skip the
+    //    all-zero blocks)
+    static_for<0, bs_levels<G>(), 1>([&](auto LV) {
+        constexpr int D = 1 << decltype(LV)::value;
+        f4 S[NB];
+        shift_tiles<D, NSL, NB>(Z, S);
+        const f4 P = ops.pow2(decltype(LV)::value);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+            if (D < 16 || nb >= (D >> 4)) Z[nb] = apply<NSL>(P, S[nb], Z[nb]);
+    });
+
+    // 4. chunk partial = Z(:, G-1): lanes 15, 31, 47, 63 hold components 4 g + j
+    uint64_t *rec = lb + (size_t)chunk * 32;
+    if (col == 15) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) rec_store(rec + 4 * g + j, part_flag, Z[NB - 1][j]);
+    }
+    // the predecessor's record is in flight while Y0 = T X runs on the matrix cores
+    const uint64_t poll0 = (chunk > first) ? lookback_first_poll<16>(lb, 32, chunk) : 0ull;
+    f4 Y0[MB][NB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+        const int qmax = bs_tq(mb);  // T(16 mb + r, 32 q + c) = 0 beyond
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int q = 0; q < KQ; ++q) {
+                if (q > qmax) break;
+                const h8 th = ops.tmat(0, mb, q), tl = ops.tmat(1, mb, q);
+                acc = mfma_h(tl, xh[nb][q], acc);
+                acc = mfma_h(th, xl[nb][q], acc);
+                acc = mfma_h(th, xh[nb][q], acc);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Y0[mb][nb][j] = ldexpf(acc[j], -(sT + eX));
+        }
+    }
+
+    const float Xj = lookbackK<16, kBsSlots>(lb, 32, chunk, part_flag, full_flag, Mc, depth, lp, spin_limit, err,
+                                             &poll0, first);
+    // the output stage's O blocks, fetched before any store of this chunk:
+    // vmcnt retires in issue order, so a load issued behind stores waits for them
+    f4 Ob[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) Ob[mb] = ops.obs(mb);
+    asm volatile("" ::: "memory");
+    // X_in in B layout on column 0 (component 4 g + m lives on lane 4 g + m)
+    f4 xin;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xin[j] = (j < NSL) ? __shfl(Xj, 4 * g + j, 64) : 0.0f;
+
+    // injection W_k = Phi^(k+1) X_in: Phi X_in in column 0 of one block, a
+    // 4-level scan inside the block, then Phi^(16 nb) for blocks 1..NB-1
+    f4 U[1];
+    {
+        f4 b0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b0[j] = (col == 0) ? xin[j] : 0.0f;
+        U[0] = apply<NSL>(ops.phi(), b0, f4{0.f, 0.f, 0.f, 0.f});
+    }
+    static_for<0, 4, 1>([&](auto LV) {
+        constexpr int D = 1 << decltype(LV)::value;
+        f4 S[1];
+        shift_tiles<D, NSL, 1>(U, S);
+        U[0] = apply<NSL>(ops.pow2(decltype(LV)::value), S[0], U[0]);
+    });
+#pragma unroll
+    for (int nb = 1; nb < NB; ++nb) Z[nb] = apply<NSL>(ops.pow16(nb), U[0], Z[nb]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Z[0][j] += U[0][j];
+
+    // chunk full = Z(:, G-1)
+    if (col == 15) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) rec_store(rec + 16 + 4 * g + j, full_flag, Z[NB - 1][j]);
+    }
+
+    // 5. S_k = state entering tile k (X_in for tile 0); Y = Y0 + O S
+    f4 S[NB];
+    shift_tiles<1, NSL, NB>(Z, S);
+    if (col == 0) S[0] = xin;
+    // a store covers 64 B of each of 16 tiles; the two halves of a 128-byte
+    // line (row blocks mb, mb + 1) are issued back to back and through the L2
+    // (not nontemporal), so that they merge there: measured WRITE_SIZE 1.06x
+    // the output bytes this way, 1.43x with mb outer and nontemporal stores
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) store(nb, mb, apply<NSL>(Ob[mb], S[nb], Y0[mb][nb]));
+    }
+}
+
+}  // namespace iirx

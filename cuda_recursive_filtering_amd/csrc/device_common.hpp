@@ -319,7 +319,8 @@ __device__ __forceinline__ uint64_t lookback_first_poll(uint64_t *lb, size_t rec
 template <int K, int SLOTS = kWave>
 __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint32_t chunk, uint32_t part_flag,
                                           uint32_t full_flag, const float *F, int depth, float *lp,
-                                          uint32_t spin_limit, uint32_t *err, const uint64_t *first_poll = nullptr)
+                                          uint32_t spin_limit, uint32_t *err, const uint64_t *first_poll = nullptr,
+                                          uint32_t first = 0)
 {
     static_assert(2 * K <= kWave, "a record's two halves must fit on one wave");
     const int lane = threadIdx.x & (kWave - 1);
@@ -331,8 +332,8 @@ __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint
     uint32_t spins = 0;
     for (int d = 0; d < maxd && !done; ++d) {
         const long pred = (long)chunk - 1 - d;
-        if (pred < 0) {
-            D = d;  // signal start: zero state
+        if (pred < (long)first) {
+            D = d;  // signal (or channel) start: zero state
             done = true;
             break;
         }
