@@ -46,6 +46,8 @@ def parse():
                     help="single: one 2^25-sample channel per order; ragged: the 4096-channel batch; "
                          "sweep: order 8 at Wn 0.02/0.1/0.4 over lengths 2^16..2^25")
     ap.add_argument("--batch-n", type=int, default=32, help="ragged batch: samples per lane (16, 32 or 64)")
+    ap.add_argument("--batch-bs-min-sections", type=int, default=0,
+                    help="ragged batch: channels with at least this many sections run the block-state form (0: none)")
     ap.add_argument("--copy-steps", type=int, default=50,
                     help="device-to-device copies of one 2^25-sample buffer, timed as the measured HBM ceiling "
                          "(also the known-size kernel that scales rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -151,7 +153,8 @@ def bench_ragged(args):
     x.mul_(torch.repeat_interleave(torch.from_numpy(gains.astype(np.float32)).to(dev), lens))
     out = torch.empty_like(x)
     s = torch.cuda.Stream()
-    b = iirx.Batch(offsets, sections, n_blocks=args.batch_n, schedule=args.schedule)
+    b = iirx.Batch(offsets, sections, n_blocks=args.batch_n, schedule=args.schedule,
+                   bs_min_sections=args.batch_bs_min_sections)
     b.capture(x, out, s)
     for _ in range(args.warmup):
         b.replay(s)
@@ -182,7 +185,10 @@ def bench_ragged(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "ragged batch: 4096 channels, lengths log-uniform [2^10, 2^20], order uniform "
                                "{2,4,8,16}, Butterworth Wn uniform [0.02, 0.45] (seed 4), gain * N(0,1)",
-                   "n_samples": n, "n_channels": 4096, "kernel": "PH per chunk", "samples_per_lane": args.batch_n,
+                   "n_samples": n, "n_channels": 4096,
+                   "kernel": ("PH per chunk" if not info["mixed"] else
+                              f"PH, block-state for >= {args.batch_bs_min_sections} sections"),
+                   "blockstate_channels": info["n_blockstate"], "samples_per_lane": args.batch_n,
                    "chunks": info["num_chunks"], "grid": info["grid"], "schedule": args.schedule},
         "eager_ms_per_step": round(ms_eager, 4),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -266,9 +266,15 @@ class Batch:
                  (total_sections, 6) array with `n_sections` given
     n_blocks     samples per lane (16, 32 or 64): chunks of 64 * n_blocks
     schedule     'persistent' (default) or 'ticket'
+    algo         per-channel algorithm: one of 'auto', 'ph', 'blockstate'
+                 for every channel, or a sequence of them (or of the
+                 BATCH_ALGOS codes), one per channel
+    bs_min_sections  'auto' channels with at least this many sections run
+                 the block-state form (0: none)
     """
 
-    def __init__(self, offsets, sections, n_sections=None, n_blocks=32, schedule="persistent"):
+    def __init__(self, offsets, sections, n_sections=None, n_blocks=32, schedule="persistent", algo="auto",
+                 bs_min_sections=0):
         off = np.ascontiguousarray(np.asarray(offsets, np.int64))
         if n_sections is None:
             secs = [_sos_array(s) for s in sections]
@@ -286,9 +292,16 @@ class Batch:
         self._cfg = _lib.BatchConfig()
         self._cfg.n_blocks = int(n_blocks)
         self._cfg.schedule = _lib.SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
+        self._cfg.bs_min_sections = int(bs_min_sections)
+        if isinstance(algo, str):
+            algo = [algo] * self.n_channels
+        codes = np.ascontiguousarray(
+            [_lib.BATCH_ALGOS[a] if isinstance(a, str) else int(a) for a in algo], np.int32)
+        if codes.size != self.n_channels:
+            raise ValueError("algo needs one entry per channel")
         h = ctypes.c_void_p()
-        check(lib().iirx_batch_create(ctypes.byref(self._cfg), self.n_channels, off.ctypes.data, ns.ctypes.data,
-                                      sos.ctypes.data, ctypes.byref(h)), "batch_create")
+        check(lib().iirx_batch_create_ex(ctypes.byref(self._cfg), self.n_channels, off.ctypes.data, ns.ctypes.data,
+                                         sos.ctypes.data, codes.ctypes.data, ctypes.byref(h)), "batch_create")
         self._h = h
 
     def _check_len(self, *ts):

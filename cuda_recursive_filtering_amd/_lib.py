@@ -54,6 +54,8 @@ class PlanInfo(ctypes.Structure):
     ]
 
 
+BATCH_ALGOS = {"auto": -1, "ph": 0, "blockstate": 1}
+
 TUNE_NO_BLOCKSTATE = 1
 TUNE_NO_CR = 2
 TUNE_PLR_ALL_ORDERS = 4
@@ -72,7 +74,8 @@ class BatchConfig(ctypes.Structure):
     _fields_ = [
         ("n_blocks", ctypes.c_int32),
         ("schedule", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("bs_min_sections", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -87,7 +90,8 @@ class BatchInfo(ctypes.Structure):
         ("vgprs", ctypes.c_int32),
         ("waves_per_simd", ctypes.c_int32),
         ("graph_ready", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("mixed", ctypes.c_int32),
+        ("n_blockstate", ctypes.c_int64),
     ]
 
 
@@ -117,6 +121,8 @@ _SIGS = {
                                      ctypes.POINTER(ctypes.c_int32)]),
     "iirx_batch_create": (ctypes.c_int, [ctypes.POINTER(BatchConfig), ctypes.c_int64, _P, _P, _P,
                                          ctypes.POINTER(_P)]),
+    "iirx_batch_create_ex": (ctypes.c_int, [ctypes.POINTER(BatchConfig), ctypes.c_int64, _P, _P, _P, _P,
+                                            ctypes.POINTER(_P)]),
     "iirx_batch_execute": (ctypes.c_int, [_P, _P, _P, _P]),
     "iirx_batch_capture": (ctypes.c_int, [_P, _P, _P, _P]),
     "iirx_batch_replay": (ctypes.c_int, [_P, _P]),

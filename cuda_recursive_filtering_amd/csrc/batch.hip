@@ -10,8 +10,10 @@
 // filter is causal, so they cannot reach earlier outputs) and not written.
 // Channel offsets are arbitrary (CSR packing), so HBM is accessed with
 // 4-byte coalesced loads and stores: 64 consecutive floats per instruction.
-#include "device_common.hpp"
-#include "kernels.hpp"
+// A channel may instead run the block-state form (bs_core.hpp) with tiles of
+// 64 samples and N tiles per chunk (the same 64 N samples), its tables read
+// from global memory; the MIXED instance carries both paths.
+#include "bs_core.hpp"
 
 namespace iirx {
 
@@ -20,17 +22,100 @@ typedef const __attribute__((address_space(4))) uint32_t cu32;
 
 // waves per SIMD the register budget is set for: at N = 32 four waves (a
 // 20-byte/lane spill) measured 5% faster than three without one
-template <int N>
-constexpr int batch_waves() { return N == 16 ? 5 : N == 32 ? 4 : 1; }
+template <int N, bool MIXED>
+constexpr int batch_waves() { return MIXED ? (N == 64 ? 1 : N == 32 ? 3 : 4) : (N == 16 ? 5 : N == 32 ? 4 : 1); }
 
+// block-state operands of one channel, from its tables in global memory
+struct BsGlobalOps {
+    static constexpr BsLayout Y = bs_layout(64);
+    const float *tab;
+    int lane;
+    __device__ h8 tmat(int hl, int mb, int q) const
+    {
+        return __builtin_bit_cast(h8, reinterpret_cast<const uint4 *>(tab + bs_T(Y, hl, mb, q))[lane]);
+    }
+    __device__ h8 gam(int hl, int q) const { return __builtin_bit_cast(h8, reinterpret_cast<const uint4 *>(tab + bs_G(Y, hl, q))[lane]); }
+    __device__ f4 pow2(int j) const { return reinterpret_cast<const f4 *>(tab + bs_P(Y, j))[lane]; }
+    __device__ f4 phi() const { return reinterpret_cast<const f4 *>(tab + Y.kPhi)[lane]; }
+    __device__ f4 pow16(int nb) const { return reinterpret_cast<const f4 *>(tab + bs_Q(Y, nb))[lane]; }
+    __device__ f4 obs(int mb) const { return reinterpret_cast<const f4 *>(tab + bs_O(Y, mb))[lane]; }
+};
+
+// LDS of the block-state path: tile t (64 samples) at 68 t, so that the 16
+// tiles a row of lanes reads with one ds_read_b128 fall in distinct banks
+constexpr int kBsTilePitch = 68;
+
+// one block-state chunk of a batch channel: G = N tiles of L = 64 samples,
+// four FP32 K slabs (any order up to 16).  The chunk is staged in and out
+// through LDS with coalesced 4-byte accesses (channel offsets are arbitrary);
+// the last chunk reads zeros past the channel end and stores only r samples.
 template <int N>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(batch_waves<N>(), batch_waves<N>())))
+__device__ __forceinline__ void batch_bs_chunk(const BatchArgs &a, uint32_t chunk, uint32_t part_flag,
+                                               uint32_t full_flag, const float *coef, uint32_t first, long base, int r,
+                                               float *lds, float *lp)
+{
+    constexpr int L = 64, G = N, NB = G / 16, KQ = 2, CH = L * G;
+    constexpr BsLayout Y = bs_layout(L);
+    const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
+    const cfloat *ctab = as_const(coef);
+
+    __syncthreads();  // LDS of the previous chunk fully consumed
+    {
+        const float *src = a.in + base + lane;
+        if (r == CH) {
+            float v[G];
+#pragma unroll
+            for (int t = 0; t < G; ++t) v[t] = src[t * L];
+#pragma unroll
+            for (int t = 0; t < G; ++t) lds[t * kBsTilePitch + lane] = v[t];
+        } else {
+            for (int t = 0; t < G; ++t) {
+                const int e = t * L + lane;
+                const float v = a.in[base + (e < r ? e : r - 1)];
+                lds[t * kBsTilePitch + lane] = (e < r) ? v : 0.0f;
+            }
+        }
+    }
+    __syncthreads();
+    // lane (g, col), tile 16 nb + col: samples 32 q + 4 g + (0..3) and 32 q + 16 + 4 g + (0..3)
+    BsRaw<NB, KQ> xr;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) {
+            const f4 *t = reinterpret_cast<const f4 *>(lds + (16 * nb + col) * kBsTilePitch + 32 * q + 4 * g);
+            xr.a[nb][q] = t[0];
+            xr.b[nb][q] = t[4];
+        }
+    const BsGlobalOps ops{coef, lane};
+    // the chunk's inputs are in registers before the first output block
+    // overwrites LDS (a wave's LDS accesses complete in order)
+    auto store = [&](int nb, int mb, const f4 &y) {
+        *reinterpret_cast<f4 *>(lds + (16 * nb + col) * kBsTilePitch + 16 * mb + 4 * g) = y;
+    };
+    bs_chunk<L, G, 4>(xr, chunk, part_flag, full_flag, a.lb, coef + Y.kMc, (int)ctab[Y.kDepth], lp, a.spin_limit,
+                      a.err, first, (int)ctab[Y.kScale + 0], (int)ctab[Y.kScale + 1], ops, store);
+    __syncthreads();
+    float *dst = a.out + base + lane;
+    if (r == CH) {
+#pragma unroll
+        for (int t = 0; t < G; ++t) dst[t * L] = lds[t * kBsTilePitch + lane];
+    } else {
+        for (int t = 0; t < G; ++t)
+            if (t * L + lane < r) dst[t * L] = lds[t * kBsTilePitch + lane];
+    }
+}
+
+template <int N, bool MIXED>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(batch_waves<N, MIXED>(), batch_waves<N, MIXED>())))
 void batch_ph_kernel(BatchArgs a)
 {
     constexpr int L = 64;
     constexpr int CH = L * N;
     constexpr PhLayout Y = ph_layout(L, N);
-    __shared__ float lds[L * (N + 1)];
+    constexpr int kLds = L * (N + 1), kLdsBs = kBsTilePitch * N;
+    __shared__ __align__(16) float lds[MIXED && kLdsBs > kLds ? kLdsBs : kLds];
+    __shared__ float lp[MIXED ? kBsSlots * 16 : 1];  // block-state lookback slots
 
     const int lane = threadIdx.x;
     const cchannel *chan = (const cchannel *)a.chan;
@@ -48,6 +133,12 @@ void batch_ph_kernel(BatchArgs a)
     const long base = ch_off + j * CH;
     const long rem = ch_len - j * CH;
     const int r = rem < CH ? (int)rem : CH;  // valid samples in this chunk
+    if constexpr (MIXED) {
+        if (chan[ci].algo == kBatchBS) {
+            batch_bs_chunk<N>(a, chunk, part_flag, full_flag, coef, first, base, r, lds, lp);
+            return;
+        }
+    }
 
     // stage in: chunk element e = 64 i + lane (coalesced), lane t owns
     // [t N, t N + N).  Full chunks use one base address and immediate offsets;
@@ -181,29 +272,29 @@ void batch_ph_kernel(BatchArgs a)
     });
 }
 
-template <int N>
+template <int N, bool MIXED>
 static hipError_t launch_n(const BatchArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(batch_ph_kernel<N>, dim3(a.grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((batch_ph_kernel<N, MIXED>), dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_batch(int n_blocks, const BatchArgs &a, hipStream_t s)
+hipError_t launch_batch(int n_blocks, bool mixed, const BatchArgs &a, hipStream_t s)
 {
     switch (n_blocks) {
-    case 16: return launch_n<16>(a, s);
-    case 32: return launch_n<32>(a, s);
-    case 64: return launch_n<64>(a, s);
+    case 16: return mixed ? launch_n<16, true>(a, s) : launch_n<16, false>(a, s);
+    case 32: return mixed ? launch_n<32, true>(a, s) : launch_n<32, false>(a, s);
+    case 64: return mixed ? launch_n<64, true>(a, s) : launch_n<64, false>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-const void *kernel_batch(int n_blocks)
+const void *kernel_batch(int n_blocks, bool mixed)
 {
     switch (n_blocks) {
-    case 16: return (const void *)batch_ph_kernel<16>;
-    case 32: return (const void *)batch_ph_kernel<32>;
-    case 64: return (const void *)batch_ph_kernel<64>;
+    case 16: return mixed ? (const void *)batch_ph_kernel<16, true> : (const void *)batch_ph_kernel<16, false>;
+    case 32: return mixed ? (const void *)batch_ph_kernel<32, true> : (const void *)batch_ph_kernel<32, false>;
+    case 64: return mixed ? (const void *)batch_ph_kernel<64, true> : (const void *)batch_ph_kernel<64, false>;
     default: return nullptr;
     }
 }

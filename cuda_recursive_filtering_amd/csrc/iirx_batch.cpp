@@ -3,8 +3,9 @@
 // once-zeroed lookback state, the persistent launch and its hipGraph capture.
 //
 // No counterpart in the reference, which filters one channel per binary; the
-// per-channel tables are those of setup_kernel_coefficients
-// (ph_kernels.cuh:151-200), built by ph_tables.
+// per-channel PH tables are those of setup_kernel_coefficients
+// (ph_kernels.cuh:151-200), built by ph_tables; block-state channels take
+// bs_tables for tiles of 64 samples and n_blocks tiles per chunk.
 #include "../../include/iirx.h"
 #include "kernels.hpp"
 #include "precompute.hpp"
@@ -23,6 +24,8 @@ struct iirx_batch {
     int64_t n_channels, n_samples;
     uint32_t num_chunks, grid, spin_limit;
     int N, coef_stride, device;
+    bool mixed;
+    int64_t n_blockstate;
     float *d_coef;
     uint64_t *d_lb;
     BatchChannel *d_chan;
@@ -76,16 +79,23 @@ extern "C" {
 int iirx_batch_create(const iirx_batch_config *cfg, int64_t n_channels, const int64_t *offsets,
                       const int32_t *n_sections, const float *sos, iirx_batch **out)
 {
+    return iirx_batch_create_ex(cfg, n_channels, offsets, n_sections, sos, nullptr, out);
+}
+
+int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels, const int64_t *offsets,
+                         const int32_t *n_sections, const float *sos, const int32_t *algo, iirx_batch **out)
+{
     if (!cfg || !offsets || !n_sections || !sos || !out || n_channels < 1) return IIRX_ERR_INVALID;
     *out = nullptr;
     const int N = cfg->n_blocks == 0 ? 32 : cfg->n_blocks;
     if (!(N == 16 || N == 32 || N == 64)) return IIRX_ERR_GEOMETRY;
     if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && cfg->schedule != IIRX_SCHEDULE_TICKET) return IIRX_ERR_INVALID;
-    if (offsets[0] < 0) return IIRX_ERR_INVALID;
+    if (offsets[0] < 0 || cfg->bs_min_sections < 0) return IIRX_ERR_INVALID;
     int64_t total_sec = 0;
     for (int64_t c = 0; c < n_channels; ++c) {
         if (offsets[c + 1] < offsets[c]) return IIRX_ERR_INVALID;
         if (n_sections[c] < 1 || n_sections[c] > kBatchMaxSections) return IIRX_ERR_INVALID;
+        if (algo && (algo[c] < IIRX_BATCH_AUTO || algo[c] > IIRX_BATCH_BLOCKSTATE)) return IIRX_ERR_INVALID;
         total_sec += n_sections[c];
     }
     for (int64_t i = 0; i < 6 * total_sec; ++i)
@@ -100,14 +110,23 @@ int iirx_batch_create(const iirx_batch_config *cfg, int64_t n_channels, const in
     std::vector<BatchChannel> chan((size_t)n_channels);
     std::vector<uint32_t> chunk_chan;
     int stride = 0;
-    int64_t sec0 = 0;
+    int64_t sec0 = 0, n_bs = 0;
     for (int64_t c = 0; c < n_channels; ++c) {
         std::vector<float> t;
         const int ns = n_sections[c];
-        const int st = ph_tables(sos + 6 * sec0, ns, 64, N, t);
+        const int32_t want = algo ? algo[c] : IIRX_BATCH_AUTO;
+        const bool bs = want == IIRX_BATCH_BLOCKSTATE ||
+                        (want == IIRX_BATCH_AUTO && cfg->bs_min_sections > 0 && ns >= cfg->bs_min_sections);
+        if (bs) {
+            if (bs_tables(sos + 6 * sec0, ns, 64, N, t) <= 0) return IIRX_ERR_INVALID;
+            ++n_bs;
+        } else {
+            const int st = ph_tables(sos + 6 * sec0, ns, 64, N, t);
+            if (st <= 0) return IIRX_ERR_INVALID;
+            stride = st;
+        }
         sec0 += ns;
-        if (st <= 0) return IIRX_ERR_INVALID;
-        stride = st;
+        t.resize((t.size() + 3) & ~(size_t)3, 0.0f);  // 16-byte aligned tables
         const int64_t len = offsets[c + 1] - offsets[c];
         const int64_t k = (len + CH - 1) / CH;
         if ((int64_t)chunk_chan.size() + k > 0x7fffffffLL || coef.size() + t.size() > 0x7fffffffULL)
@@ -118,7 +137,7 @@ int iirx_batch_create(const iirx_batch_config *cfg, int64_t n_channels, const in
         d.ns = ns;
         d.coef_off = (int32_t)coef.size();
         d.first_chunk = (uint32_t)chunk_chan.size();
-        d.pad = 0;
+        d.algo = bs ? kBatchBS : kBatchPH;
         coef.insert(coef.end(), t.begin(), t.end());
         chunk_chan.insert(chunk_chan.end(), (size_t)k, (uint32_t)c);
     }
@@ -133,9 +152,11 @@ int iirx_batch_create(const iirx_batch_config *cfg, int64_t n_channels, const in
     b->num_chunks = (uint32_t)chunk_chan.size();
     b->N = N;
     b->coef_stride = stride;
+    b->mixed = n_bs > 0;
+    b->n_blockstate = n_bs;
     b->spin_limit = 1u << 22;
     hipGetDevice(&b->device);
-    b->fn = kernel_batch(N);
+    b->fn = kernel_batch(N, b->mixed);
     const size_t lb_count = (size_t)b->num_chunks * kBatchMaxSections * 4u;
     if (hipMalloc((void **)&b->d_coef, coef.size() * sizeof(float)) != hipSuccess ||
         hipMalloc((void **)&b->d_lb, lb_count * sizeof(uint64_t)) != hipSuccess ||
@@ -176,7 +197,7 @@ int iirx_batch_create(const iirx_batch_config *cfg, int64_t n_channels, const in
 int iirx_batch_execute(iirx_batch *b, const float *d_in, float *d_out, void *stream)
 {
     if (!b || !d_in || !d_out) return IIRX_ERR_INVALID;
-    if (launch_batch(b->N, args(b, d_in, d_out), (hipStream_t)stream) != hipSuccess) return IIRX_ERR_HIP;
+    if (launch_batch(b->N, b->mixed, args(b, d_in, d_out), (hipStream_t)stream) != hipSuccess) return IIRX_ERR_HIP;
     b->launches++;
     return IIRX_OK;
 }
@@ -194,7 +215,7 @@ int iirx_batch_capture(iirx_batch *b, const float *d_in, float *d_out, void *str
         b->graph = nullptr;
     }
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return IIRX_ERR_HIP;
-    const hipError_t le = launch_batch(b->N, args(b, d_in, d_out), s);
+    const hipError_t le = launch_batch(b->N, b->mixed, args(b, d_in, d_out), s);
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(s, &g);
     if (le != hipSuccess || ee != hipSuccess || !g) {
@@ -245,6 +266,8 @@ int iirx_batch_info_get(const iirx_batch *b, iirx_batch_info *info)
     info->grid = b->grid;
     info->launches = b->launches;
     info->graph_ready = b->exec != nullptr;
+    info->mixed = b->mixed;
+    info->n_blockstate = b->n_blockstate;
     hipFuncAttributes at;
     if (hipFuncGetAttributes(&at, b->fn) == hipSuccess) info->vgprs = at.numRegs;
     int nb = 0;

@@ -245,8 +245,10 @@ struct BatchChannel {
     int32_t ns;
     int32_t coef_off;
     uint32_t first_chunk;
-    uint32_t pad;
+    int32_t algo;  // kBatchPH or kBatchBS
 };
+constexpr int32_t kBatchPH = 0;
+constexpr int32_t kBatchBS = 1;  // block-state form, tiles of 64 samples, N tiles per chunk
 
 struct BatchArgs {
     const float *in;
@@ -267,8 +269,8 @@ struct BatchArgs {
     int32_t pad;
 };
 
-hipError_t launch_batch(int n_blocks, const BatchArgs &a, hipStream_t s);
-const void *kernel_batch(int n_blocks);
+hipError_t launch_batch(int n_blocks, bool mixed, const BatchArgs &a, hipStream_t s);
+const void *kernel_batch(int n_blocks, bool mixed);
 
 hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s);
 const void *kernel_bs(int L, int G, int ns);

@@ -279,18 +279,31 @@ int iirx_autotune(const float *sos, int32_t n_sections, int64_t n_samples,
  * output, filtered with sections sos[sec_off[c] .. sec_off[c] + n_sections[c])
  * where sec_off is the running sum of n_sections (rows of 6 floats, the same
  * contract as iirx_plan_create), zero initial state per channel.  One
- * persistent launch filters every channel (PH per chunk of 64 x n_blocks
- * samples, lookback chains per channel); a channel's last chunk may be
- * partial.  The launch can be captured into a hipGraph and replayed: the
- * lookback epoch lives on the device, so replays stay correct.
+ * persistent launch filters every channel, chunk by chunk (64 x n_blocks
+ * samples), with lookback chains per channel; a channel's last chunk may be
+ * partial.  Each channel runs PH (ph_kernels.cuh PH_64) or the block-state
+ * form (tiles of 64 samples, n_blocks tiles per chunk, matrix cores); both
+ * give the reference's recurrence within the FP32 bounds of iirx_plan.  The
+ * launch can be captured into a hipGraph and replayed: the lookback epoch
+ * lives on the device, so replays stay correct.
  * ------------------------------------------------------------------------ */
 typedef struct iirx_batch iirx_batch;
 
 typedef struct {
-    int32_t n_blocks;   /* samples per lane: 16, 32 or 64 (0 = 32)          */
-    int32_t schedule;   /* iirx_schedule                                    */
-    int32_t reserved[2];
+    int32_t n_blocks;         /* samples per lane: 16, 32 or 64 (0 = 32)     */
+    int32_t schedule;         /* iirx_schedule                               */
+    int32_t bs_min_sections;  /* IIRX_BATCH_AUTO channels with at least this
+                                 many sections run the block-state form
+                                 (0 = none)                                  */
+    int32_t reserved;
 } iirx_batch_config;
+
+/* per-channel algorithm of iirx_batch_create_ex */
+enum {
+    IIRX_BATCH_AUTO = -1,      /* by cfg->bs_min_sections */
+    IIRX_BATCH_PH = 0,
+    IIRX_BATCH_BLOCKSTATE = 1
+};
 
 typedef struct {
     int64_t n_channels;
@@ -301,7 +314,8 @@ typedef struct {
     int64_t launches;
     int32_t vgprs, waves_per_simd;
     int32_t graph_ready;    /* a captured graph can be replayed */
-    int32_t pad;
+    int32_t mixed;          /* the launch carries the block-state path */
+    int64_t n_blockstate;   /* channels that run the block-state form */
 } iirx_batch_info;
 
 /* Validate the channels and build the per-channel FP64 tables on the host,
@@ -310,6 +324,13 @@ typedef struct {
 int iirx_batch_create(const iirx_batch_config *cfg, int64_t n_channels,
                       const int64_t *offsets, const int32_t *n_sections,
                       const float *sos, iirx_batch **out);
+/* As iirx_batch_create with each channel's algorithm chosen: algo[c] is
+ * IIRX_BATCH_AUTO, IIRX_BATCH_PH or IIRX_BATCH_BLOCKSTATE (algo NULL: all
+ * AUTO). */
+int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels,
+                         const int64_t *offsets, const int32_t *n_sections,
+                         const float *sos, const int32_t *algo,
+                         iirx_batch **out);
 /* One launch over every channel; d_in/d_out hold at least offsets[n_channels]
  * floats and must not overlap. */
 int iirx_batch_execute(iirx_batch *b, const float *d_in, float *d_out,

@@ -39,17 +39,19 @@ def _check_channels(y, x, offsets, sections, chans=None):
 
 # --- host-side validation (no device needed: it is checked before any HIP call)
 
-def _create_status(offsets, ns, sos, n_blocks=16):
+def _create_status(offsets, ns, sos, n_blocks=16, algo=None, bs_min_sections=0):
     import ctypes
     from cuda_recursive_filtering_amd import _lib
     cfg = _lib.BatchConfig()
     cfg.n_blocks = n_blocks
+    cfg.bs_min_sections = bs_min_sections
     off = np.ascontiguousarray(offsets, np.int64)
     nsa = np.ascontiguousarray(ns, np.int32)
     s = np.ascontiguousarray(sos, np.float32)
     h = ctypes.c_void_p()
-    st = _lib.lib().iirx_batch_create(ctypes.byref(cfg), nsa.size, off.ctypes.data, nsa.ctypes.data, s.ctypes.data,
-                                      ctypes.byref(h))
+    al = None if algo is None else np.ascontiguousarray(algo, np.int32)
+    st = _lib.lib().iirx_batch_create_ex(ctypes.byref(cfg), nsa.size, off.ctypes.data, nsa.ctypes.data,
+                                         s.ctypes.data, None if al is None else al.ctypes.data, ctypes.byref(h))
     if st == 0:
         _lib.lib().iirx_batch_destroy(h)
     return st
@@ -66,6 +68,17 @@ def test_batch_rejects_bad_channels():
     assert _create_status([0, 10, 20], [2, 2], bad) == 1          # non-finite coefficient
     assert _create_status([0, 10, 20], [2, 2], good, n_blocks=8) == 2  # no instance for N=8
     assert _create_status([0, 0, 0], [2, 2], good) == 1           # no samples at all
+    assert _create_status([0, 10, 20], [2, 2], good, algo=[0, 2]) == 1   # unknown algorithm
+    assert _create_status([0, 10, 20], [2, 2], good, algo=[-2, 0]) == 1
+    assert _create_status([0, 10, 20], [2, 2], good, bs_min_sections=-1) == 1
+
+
+def _algos(kind, n):
+    """Per-channel algorithm list of a test case: all PH, all block-state, or
+    alternating."""
+    if kind == "mixed":
+        return ["ph" if c % 2 else "blockstate" for c in range(n)]
+    return kind
 
 
 # --- GPU parity
@@ -73,19 +86,54 @@ def test_batch_rejects_bad_channels():
 @pytest.mark.gpu
 @pytest.mark.parametrize("n_blocks", [16, 32, 64])
 @pytest.mark.parametrize("schedule", ["persistent", "ticket"])
-def test_batch_edge_lengths(device, n_blocks, schedule):
+@pytest.mark.parametrize("algo", ["ph", "blockstate", "mixed"])
+def test_batch_edge_lengths(device, n_blocks, schedule, algo):
     import torch
-    print(f"case batch edge lengths N={n_blocks} {schedule}", flush=True)
+    print(f"case batch edge lengths N={n_blocks} {schedule} {algo}", flush=True)
     offsets, sections, gains = _explicit_channels()
     x = ragged_input(offsets, gains, seed=40)
     xin = torch.from_numpy(x).to(device)
     out = torch.full_like(xin, float("nan"))
-    with iirx.Batch(offsets, sections, n_blocks=n_blocks, schedule=schedule) as b:
+    with iirx.Batch(offsets, sections, n_blocks=n_blocks, schedule=schedule, algo=_algos(algo, len(sections))) as b:
         for _ in range(3):  # launch-versioned lookback: repeated launches on one batch
             b.execute(xin, out)
             b.check()
         y = out.cpu().numpy()
+        info = b.info()
+    assert info["mixed"] == (algo != "ph")
+    assert info["n_blockstate"] == {"ph": 0, "blockstate": len(sections), "mixed": (len(sections) + 1) // 2}[algo]
     _check_channels(y, x, offsets, sections)
+    # no write past any channel: the packed output is covered exactly once
+    assert np.isfinite(y).all()
+
+
+@pytest.mark.gpu
+def test_batch_algorithms_agree(device):
+    """The same channels run as PH and as the block-state form: both pass the
+    FP64 gate, and the block-state RMS error stays at PH's level (an operand
+    or slab mistake that stays within the max gate shows up there)."""
+    import torch
+    print("case batch PH vs block-state", flush=True)
+    offsets, sections, gains = ragged_channels(48, seed=6, hi_log2=15)
+    x = ragged_input(offsets, gains, seed=600)
+    xin = torch.from_numpy(x).to(device)
+    ys = {}
+    for algo in ("ph", "blockstate"):
+        out = torch.empty_like(xin)
+        with iirx.Batch(offsets, sections, algo=algo) as b:
+            b.execute(xin, out)
+            b.check()
+        ys[algo] = out.cpu().numpy()
+        _check_channels(ys[algo], x, offsets, sections)
+    rms = {k: [] for k in ys}
+    for c in range(len(sections)):
+        a, e = int(offsets[c]), int(offsets[c + 1])
+        ref = O.golden(sections[c], x[a:e])
+        scale = np.sqrt(np.mean(ref * ref))
+        for k in ys:
+            rms[k].append(np.sqrt(np.mean((ys[k][a:e] - ref) ** 2)) / scale)
+    print({k: float(np.median(v)) for k, v in rms.items()})
+    assert np.median(rms["blockstate"]) <= 2.0 * np.median(rms["ph"])
 
 
 @pytest.mark.gpu
@@ -97,11 +145,12 @@ def test_batch_ragged_workload(device):
     x = ragged_input(offsets, gains, seed=400)
     xin = torch.from_numpy(x).to(device)
     out = torch.empty_like(xin)
-    with iirx.Batch(offsets, sections) as b:
+    with iirx.Batch(offsets, sections, bs_min_sections=4) as b:
         b.execute(xin, out)
         b.check()
         info = b.info()
     assert info["n_channels"] == 256 and info["num_chunks"] > 256
+    assert info["n_blockstate"] == int(sum(s.shape[0] >= 4 for s in sections)) > 0
     _check_channels(out.cpu().numpy(), x, offsets, sections)
 
 
@@ -115,7 +164,7 @@ def test_batch_graph_replay(device):
     xin = torch.empty(int(offsets[-1]), dtype=torch.float32, device=device)
     out = torch.empty_like(xin)
     s = torch.cuda.Stream()
-    with iirx.Batch(offsets, sections) as b:
+    with iirx.Batch(offsets, sections, bs_min_sections=4) as b:
         b.capture(xin, out, s)
         assert b.info()["graph_ready"] == 1
         for r in range(4):
