@@ -47,7 +47,10 @@ def parse():
                          "sweep: order 8 at Wn 0.02/0.1/0.4 over lengths 2^16..2^25")
     ap.add_argument("--batch-n", type=int, default=32, help="ragged batch: samples per lane (16, 32 or 64)")
     ap.add_argument("--batch-bs-min-sections", type=int, default=0,
-                    help="ragged batch: channels with at least this many sections run the block-state form (0: none)")
+                    help="ragged batch: channels with at least this many sections run the block-state form "
+                         "(0: the library's choice, 8; 9: none)")
+    ap.add_argument("--batch-bs-tiles", type=int, default=32,
+                    help="ragged batch: block-state tiles of 64 samples per chunk (16, 32 or 64)")
     ap.add_argument("--copy-steps", type=int, default=50,
                     help="device-to-device copies of one 2^25-sample buffer, timed as the measured HBM ceiling "
                          "(also the known-size kernel that scales rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -154,7 +157,7 @@ def bench_ragged(args):
     out = torch.empty_like(x)
     s = torch.cuda.Stream()
     b = iirx.Batch(offsets, sections, n_blocks=args.batch_n, schedule=args.schedule,
-                   bs_min_sections=args.batch_bs_min_sections)
+                   bs_min_sections=args.batch_bs_min_sections, bs_tiles=args.batch_bs_tiles)
     b.capture(x, out, s)
     for _ in range(args.warmup):
         b.replay(s)
@@ -186,9 +189,10 @@ def bench_ragged(args):
         "config": {"workload": "ragged batch: 4096 channels, lengths log-uniform [2^10, 2^20], order uniform "
                                "{2,4,8,16}, Butterworth Wn uniform [0.02, 0.45] (seed 4), gain * N(0,1)",
                    "n_samples": n, "n_channels": 4096,
-                   "kernel": ("PH per chunk" if not info["mixed"] else
-                              f"PH, block-state for >= {args.batch_bs_min_sections} sections"),
+                   "kernel": ("PH per chunk" if not info["n_blockstate"] else
+                              f"PH, block-state for >= {args.batch_bs_min_sections or 8} sections"),
                    "blockstate_channels": info["n_blockstate"], "samples_per_lane": args.batch_n,
+                   "blockstate_tiles": args.batch_bs_tiles, "launches_per_step": info["n_launches"],
                    "chunks": info["num_chunks"], "grid": info["grid"], "schedule": args.schedule},
         "eager_ms_per_step": round(ms_eager, 4),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -270,11 +270,13 @@ class Batch:
                  for every channel, or a sequence of them (or of the
                  BATCH_ALGOS codes), one per channel
     bs_min_sections  'auto' channels with at least this many sections run
-                 the block-state form (0: none)
+                 the block-state form (0: the library's choice, 8 sections =
+                 order 16; 9: none)
+    bs_tiles     block-state tiles of 64 samples per chunk (16, 32 or 64)
     """
 
     def __init__(self, offsets, sections, n_sections=None, n_blocks=32, schedule="persistent", algo="auto",
-                 bs_min_sections=0):
+                 bs_min_sections=0, bs_tiles=32):
         off = np.ascontiguousarray(np.asarray(offsets, np.int64))
         if n_sections is None:
             secs = [_sos_array(s) for s in sections]
@@ -293,6 +295,7 @@ class Batch:
         self._cfg.n_blocks = int(n_blocks)
         self._cfg.schedule = _lib.SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
         self._cfg.bs_min_sections = int(bs_min_sections)
+        self._cfg.bs_tiles = int(bs_tiles)
         if isinstance(algo, str):
             algo = [algo] * self.n_channels
         codes = np.ascontiguousarray(

@@ -75,7 +75,7 @@ class BatchConfig(ctypes.Structure):
         ("n_blocks", ctypes.c_int32),
         ("schedule", ctypes.c_int32),
         ("bs_min_sections", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("bs_tiles", ctypes.c_int32),
     ]
 
 
@@ -90,8 +90,13 @@ class BatchInfo(ctypes.Structure):
         ("vgprs", ctypes.c_int32),
         ("waves_per_simd", ctypes.c_int32),
         ("graph_ready", ctypes.c_int32),
-        ("mixed", ctypes.c_int32),
+        ("n_launches", ctypes.c_int32),
         ("n_blockstate", ctypes.c_int64),
+        ("bs_chunks", ctypes.c_int64),
+        ("bs_chunk_samples", ctypes.c_int64),
+        ("bs_grid", ctypes.c_int64),
+        ("bs_vgprs", ctypes.c_int32),
+        ("bs_waves_per_simd", ctypes.c_int32),
     ]
 
 

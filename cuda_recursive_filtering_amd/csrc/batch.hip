@@ -10,9 +10,10 @@
 // filter is causal, so they cannot reach earlier outputs) and not written.
 // Channel offsets are arbitrary (CSR packing), so HBM is accessed with
 // 4-byte coalesced loads and stores: 64 consecutive floats per instruction.
-// A channel may instead run the block-state form (bs_core.hpp) with tiles of
-// 64 samples and N tiles per chunk (the same 64 N samples), its tables read
-// from global memory; the MIXED instance carries both paths.
+// A channel may instead run the block-state form (bs_core.hpp): tiles of 64
+// samples, G tiles per chunk, its tables read from global memory.  The two
+// forms are separate kernels, each with its own register budget, launched one
+// after the other over their own ranges of the chunk list (PH chunks first).
 #include "bs_core.hpp"
 
 namespace iirx {
@@ -20,10 +21,12 @@ namespace iirx {
 typedef const __attribute__((address_space(4))) BatchChannel cchannel;
 typedef const __attribute__((address_space(4))) uint32_t cu32;
 
-// waves per SIMD the register budget is set for: at N = 32 four waves (a
-// 20-byte/lane spill) measured 5% faster than three without one
-template <int N, bool MIXED>
-constexpr int batch_waves() { return MIXED ? (N == 64 ? 1 : N == 32 ? 3 : 4) : (N == 16 ? 5 : N == 32 ? 4 : 1); }
+// waves per SIMD the register budgets are set for: PH at N = 32 four waves
+// (a 20-byte/lane spill) measured 5% faster than three without one
+template <int N>
+constexpr int batch_waves() { return N == 16 ? 5 : N == 32 ? 4 : 1; }
+template <int G>
+constexpr int batch_bs_waves() { return G == 16 ? 5 : G == 32 ? 4 : 2; }
 
 // block-state operands of one channel, from its tables in global memory
 struct BsGlobalOps {
@@ -106,24 +109,48 @@ __device__ __forceinline__ void batch_bs_chunk(const BatchArgs &a, uint32_t chun
     }
 }
 
-template <int N, bool MIXED>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(batch_waves<N, MIXED>(), batch_waves<N, MIXED>())))
+// block-state channels: chunks chunk_base .. chunk_base + num_chunks - 1
+template <int G>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(batch_bs_waves<G>(), batch_bs_waves<G>())))
+void batch_bs_kernel(BatchArgs a)
+{
+    constexpr int CH = 64 * G;
+    __shared__ __align__(16) float lds[kBsTilePitch * G];
+    __shared__ float lp[kBsSlots * 16];
+    const cchannel *chan = (const cchannel *)a.chan;
+    const cu32 *chunk_chan = (const cu32 *)a.chunk_chan;
+
+    for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err,
+                   [&](uint32_t local, uint32_t part_flag, uint32_t full_flag) {
+        const uint32_t chunk = a.chunk_base + local;
+        const uint32_t ci = chunk_chan[chunk];
+        const uint32_t first = chan[ci].first_chunk;
+        const long j = (long)(chunk - first);
+        const long base = chan[ci].offset + j * CH;
+        const long rem = chan[ci].length - j * CH;
+        const int r = rem < CH ? (int)rem : CH;
+        batch_bs_chunk<G>(a, chunk, part_flag, full_flag, a.coef + chan[ci].coef_off, first, base, r, lds, lp);
+    });
+}
+
+// PH channels: chunks chunk_base .. chunk_base + num_chunks - 1
+template <int N>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(batch_waves<N>(), batch_waves<N>())))
 void batch_ph_kernel(BatchArgs a)
 {
     constexpr int L = 64;
     constexpr int CH = L * N;
     constexpr PhLayout Y = ph_layout(L, N);
-    constexpr int kLds = L * (N + 1), kLdsBs = kBsTilePitch * N;
-    __shared__ __align__(16) float lds[MIXED && kLdsBs > kLds ? kLdsBs : kLds];
-    __shared__ float lp[MIXED ? kBsSlots * 16 : 1];  // block-state lookback slots
+    __shared__ float lds[L * (N + 1)];
 
     const int lane = threadIdx.x;
     const cchannel *chan = (const cchannel *)a.chan;
     const cu32 *chunk_chan = (const cu32 *)a.chunk_chan;
 
     for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err,
-                   [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
+                   [&](uint32_t local, uint32_t part_flag, uint32_t full_flag) {
     // channel of this chunk (uniform: scalar loads)
+    const uint32_t chunk = a.chunk_base + local;
     const uint32_t ci = chunk_chan[chunk];
     const long ch_off = chan[ci].offset, ch_len = chan[ci].length;
     const int ns = chan[ci].ns;
@@ -133,12 +160,6 @@ void batch_ph_kernel(BatchArgs a)
     const long base = ch_off + j * CH;
     const long rem = ch_len - j * CH;
     const int r = rem < CH ? (int)rem : CH;  // valid samples in this chunk
-    if constexpr (MIXED) {
-        if (chan[ci].algo == kBatchBS) {
-            batch_bs_chunk<N>(a, chunk, part_flag, full_flag, coef, first, base, r, lds, lp);
-            return;
-        }
-    }
 
     // stage in: chunk element e = 64 i + lane (coalesced), lane t owns
     // [t N, t N + N).  Full chunks use one base address and immediate offsets;
@@ -272,31 +293,48 @@ void batch_ph_kernel(BatchArgs a)
     });
 }
 
-template <int N, bool MIXED>
-static hipError_t launch_n(const BatchArgs &a, hipStream_t s)
+template <class K>
+static hipError_t launch_k(K kernel, const BatchArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((batch_ph_kernel<N, MIXED>), dim3(a.grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(kernel, dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_batch(int n_blocks, bool mixed, const BatchArgs &a, hipStream_t s)
+// algo kBatchPH: n = samples per lane (16, 32, 64); kBatchBS: n = tiles per chunk
+const void *kernel_batch(int algo, int n)
 {
-    switch (n_blocks) {
-    case 16: return mixed ? launch_n<16, true>(a, s) : launch_n<16, false>(a, s);
-    case 32: return mixed ? launch_n<32, true>(a, s) : launch_n<32, false>(a, s);
-    case 64: return mixed ? launch_n<64, true>(a, s) : launch_n<64, false>(a, s);
-    default: return hipErrorInvalidValue;
+    if (algo == kBatchPH) {
+        switch (n) {
+        case 16: return (const void *)batch_ph_kernel<16>;
+        case 32: return (const void *)batch_ph_kernel<32>;
+        case 64: return (const void *)batch_ph_kernel<64>;
+        }
+    } else if (algo == kBatchBS) {
+        switch (n) {
+        case 16: return (const void *)batch_bs_kernel<16>;
+        case 32: return (const void *)batch_bs_kernel<32>;
+        case 64: return (const void *)batch_bs_kernel<64>;
+        }
     }
+    return nullptr;
 }
 
-const void *kernel_batch(int n_blocks, bool mixed)
+hipError_t launch_batch(int algo, int n, const BatchArgs &a, hipStream_t s)
 {
-    switch (n_blocks) {
-    case 16: return mixed ? (const void *)batch_ph_kernel<16, true> : (const void *)batch_ph_kernel<16, false>;
-    case 32: return mixed ? (const void *)batch_ph_kernel<32, true> : (const void *)batch_ph_kernel<32, false>;
-    case 64: return mixed ? (const void *)batch_ph_kernel<64, true> : (const void *)batch_ph_kernel<64, false>;
-    default: return nullptr;
+    if (algo == kBatchPH) {
+        switch (n) {
+        case 16: return launch_k(batch_ph_kernel<16>, a, s);
+        case 32: return launch_k(batch_ph_kernel<32>, a, s);
+        case 64: return launch_k(batch_ph_kernel<64>, a, s);
+        }
+    } else if (algo == kBatchBS) {
+        switch (n) {
+        case 16: return launch_k(batch_bs_kernel<16>, a, s);
+        case 32: return launch_k(batch_bs_kernel<32>, a, s);
+        case 64: return launch_k(batch_bs_kernel<64>, a, s);
+        }
     }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace iirx

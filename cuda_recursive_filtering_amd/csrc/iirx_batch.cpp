@@ -19,22 +19,29 @@
 
 using namespace iirx;
 
+// one launch per form present: PH chunks [0, base_bs), block-state chunks
+// [base_bs, num_chunks), each with its own control block
+struct BatchGroup {
+    int algo, n;  // kernel_batch(algo, n)
+    uint32_t base, num_chunks, grid;
+    const void *fn;
+    unsigned long long *d_ctrl;  // [ticket u64][epoch u32][err u32], arrival slots from byte 128
+};
+
 struct iirx_batch {
     iirx_batch_config cfg;
     int64_t n_channels, n_samples;
-    uint32_t num_chunks, grid, spin_limit;
-    int N, coef_stride, device;
-    bool mixed;
+    uint32_t num_chunks, spin_limit;
+    int coef_stride, device;
     int64_t n_blockstate;
+    BatchGroup grp[2];  // [0] PH, [1] block-state
     float *d_coef;
     uint64_t *d_lb;
     BatchChannel *d_chan;
     uint32_t *d_chunk_chan;
-    unsigned long long *d_ctrl;  // [ticket u64][epoch u32][err u32], arrival slots from byte 128
     hipGraph_t graph;
     hipGraphExec_t exec;
     int64_t launches;
-    const void *fn;
 };
 
 namespace {
@@ -47,11 +54,12 @@ void release(iirx_batch *b)
     if (b->d_lb) hipFree(b->d_lb);
     if (b->d_chan) hipFree(b->d_chan);
     if (b->d_chunk_chan) hipFree(b->d_chunk_chan);
-    if (b->d_ctrl) hipFree(b->d_ctrl);
+    for (BatchGroup &g : b->grp)
+        if (g.d_ctrl) hipFree(g.d_ctrl);
     delete b;
 }
 
-BatchArgs args(const iirx_batch *b, const float *d_in, float *d_out)
+BatchArgs args(const iirx_batch *b, const BatchGroup &g, const float *d_in, float *d_out)
 {
     BatchArgs a{};
     a.in = d_in;
@@ -60,16 +68,28 @@ BatchArgs args(const iirx_batch *b, const float *d_in, float *d_out)
     a.lb = b->d_lb;
     a.chan = b->d_chan;
     a.chunk_chan = b->d_chunk_chan;
-    a.ticket = b->d_ctrl;
-    a.epoch = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(b->d_ctrl) + 8);
+    a.ticket = g.d_ctrl;
+    a.epoch = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(g.d_ctrl) + 8);
     a.err = a.epoch + 1;
-    a.arrive = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(b->d_ctrl) + 128);
-    a.num_chunks = b->num_chunks;
+    a.arrive = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(g.d_ctrl) + 128);
+    a.num_chunks = g.num_chunks;
     a.spin_limit = b->spin_limit;
     a.persistent = b->cfg.schedule == IIRX_SCHEDULE_PERSISTENT;
-    a.grid = b->grid;
+    a.grid = g.grid;
     a.coef_stride = b->coef_stride;
+    a.chunk_base = g.base;
     return a;
+}
+
+// every form's launch, in chunk order, on one stream
+hipError_t launch_all(const iirx_batch *b, const float *d_in, float *d_out, hipStream_t s)
+{
+    for (const BatchGroup &g : b->grp) {
+        if (g.num_chunks == 0) continue;
+        const hipError_t e = launch_batch(g.algo, g.n, args(b, g, d_in, d_out), s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace
@@ -88,7 +108,8 @@ int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels, const
     if (!cfg || !offsets || !n_sections || !sos || !out || n_channels < 1) return IIRX_ERR_INVALID;
     *out = nullptr;
     const int N = cfg->n_blocks == 0 ? 32 : cfg->n_blocks;
-    if (!(N == 16 || N == 32 || N == 64)) return IIRX_ERR_GEOMETRY;
+    const int G = cfg->bs_tiles == 0 ? 32 : cfg->bs_tiles;
+    if (!(N == 16 || N == 32 || N == 64) || !(G == 16 || G == 32 || G == 64)) return IIRX_ERR_GEOMETRY;
     if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && cfg->schedule != IIRX_SCHEDULE_TICKET) return IIRX_ERR_INVALID;
     if (offsets[0] < 0 || cfg->bs_min_sections < 0) return IIRX_ERR_INVALID;
     int64_t total_sec = 0;
@@ -104,11 +125,17 @@ int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels, const
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IIRX_ERR_NO_DEVICE;
 
-    // per-channel tables, chunk list channel after channel
-    const int64_t CH = 64LL * N;
+    // per-channel tables; chunk list: the PH channels' chunks channel after
+    // channel, then the block-state channels'.  AUTO channels of order 16 or
+    // more take the block-state form by default: in the batch it measured
+    // 1.38x PH at order 16 and 0.86x at order 8 (Butterworth, 2^18-sample
+    // channels).
+    const int min_bs = cfg->bs_min_sections == 0 ? kBatchMaxSections : cfg->bs_min_sections;
+    const int64_t CHP = 64LL * N, CHB = 64LL * G;
     std::vector<float> coef;
     std::vector<BatchChannel> chan((size_t)n_channels);
-    std::vector<uint32_t> chunk_chan;
+    std::vector<uint32_t> chunks[2];
+    std::vector<char> is_bs((size_t)n_channels, 0);
     int stride = 0;
     int64_t sec0 = 0, n_bs = 0;
     for (int64_t c = 0; c < n_channels; ++c) {
@@ -116,9 +143,9 @@ int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels, const
         const int ns = n_sections[c];
         const int32_t want = algo ? algo[c] : IIRX_BATCH_AUTO;
         const bool bs = want == IIRX_BATCH_BLOCKSTATE ||
-                        (want == IIRX_BATCH_AUTO && cfg->bs_min_sections > 0 && ns >= cfg->bs_min_sections);
+                        (want == IIRX_BATCH_AUTO && ns >= min_bs);
         if (bs) {
-            if (bs_tables(sos + 6 * sec0, ns, 64, N, t) <= 0) return IIRX_ERR_INVALID;
+            if (bs_tables(sos + 6 * sec0, ns, 64, G, t) <= 0) return IIRX_ERR_INVALID;
             ++n_bs;
         } else {
             const int st = ph_tables(sos + 6 * sec0, ns, 64, N, t);
@@ -128,41 +155,51 @@ int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels, const
         sec0 += ns;
         t.resize((t.size() + 3) & ~(size_t)3, 0.0f);  // 16-byte aligned tables
         const int64_t len = offsets[c + 1] - offsets[c];
+        const int64_t CH = bs ? CHB : CHP;
         const int64_t k = (len + CH - 1) / CH;
-        if ((int64_t)chunk_chan.size() + k > 0x7fffffffLL || coef.size() + t.size() > 0x7fffffffULL)
+        if ((int64_t)(chunks[0].size() + chunks[1].size()) + k > 0x7fffffffLL ||
+            coef.size() + t.size() > 0x7fffffffULL)
             return IIRX_ERR_GEOMETRY;
         BatchChannel &d = chan[(size_t)c];
         d.offset = offsets[c];
         d.length = len;
         d.ns = ns;
         d.coef_off = (int32_t)coef.size();
-        d.first_chunk = (uint32_t)chunk_chan.size();
+        d.first_chunk = (uint32_t)chunks[bs].size();  // within its form's list until merged
         d.algo = bs ? kBatchBS : kBatchPH;
+        is_bs[(size_t)c] = bs;
         coef.insert(coef.end(), t.begin(), t.end());
-        chunk_chan.insert(chunk_chan.end(), (size_t)k, (uint32_t)c);
+        chunks[bs].insert(chunks[bs].end(), (size_t)k, (uint32_t)c);
     }
+    const uint32_t n_ph = (uint32_t)chunks[0].size();
+    for (int64_t c = 0; c < n_channels; ++c)
+        if (is_bs[(size_t)c]) chan[(size_t)c].first_chunk += n_ph;
+    std::vector<uint32_t> chunk_chan(chunks[0]);
+    chunk_chan.insert(chunk_chan.end(), chunks[1].begin(), chunks[1].end());
     if (chunk_chan.empty()) return IIRX_ERR_INVALID;
 
     iirx_batch *b = new (std::nothrow) iirx_batch();
     if (!b) return IIRX_ERR_OOM;
     b->cfg = *cfg;
     b->cfg.n_blocks = N;
+    b->cfg.bs_tiles = G;
+    b->cfg.bs_min_sections = min_bs;
     b->n_channels = n_channels;
     b->n_samples = offsets[n_channels] - offsets[0];
     b->num_chunks = (uint32_t)chunk_chan.size();
-    b->N = N;
     b->coef_stride = stride;
-    b->mixed = n_bs > 0;
     b->n_blockstate = n_bs;
     b->spin_limit = 1u << 22;
     hipGetDevice(&b->device);
-    b->fn = kernel_batch(N, b->mixed);
+    b->grp[0] = BatchGroup{kBatchPH, N, 0u, n_ph, 0u, kernel_batch(kBatchPH, N), nullptr};
+    b->grp[1] = BatchGroup{kBatchBS, G, n_ph, (uint32_t)chunks[1].size(), 0u, kernel_batch(kBatchBS, G), nullptr};
     const size_t lb_count = (size_t)b->num_chunks * kBatchMaxSections * 4u;
     if (hipMalloc((void **)&b->d_coef, coef.size() * sizeof(float)) != hipSuccess ||
         hipMalloc((void **)&b->d_lb, lb_count * sizeof(uint64_t)) != hipSuccess ||
         hipMalloc((void **)&b->d_chan, chan.size() * sizeof(BatchChannel)) != hipSuccess ||
         hipMalloc((void **)&b->d_chunk_chan, chunk_chan.size() * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc((void **)&b->d_ctrl, kControlBytes) != hipSuccess) {
+        hipMalloc((void **)&b->grp[0].d_ctrl, kControlBytes) != hipSuccess ||
+        hipMalloc((void **)&b->grp[1].d_ctrl, kControlBytes) != hipSuccess) {
         release(b);
         return IIRX_ERR_OOM;
     }
@@ -172,19 +209,26 @@ int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels, const
             hipSuccess ||
         // zeroed once; launch-versioned flags never need a reset afterwards
         hipMemset(b->d_lb, 0, lb_count * sizeof(uint64_t)) != hipSuccess ||
-        hipMemset(b->d_ctrl, 0, kControlBytes) != hipSuccess) {
+        hipMemset(b->grp[0].d_ctrl, 0, kControlBytes) != hipSuccess ||
+        hipMemset(b->grp[1].d_ctrl, 0, kControlBytes) != hipSuccess) {
         release(b);
         return IIRX_ERR_HIP;
     }
-    b->grid = b->num_chunks;
-    if (cfg->schedule == IIRX_SCHEDULE_PERSISTENT) {
-        int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn, 64, 0) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess) {
-            release(b);
-            return IIRX_ERR_HIP;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess) {
+        release(b);
+        return IIRX_ERR_HIP;
+    }
+    for (BatchGroup &g : b->grp) {
+        g.grid = g.num_chunks;
+        if (cfg->schedule == IIRX_SCHEDULE_PERSISTENT && g.num_chunks > 0) {
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, g.fn, 64, 0) != hipSuccess) {
+                release(b);
+                return IIRX_ERR_HIP;
+            }
+            g.grid = (uint32_t)std::min<int64_t>((int64_t)std::max(per_cu, 1) * std::max(cus, 1), g.num_chunks);
         }
-        b->grid = (uint32_t)std::min<int64_t>((int64_t)std::max(per_cu, 1) * std::max(cus, 1), b->num_chunks);
     }
     if (hipDeviceSynchronize() != hipSuccess) {
         release(b);
@@ -197,7 +241,7 @@ int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels, const
 int iirx_batch_execute(iirx_batch *b, const float *d_in, float *d_out, void *stream)
 {
     if (!b || !d_in || !d_out) return IIRX_ERR_INVALID;
-    if (launch_batch(b->N, b->mixed, args(b, d_in, d_out), (hipStream_t)stream) != hipSuccess) return IIRX_ERR_HIP;
+    if (launch_all(b, d_in, d_out, (hipStream_t)stream) != hipSuccess) return IIRX_ERR_HIP;
     b->launches++;
     return IIRX_OK;
 }
@@ -215,7 +259,7 @@ int iirx_batch_capture(iirx_batch *b, const float *d_in, float *d_out, void *str
         b->graph = nullptr;
     }
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return IIRX_ERR_HIP;
-    const hipError_t le = launch_batch(b->N, b->mixed, args(b, d_in, d_out), s);
+    const hipError_t le = launch_all(b, d_in, d_out, s);
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(s, &g);
     if (le != hipSuccess || ee != hipSuccess || !g) {
@@ -244,15 +288,18 @@ int iirx_batch_check(iirx_batch *b)
 {
     if (!b) return IIRX_ERR_INVALID;
     if (hipDeviceSynchronize() != hipSuccess) return IIRX_ERR_HIP;
-    uint32_t *d_err = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(b->d_ctrl) + 12);
-    uint32_t err = 0;
-    if (hipMemcpy(&err, d_err, sizeof err, hipMemcpyDeviceToHost) != hipSuccess) return IIRX_ERR_HIP;
-    if (err) {
-        const uint32_t zero = 0;
-        if (hipMemcpy(d_err, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) return IIRX_ERR_HIP;
-        return IIRX_ERR_TIMEOUT;
+    int st = IIRX_OK;
+    for (const BatchGroup &g : b->grp) {
+        uint32_t *d_err = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(g.d_ctrl) + 12);
+        uint32_t err = 0;
+        if (hipMemcpy(&err, d_err, sizeof err, hipMemcpyDeviceToHost) != hipSuccess) return IIRX_ERR_HIP;
+        if (err) {
+            const uint32_t zero = 0;
+            if (hipMemcpy(d_err, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) return IIRX_ERR_HIP;
+            st = IIRX_ERR_TIMEOUT;
+        }
     }
-    return IIRX_OK;
+    return st;
 }
 
 int iirx_batch_info_get(const iirx_batch *b, iirx_batch_info *info)
@@ -262,16 +309,24 @@ int iirx_batch_info_get(const iirx_batch *b, iirx_batch_info *info)
     info->n_channels = b->n_channels;
     info->n_samples = b->n_samples;
     info->num_chunks = b->num_chunks;
-    info->chunk_samples = 64LL * b->N;
-    info->grid = b->grid;
+    info->chunk_samples = 64LL * b->cfg.n_blocks;
+    info->grid = b->grp[0].grid;
     info->launches = b->launches;
     info->graph_ready = b->exec != nullptr;
-    info->mixed = b->mixed;
+    info->n_launches = (b->grp[0].num_chunks > 0) + (b->grp[1].num_chunks > 0);
     info->n_blockstate = b->n_blockstate;
+    info->bs_chunks = b->grp[1].num_chunks;
+    info->bs_chunk_samples = 64LL * b->cfg.bs_tiles;
+    info->bs_grid = b->grp[1].grid;
     hipFuncAttributes at;
-    if (hipFuncGetAttributes(&at, b->fn) == hipSuccess) info->vgprs = at.numRegs;
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, b->fn, 64, 0) == hipSuccess) info->waves_per_simd = nb / 4;
+    for (int k = 0; k < 2; ++k) {
+        const BatchGroup &g = b->grp[k];
+        int32_t &vg = k ? info->bs_vgprs : info->vgprs;
+        int32_t &ws = k ? info->bs_waves_per_simd : info->waves_per_simd;
+        if (hipFuncGetAttributes(&at, g.fn) == hipSuccess) vg = at.numRegs;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, g.fn, 64, 0) == hipSuccess) ws = nb / 4;
+    }
     (void)hipGetLastError();
     return IIRX_OK;
 }

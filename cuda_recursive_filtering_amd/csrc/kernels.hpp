@@ -233,9 +233,12 @@ constexpr int bs_wg_waves(int L, int G)
 // ---------------------------------------------------------------------------
 // Ragged batch (batch.hip): channel c holds samples [offset, offset + length)
 // of the packed input and output, chunks first_chunk .. first_chunk +
-// ceil(length / chunk) - 1 of the global chunk list, its PH tables at
-// coef + coef_off (ns sections, coef_stride floats each).  Lookback records
-// are laid out for kBatchMaxSections sections per chunk.
+// ceil(length / chunk) - 1 of the global chunk list (PH channels' chunks
+// first, then the block-state channels'; the chunk size is that of the
+// channel's form), its tables at coef + coef_off (PH: ns sections,
+// coef_stride floats each; block-state: bs_layout(64)).  Lookback records are
+// 32 words per chunk: kBatchMaxSections PH sections of 4 words, or the 16
+// partial + 16 full state words of the block-state form.
 // ---------------------------------------------------------------------------
 constexpr int kBatchMaxSections = 8;
 
@@ -248,7 +251,7 @@ struct BatchChannel {
     int32_t algo;  // kBatchPH or kBatchBS
 };
 constexpr int32_t kBatchPH = 0;
-constexpr int32_t kBatchBS = 1;  // block-state form, tiles of 64 samples, N tiles per chunk
+constexpr int32_t kBatchBS = 1;  // block-state form, tiles of 64 samples
 
 struct BatchArgs {
     const float *in;
@@ -266,11 +269,12 @@ struct BatchArgs {
     int32_t persistent;
     uint32_t grid;
     int32_t coef_stride;
-    int32_t pad;
+    uint32_t chunk_base;  // this launch's chunks: chunk_base .. chunk_base + num_chunks - 1
 };
 
-hipError_t launch_batch(int n_blocks, bool mixed, const BatchArgs &a, hipStream_t s);
-const void *kernel_batch(int n_blocks, bool mixed);
+// algo kBatchPH: n = samples per lane; kBatchBS: n = tiles per chunk
+hipError_t launch_batch(int algo, int n, const BatchArgs &a, hipStream_t s);
+const void *kernel_batch(int algo, int n);
 
 hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s);
 const void *kernel_bs(int L, int G, int ns);

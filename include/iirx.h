@@ -282,10 +282,12 @@ int iirx_autotune(const float *sos, int32_t n_sections, int64_t n_samples,
  * persistent launch filters every channel, chunk by chunk (64 x n_blocks
  * samples), with lookback chains per channel; a channel's last chunk may be
  * partial.  Each channel runs PH (ph_kernels.cuh PH_64) or the block-state
- * form (tiles of 64 samples, n_blocks tiles per chunk, matrix cores); both
- * give the reference's recurrence within the FP32 bounds of iirx_plan.  The
- * launch can be captured into a hipGraph and replayed: the lookback epoch
- * lives on the device, so replays stay correct.
+ * form (tiles of 64 samples, bs_tiles tiles per chunk, matrix cores); both
+ * give the reference's recurrence within the FP32 bounds of iirx_plan.  When
+ * both forms are present they are two persistent launches, one after the
+ * other on the stream, each with its own register budget.  The launches can
+ * be captured into a hipGraph and replayed: the lookback epochs live on the
+ * device, so replays stay correct.
  * ------------------------------------------------------------------------ */
 typedef struct iirx_batch iirx_batch;
 
@@ -293,9 +295,11 @@ typedef struct {
     int32_t n_blocks;         /* samples per lane: 16, 32 or 64 (0 = 32)     */
     int32_t schedule;         /* iirx_schedule                               */
     int32_t bs_min_sections;  /* IIRX_BATCH_AUTO channels with at least this
-                                 many sections run the block-state form
-                                 (0 = none)                                  */
-    int32_t reserved;
+                                 many sections run the block-state form (0 =
+                                 8, i.e. order 16, where it measured faster
+                                 than PH; above 8 = none)                    */
+    int32_t bs_tiles;         /* block-state tiles per chunk: 16, 32 or 64
+                                 (0 = 32)                                    */
 } iirx_batch_config;
 
 /* per-channel algorithm of iirx_batch_create_ex */
@@ -308,14 +312,18 @@ enum {
 typedef struct {
     int64_t n_channels;
     int64_t n_samples;      /* offsets[n_channels] - offsets[0] */
-    int64_t num_chunks;
-    int64_t chunk_samples;
-    int64_t grid;
-    int64_t launches;
-    int32_t vgprs, waves_per_simd;
+    int64_t num_chunks;     /* both forms */
+    int64_t chunk_samples;  /* PH chunk */
+    int64_t grid;           /* PH launch */
+    int64_t launches;       /* execute/replay calls so far */
+    int32_t vgprs, waves_per_simd;        /* PH kernel */
     int32_t graph_ready;    /* a captured graph can be replayed */
-    int32_t mixed;          /* the launch carries the block-state path */
+    int32_t n_launches;     /* kernel launches per call: one per form present */
     int64_t n_blockstate;   /* channels that run the block-state form */
+    int64_t bs_chunks;
+    int64_t bs_chunk_samples;
+    int64_t bs_grid;
+    int32_t bs_vgprs, bs_waves_per_simd;  /* block-state kernel */
 } iirx_batch_info;
 
 /* Validate the channels and build the per-channel FP64 tables on the host,

@@ -39,12 +39,13 @@ def _check_channels(y, x, offsets, sections, chans=None):
 
 # --- host-side validation (no device needed: it is checked before any HIP call)
 
-def _create_status(offsets, ns, sos, n_blocks=16, algo=None, bs_min_sections=0):
+def _create_status(offsets, ns, sos, n_blocks=16, algo=None, bs_min_sections=0, bs_tiles=0):
     import ctypes
     from cuda_recursive_filtering_amd import _lib
     cfg = _lib.BatchConfig()
     cfg.n_blocks = n_blocks
     cfg.bs_min_sections = bs_min_sections
+    cfg.bs_tiles = bs_tiles
     off = np.ascontiguousarray(offsets, np.int64)
     nsa = np.ascontiguousarray(ns, np.int32)
     s = np.ascontiguousarray(sos, np.float32)
@@ -67,6 +68,7 @@ def test_batch_rejects_bad_channels():
     bad[1, 4] = np.nan
     assert _create_status([0, 10, 20], [2, 2], bad) == 1          # non-finite coefficient
     assert _create_status([0, 10, 20], [2, 2], good, n_blocks=8) == 2  # no instance for N=8
+    assert _create_status([0, 10, 20], [2, 2], good, bs_tiles=48) == 2  # nor for 48 block-state tiles
     assert _create_status([0, 0, 0], [2, 2], good) == 1           # no samples at all
     assert _create_status([0, 10, 20], [2, 2], good, algo=[0, 2]) == 1   # unknown algorithm
     assert _create_status([0, 10, 20], [2, 2], good, algo=[-2, 0]) == 1
@@ -94,13 +96,14 @@ def test_batch_edge_lengths(device, n_blocks, schedule, algo):
     x = ragged_input(offsets, gains, seed=40)
     xin = torch.from_numpy(x).to(device)
     out = torch.full_like(xin, float("nan"))
-    with iirx.Batch(offsets, sections, n_blocks=n_blocks, schedule=schedule, algo=_algos(algo, len(sections))) as b:
+    with iirx.Batch(offsets, sections, n_blocks=n_blocks, schedule=schedule, algo=_algos(algo, len(sections)),
+                    bs_tiles=n_blocks) as b:
         for _ in range(3):  # launch-versioned lookback: repeated launches on one batch
             b.execute(xin, out)
             b.check()
         y = out.cpu().numpy()
         info = b.info()
-    assert info["mixed"] == (algo != "ph")
+    assert info["n_launches"] == (2 if algo == "mixed" else 1)
     assert info["n_blockstate"] == {"ph": 0, "blockstate": len(sections), "mixed": (len(sections) + 1) // 2}[algo]
     _check_channels(y, x, offsets, sections)
     # no write past any channel: the packed output is covered exactly once
@@ -199,6 +202,9 @@ def test_batch_full_ragged_workload(device):
     with iirx.Batch(offsets, sections) as b:
         b.execute(xin, out)
         b.check()
+        info = b.info()
+    # by default the order-16 channels run the block-state form, in a second launch
+    assert info["n_blockstate"] == sum(s.shape[0] == 8 for s in sections) and info["n_launches"] == 2
     x_cpu = {c: xin[int(offsets[c]):int(offsets[c + 1])].cpu().numpy() for c in chans}
     y_cpu = {c: out[int(offsets[c]):int(offsets[c + 1])].cpu().numpy() for c in chans}
     for c in chans:
