@@ -38,7 +38,8 @@ std::vector<iirx_config> enumerate(int32_t ns, uint32_t flags)
                 for (int u = 0; u < 2; ++u) add(k, 64, N, u);
     if (!(flags & IIRX_TUNE_NO_BLOCKSTATE))
         for (int G : {16, 32, 64})
-            for (int L : {32, 64}) add(IIRX_KERNEL_BLOCKSTATE, G, L, 0);
+            for (int L : {32, 64, 128})
+                if (L * G <= 4096) add(IIRX_KERNEL_BLOCKSTATE, G, L, 0);
     if (ns <= 2 || (flags & IIRX_TUNE_PLR_ALL_ORDERS)) add(IIRX_KERNEL_PLR, 0, 0, 0);
     return c;
 }

@@ -4,8 +4,8 @@
 // ph/stcr/dtcr (the whole cascade, zero initial state) in a form whose cost
 // grows with the order only through the state dimension of small products.
 // The cascade is one state-space system of dimension p = 2 N_SECTIONS,
-// embedded in 16 rows.  A chunk of G (16, 32 or 64) tiles of L samples is one
-// wave64:
+// embedded in 16 rows.  A chunk of G (16, 32 or 64) tiles of L (32, 64 or
+// 128; G <= 32 at L = 128) samples is one wave64:
 //   1. the zero-state parts Y0 = T X (T: L x L lower-triangular Toeplitz of
 //      the impulse response) and V = Gam X (Gam: 16 x L), as
 //      v_mfma_f32_16x16x32_f16 products of FP16 hi/lo operands
@@ -220,7 +220,7 @@ static hipError_t launch_l(int G, int nsl, const LaunchArgs &a, hipStream_t s)
     switch (G) {
     case 16: return launch_lg<L, 16>(nsl, a, s);
     case 32: return launch_lg<L, 32>(nsl, a, s);
-    case 64: return launch_lg<L, 64>(nsl, a, s);
+    case 64: if constexpr (L < 128) return launch_lg<L, 64>(nsl, a, s); else return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 }
@@ -231,7 +231,7 @@ static const void *fn_l(int G, int nsl)
     switch (G) {
     case 16: return fn_lg<L, 16>(nsl);
     case 32: return fn_lg<L, 32>(nsl);
-    case 64: return fn_lg<L, 64>(nsl);
+    case 64: if constexpr (L < 128) return fn_lg<L, 64>(nsl); else return nullptr;
     default: return nullptr;
     }
 }
@@ -241,6 +241,7 @@ hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s)
     switch (L) {
     case 32: return launch_l<32>(G, bs_slabs(ns), a, s);
     case 64: return launch_l<64>(G, bs_slabs(ns), a, s);
+    case 128: return launch_l<128>(G, bs_slabs(ns), a, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -250,6 +251,7 @@ const void *kernel_bs(int L, int G, int ns)
     switch (L) {
     case 32: return fn_l<32>(G, bs_slabs(ns));
     case 64: return fn_l<64>(G, bs_slabs(ns));
+    case 128: return fn_l<128>(G, bs_slabs(ns));
     default: return nullptr;
     }
 }

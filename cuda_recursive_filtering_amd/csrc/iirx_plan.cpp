@@ -85,7 +85,8 @@ int validate(const iirx_config *cfg, int64_t n)
     }
     case IIRX_KERNEL_BLOCKSTATE: {
         const int L = cfg->n_blocks, G = cfg->block_size;
-        if (!(L == 32 || L == 64) || !(G == 16 || G == 32 || G == 64)) return IIRX_ERR_GEOMETRY;
+        if (!(L == 32 || L == 64 || L == 128) || !(G == 16 || G == 32 || G == 64) || (L == 128 && G == 64))
+            return IIRX_ERR_GEOMETRY;
         const int64_t chunk = (int64_t)G * L;
         if (n % chunk != 0 || n / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
         return IIRX_OK;

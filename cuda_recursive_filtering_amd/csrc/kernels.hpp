@@ -222,10 +222,14 @@ constexpr int bs_tidx(int mb)
 // chunks with its own input buffer of G L floats and 16 lookback slots)
 constexpr int bs_waves(int L, int G)
 {
+    if (L == 128) return G == 16 ? 3 : 2;
     return G == 64 ? (L == 32 ? 3 : 2) : G == 32 ? (L == 32 ? 5 : 3) : (L == 32 ? 6 : 4);
 }
+// L = 128: T alone takes 40 KB of LDS, so the per-wave input buffers (8 KB
+// per 16 tiles) cap the workgroup at 11 waves (G = 16) or 6 (G = 32)
 constexpr int bs_wg_waves(int L, int G)
 {
+    if (L == 128) return G == 16 ? 11 : 6;
     return G == 64 ? (L == 32 ? 12 : 8) : G == 32 ? (L == 32 ? 10 : 12) : (L == 32 ? 12 : 16);
 }
 // scale slots: 0 T, 1 Gam

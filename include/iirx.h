@@ -70,8 +70,9 @@ typedef enum {
                                    matrix cores (FP16 hi/lo operands for the
                                    Toeplitz and input products, FP32 for the
                                    tile-state scan); n_blocks = tile length L
-                                   (32 or 64), block_size = tiles per chunk G
-                                   (16, 32 or 64), chunk = G * L samples */
+                                   (32, 64 or 128), block_size = tiles per
+                                   chunk G (16, 32 or 64; 16 or 32 at L =
+                                   128), chunk = G * L samples */
 } iirx_kernel;
 
 /*

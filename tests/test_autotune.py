@@ -25,7 +25,7 @@ def _by_kernel(entries):
 def test_candidates_cover_every_algorithm():
     sos, _ = iirx.butterworth_sos(8, 0.1)
     k = _by_kernel(iirx.tune_candidates(sos, 1 << 22))
-    assert len(k["PH"]) == 5 and len(k["STCR"]) == 10 and len(k["DTCR"]) == 10 and len(k["BS"]) == 6
+    assert len(k["PH"]) == 5 and len(k["STCR"]) == 10 and len(k["DTCR"]) == 10 and len(k["BS"]) == 8
     assert "PLR" not in k  # the direct form is offered at orders <= 4 only
     assert all(e["status"] == 0 for es in k.values() for e in es)
     k2 = _by_kernel(iirx.tune_candidates(iirx.butterworth_sos(4, 0.1)[0], 1 << 22))

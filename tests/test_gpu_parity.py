@@ -285,9 +285,11 @@ def err_rms(y, ref):
     return float(np.max(np.abs(np.asarray(y, np.float64) - ref)) / np.sqrt(np.mean(ref ** 2)))
 
 
+BS_GEOMS = [(G, L) for L in (32, 64, 128) for G in (16, 32, 64) if G * L <= 4096]
+
+
 @pytest.mark.parametrize("order", ORDERS + [6, 12])  # 1, 2, 3 and 4 FP32 K slabs
-@pytest.mark.parametrize("G", [16, 32, 64])
-@pytest.mark.parametrize("L", [32, 64])
+@pytest.mark.parametrize("G,L", BS_GEOMS)
 @pytest.mark.parametrize("wn", [0.1, 0.02])
 def test_blockstate_parity(device, order, G, L, wn):
     print(f"case BS order={order} G={G} L={L} wn={wn}", flush=True)

@@ -119,6 +119,11 @@ def test_lookback_depth(wn, N, lo, hi):
     ("PH", 48, 64, 1 << 20, False),       # BLOCK_SIZE not 32/64
     ("PH", 64, 48, 1 << 20, False),       # N_BLOCKS not a power of two
     ("PH", 64, 64, (1 << 20) + 64, False),  # no tail path for cascaded kernels
+    ("BS", 16, 128, 1 << 20, True),       # block-state: G = 16 tiles of L = 128
+    ("BS", 32, 128, 1 << 20, True),
+    ("BS", 64, 128, 1 << 20, False),      # G = 64 at L = 128 does not fit the LDS
+    ("BS", 16, 256, 1 << 20, False),      # no L = 256
+    ("BS", 16, 128, (1 << 20) + 1024, False),  # not a multiple of the 2048-sample chunk
 ])
 def test_config_check(kernel, B, N, n, ok):
     st = iirx.config_check(kernel, 2, n, block_size=B, n_blocks=N)
@@ -177,8 +182,7 @@ def _bs_tables(sos, L, G=64):
 
 
 @pytest.mark.parametrize("order", [2, 8, 16])
-@pytest.mark.parametrize("L", [32, 64])
-@pytest.mark.parametrize("G", [16, 64])
+@pytest.mark.parametrize("L,G", [(32, 16), (32, 64), (64, 16), (64, 64), (128, 16), (128, 32)])
 def test_blockstate_tables_reproduce_the_cascade(order, L, G):
     """The block-state tables (FP64 state space of the cascade, FP16 hi/lo
     operands) define the same filter: running y_t = T x_t + O s_t,
