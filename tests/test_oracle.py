@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 from scipy import signal
 
+import cuda_recursive_filtering_amd as iirx
 import oracle_lib as O
 from workloads import reference_workload
 
@@ -163,6 +164,28 @@ def test_direct_form_part_b_verdicts():
         verdicts[order] = O.gate(y, ref)
     assert verdicts[2][0] and verdicts[4][0]
     assert not np.isfinite(verdicts[16][1])
+
+
+def test_plr_order16_float32_taps_are_unstable():
+    """Why PLR fails at high order whatever its carry precision: the direct
+    form's feedback taps, rounded to float32 as the reference casts
+    PLR_FILTER_A (plr_kernels.cuh:423-427), define an unstable filter from
+    order 12 on at Wn = 0.1 (a pole outside the unit circle), while the FP64
+    taps and the float32 SOS cascade are stable.  Pole radii from numpy's
+    companion-matrix roots."""
+    radius = {}
+    for order in (4, 8, 12, 16):
+        sos, _ = iirx.butterworth_sos(order, 0.1)
+        a = np.array([1.0])
+        for s in sos.astype(np.float64):
+            a = np.polymul(a, s[3:])
+        a32 = a.astype(np.float32).astype(np.float64)
+        radius[order] = (np.abs(np.roots(a)).max(), np.abs(np.roots(a32)).max(),
+                         max(np.abs(np.roots(s[3:].astype(np.float64))).max() for s in sos))
+    for order, (r64, r32, rsos) in radius.items():
+        assert r64 < 1.0 and rsos < 1.0, order
+        assert (r32 > 1.0) == (order >= 12), (order, r32)
+    assert radius[16][1] > 1.4
 
 
 @pytest.mark.parametrize("order", [4, 8, 16])
