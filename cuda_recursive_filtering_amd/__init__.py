@@ -217,6 +217,14 @@ class Plan:
 
     __call__ = execute
 
+    def device_tables(self):
+        """The coefficient tables this plan holds on the device, as float32."""
+        n = ctypes.c_int64()
+        check(lib().iirx_plan_read_tables(self._h, None, 0, ctypes.byref(n)), "plan_read_tables")
+        out = np.zeros(n.value, np.float32)
+        check(lib().iirx_plan_read_tables(self._h, out.ctypes.data, n.value, ctypes.byref(n)), "plan_read_tables")
+        return out
+
     def info(self):
         i = _lib.PlanInfo()
         check(lib().iirx_plan_info_get(self._h, ctypes.byref(i)), "plan_info")
@@ -330,6 +338,14 @@ class Batch:
 
     def check(self):
         check(lib().iirx_batch_check(self._h), "batch_check")
+
+    def device_tables(self):
+        """The per-channel tables on the device (channel after channel, 16-byte aligned)."""
+        n = ctypes.c_int64()
+        check(lib().iirx_batch_read_tables(self._h, None, 0, ctypes.byref(n)), "batch_read_tables")
+        out = np.zeros(n.value, np.float32)
+        check(lib().iirx_batch_read_tables(self._h, out.ctypes.data, n.value, ctypes.byref(n)), "batch_read_tables")
+        return out
 
     def info(self):
         i = _lib.BatchInfo()

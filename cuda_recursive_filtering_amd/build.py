@@ -28,9 +28,15 @@ COMMON = [
 ]
 
 
+# bs_setup.hip reproduces the host FP64 precompute bit for bit: no fused
+# multiply-add may be formed there (a later -ffp-contract wins)
+FILE_FLAGS = {"bs_setup.hip": ["-ffp-contract=off"]}
+
+
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + COMMON + ["-Rpass-analysis=kernel-resource-usage", "-c", src, "-o", obj]
+    extra = FILE_FLAGS.get(os.path.basename(src), [])
+    cmd = [HIPCC] + COMMON + extra + ["-Rpass-analysis=kernel-resource-usage", "-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)

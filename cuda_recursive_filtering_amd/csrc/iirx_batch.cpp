@@ -4,8 +4,9 @@
 //
 // No counterpart in the reference, which filters one channel per binary; the
 // per-channel PH tables are those of setup_kernel_coefficients
-// (ph_kernels.cuh:151-200), built by ph_tables; block-state channels take
-// bs_tables for tiles of 64 samples and n_blocks tiles per chunk.
+// (ph_kernels.cuh:151-200), built by ph_tables on the host; block-state
+// channels' tables (tiles of 64 samples, bs_tiles tiles per chunk) are
+// computed in FP64 on the GPU by one bs_setup launch over those channels.
 #include "../../include/iirx.h"
 #include "kernels.hpp"
 #include "precompute.hpp"
@@ -33,7 +34,7 @@ struct iirx_batch {
     int64_t n_channels, n_samples;
     uint32_t num_chunks, spin_limit;
     int coef_stride, device;
-    int64_t n_blockstate;
+    int64_t n_blockstate, coef_size;
     BatchGroup grp[2];  // [0] PH, [1] block-state
     float *d_coef;
     uint64_t *d_lb;
@@ -136,6 +137,7 @@ int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels, const
     std::vector<BatchChannel> chan((size_t)n_channels);
     std::vector<uint32_t> chunks[2];
     std::vector<char> is_bs((size_t)n_channels, 0);
+    std::vector<BsSetupJob> jobs;
     int stride = 0;
     int64_t sec0 = 0, n_bs = 0;
     for (int64_t c = 0; c < n_channels; ++c) {
@@ -145,7 +147,12 @@ int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels, const
         const bool bs = want == IIRX_BATCH_BLOCKSTATE ||
                         (want == IIRX_BATCH_AUTO && ns >= min_bs);
         if (bs) {
-            if (bs_tables(sos + 6 * sec0, ns, 64, G, t) <= 0) return IIRX_ERR_INVALID;
+            BsSetupJob j{};
+            std::copy(sos + 6 * sec0, sos + 6 * (sec0 + ns), j.sos);
+            j.ns = ns;
+            j.out_off = (int64_t)coef.size();
+            jobs.push_back(j);
+            t.assign((size_t)bs_layout(64).stride, 0.0f);  // filled on the device
             ++n_bs;
         } else {
             const int st = ph_tables(sos + 6 * sec0, ns, 64, N, t);
@@ -213,6 +220,23 @@ int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels, const
         hipMemset(b->grp[1].d_ctrl, 0, kControlBytes) != hipSuccess) {
         release(b);
         return IIRX_ERR_HIP;
+    }
+    b->coef_size = (int64_t)coef.size();
+    if (!jobs.empty()) {
+        BsSetupJob *d_jobs = nullptr;
+        if (hipMalloc((void **)&d_jobs, jobs.size() * sizeof(BsSetupJob)) != hipSuccess) {
+            release(b);
+            return IIRX_ERR_OOM;
+        }
+        const bool ok =
+            hipMemcpy(d_jobs, jobs.data(), jobs.size() * sizeof(BsSetupJob), hipMemcpyHostToDevice) == hipSuccess &&
+            launch_bs_setup(64, G, d_jobs, (int)jobs.size(), b->d_coef, nullptr) == hipSuccess &&
+            hipDeviceSynchronize() == hipSuccess;
+        hipFree(d_jobs);
+        if (!ok) {
+            release(b);
+            return IIRX_ERR_HIP;
+        }
     }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess) {
@@ -328,6 +352,17 @@ int iirx_batch_info_get(const iirx_batch *b, iirx_batch_info *info)
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, g.fn, 64, 0) == hipSuccess) ws = nb / 4;
     }
     (void)hipGetLastError();
+    return IIRX_OK;
+}
+
+int iirx_batch_read_tables(const iirx_batch *b, float *out, int64_t cap, int64_t *size)
+{
+    if (!b) return IIRX_ERR_INVALID;
+    if (size) *size = b->coef_size;
+    if (out && cap > 0 &&
+        hipMemcpy(out, b->d_coef, sizeof(float) * (size_t)std::min(cap, b->coef_size), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return IIRX_ERR_HIP;
     return IIRX_OK;
 }
 

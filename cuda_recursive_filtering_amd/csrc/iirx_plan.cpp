@@ -280,8 +280,22 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     p->d_epoch = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(p->d_ticket) + 8);
     p->d_err = p->d_epoch + 1;
     p->d_arrive = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(p->d_ticket) + 128);
-    if (hipMemcpy(p->d_coef, p->host_coef.data(), p->host_coef.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    if (cfg->kernel == IIRX_KERNEL_BLOCKSTATE) {
+        // the block-state tables are computed in FP64 on the GPU (bs_setup.hip)
+        BsSetupJob job{};
+        std::copy(p->sos.begin(), p->sos.end(), job.sos);
+        job.ns = cfg->n_sections;
+        BsSetupJob *d_job = nullptr;
+        if (hipMalloc((void **)&d_job, sizeof job) != hipSuccess) return fail(IIRX_ERR_OOM);
+        const bool ok = hipMemcpy(d_job, &job, sizeof job, hipMemcpyHostToDevice) == hipSuccess &&
+                        launch_bs_setup(cfg->n_blocks, cfg->block_size, d_job, 1, p->d_coef, nullptr) == hipSuccess &&
+                        hipDeviceSynchronize() == hipSuccess;
+        hipFree(d_job);
+        if (!ok) return fail(IIRX_ERR_HIP);
+    } else if (hipMemcpy(p->d_coef, p->host_coef.data(), p->host_coef.size() * sizeof(float),
+                         hipMemcpyHostToDevice) != hipSuccess) {
         return fail(IIRX_ERR_HIP);
+    }
     // zeroed once; launch-versioned flags never need a reset afterwards
     if (hipMemset(p->d_lb, 0, lb_count * sizeof(uint64_t)) != hipSuccess) return fail(IIRX_ERR_HIP);
     if (hipMemset(p->d_ticket, 0, kControlBytes) != hipSuccess) return fail(IIRX_ERR_HIP);
@@ -319,6 +333,17 @@ int iirx_plan_tables(const iirx_config *cfg, const float *sos, int64_t n_samples
     if (size) *size = (int64_t)t.size();
     if (stride) *stride = sh.coef_stride;
     if (out) std::memcpy(out, t.data(), sizeof(float) * (size_t)std::min<int64_t>(cap, (int64_t)t.size()));
+    return IIRX_OK;
+}
+
+int iirx_plan_read_tables(const iirx_plan *p, float *out, int64_t cap, int64_t *size)
+{
+    if (!p) return IIRX_ERR_INVALID;
+    const int64_t n = (int64_t)p->host_coef.size();
+    if (size) *size = n;
+    if (out && cap > 0 &&
+        hipMemcpy(out, p->d_coef, sizeof(float) * (size_t)std::min(cap, n), hipMemcpyDeviceToHost) != hipSuccess)
+        return IIRX_ERR_HIP;
     return IIRX_OK;
 }
 

@@ -281,6 +281,16 @@ hipError_t launch_batch(int algo, int n, const BatchArgs &a, hipStream_t s);
 const void *kernel_batch(int algo, int n);
 
 hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s);
+
+// one block-state table built on the GPU in FP64 (bs_setup.hip): the
+// sections of one cascade and where its table goes
+struct BsSetupJob {
+    float sos[48];  // up to 8 rows of 6
+    int32_t ns;
+    int32_t pad;
+    int64_t out_off;  // floats from the output base
+};
+hipError_t launch_bs_setup(int L, int G, const BsSetupJob *jobs, int n_jobs, float *out, hipStream_t s);
 const void *kernel_bs(int L, int G, int ns);
 
 }  // namespace iirx

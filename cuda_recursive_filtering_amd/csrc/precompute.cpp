@@ -278,9 +278,10 @@ void put_block32(float *dst, const double *m, int rows, int cols, int r0, int q,
         for (int j = 0; j < 8; ++j) {
             const int g = l / 16, r = r0 + (l % 16);
             const int c = 32 * q + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
+            // hi = v rounded once to FP16, lo = the exact remainder rounded once
             const double v = (r < rows && c < cols) ? std::ldexp(m[(size_t)r * cols + c], s) : 0.0;
-            const _Float16 hi = (_Float16)(float)v;
-            const _Float16 lo = (_Float16)(float)(v - (double)(float)hi);
+            const _Float16 hi = (_Float16)v;
+            const _Float16 lo = (_Float16)(v - (double)hi);
             h[l * 8 + j] = half_bits(hl == 0 ? hi : lo);
         }
 }

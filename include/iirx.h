@@ -227,11 +227,17 @@ int iirx_fill_normal(uint64_t seed, int64_t n, float *out);
  * Host copy of the coefficient tables a plan of this configuration would
  * upload (FP64 precompute rounded to float32).  Writes min(cap, size)
  * floats to out (out may be NULL to query), the total size to *size and the
- * per-section stride to *stride.  No device needed.
+ * per-section stride to *stride.  No device needed.  A block-state plan
+ * computes the same tables in FP64 on the GPU instead (bit-identical).
  */
 int iirx_plan_tables(const iirx_config *cfg, const float *sos,
                      int64_t n_samples, float *out, int64_t cap,
                      int64_t *size, int32_t *stride);
+/* The tables a plan holds on the device, copied back: min(cap, size) floats
+ * to out (out may be NULL to query the size).  No counterpart in the
+ * reference; for checking the device-built tables against iirx_plan_tables. */
+int iirx_plan_read_tables(const iirx_plan *p, float *out, int64_t cap,
+                          int64_t *size);
 
 /* ------------------------------------------------------------------------
  * Autotuner: replaces the reference's fixed per-order geometry sweep and
@@ -352,6 +358,11 @@ int iirx_batch_replay(iirx_batch *b, void *stream);
 /* Synchronise and report the lookback watchdog, as iirx_plan_check. */
 int iirx_batch_check(iirx_batch *b);
 int iirx_batch_info_get(const iirx_batch *b, iirx_batch_info *info);
+/* The batch's per-channel tables on the device, copied back (channel after
+ * channel, each 16-byte aligned): min(cap, size) floats to out (out may be
+ * NULL to query the size). */
+int iirx_batch_read_tables(const iirx_batch *b, float *out, int64_t cap,
+                           int64_t *size);
 int iirx_batch_destroy(iirx_batch *b);
 
 #ifdef __cplusplus

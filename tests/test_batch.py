@@ -111,6 +111,25 @@ def test_batch_edge_lengths(device, n_blocks, schedule, algo):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("bs_tiles", [16, 64])
+def test_batch_device_tables_equal_host(device, bs_tiles):
+    """Block-state channel tables are built in FP64 on the GPU, one workgroup
+    per channel; each must equal the host precompute for tiles of 64 samples
+    and bs_tiles tiles per chunk, bit for bit."""
+    print(f"case batch device tables G={bs_tiles}", flush=True)
+    offsets, sections, gains = ragged_channels(12, seed=9, hi_log2=14)
+    with iirx.Batch(offsets, sections, algo="blockstate", bs_tiles=bs_tiles) as b:
+        dev = b.device_tables()
+    off = 0
+    for c, sec in enumerate(sections):
+        host, _ = iirx.plan_tables("BS", sec, 64 * bs_tiles, block_size=bs_tiles, n_blocks=64)
+        got = dev[off:off + host.size]
+        assert np.array_equal(got.view(np.uint32), host.view(np.uint32)), c
+        off += host.size
+    assert off == dev.size
+
+
+@pytest.mark.gpu
 def test_batch_algorithms_agree(device):
     """The same channels run as PH and as the block-state form: both pass the
     FP64 gate, and the block-state RMS error stays at PH's level (an operand

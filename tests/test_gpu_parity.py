@@ -304,6 +304,25 @@ def test_blockstate_parity(device, order, G, L, wn):
     assert e_bs <= max(8.0 * e_ph, 1e-6), (e_bs, e_ph)
 
 
+@pytest.mark.parametrize("order", [2, 4, 8, 12, 16])
+@pytest.mark.parametrize("G,L", [(64, 64), (16, 32), (32, 128)])
+@pytest.mark.parametrize("wn", [0.1, 0.02, 0.4])
+def test_blockstate_device_tables_equal_host(device, order, G, L, wn):
+    """The block-state tables are computed in FP64 on the GPU with the host
+    precompute's operations in the same order and no contraction; they must
+    equal the host tables bit for bit (FP16 hi/lo halves, FP32 blocks,
+    scales, depth)."""
+    print(f"case BS device tables order={order} G={G} L={L} wn={wn}", flush=True)
+    sos, g = iirx.butterworth_sos(order, wn)
+    n = G * L * 4
+    host, _ = iirx.plan_tables("BS", sos, n, block_size=G, n_blocks=L)
+    with iirx.Plan("BS", sos, n, block_size=G, n_blocks=L) as p:
+        dev = p.device_tables()
+    assert dev.size == host.size
+    diff = np.flatnonzero(dev.view(np.uint32) != host.view(np.uint32))
+    assert diff.size == 0, (diff[:8], dev[diff[:8]], host[diff[:8]])
+
+
 @pytest.mark.parametrize("schedule", ["persistent", "ticket"])
 @pytest.mark.parametrize("n_chunks", [1, 3, 70, 3000])
 @pytest.mark.parametrize("G", [16, 64])
