@@ -20,9 +20,13 @@ namespace iirx {
 constexpr int kBsSlots = 16;  // lookback partials folded at most (LDS: 16 x 16 floats)
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f4 mfma_h(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ f4 mfma_f(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+// FP64 matrix core: lane l, register r of the result holds row l/16 + 4 r
+// (the FP32 shapes hold row 4 (l/16) + r)
+__device__ __forceinline__ d4 mfma_d(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
 // DPP moves (v_mov_dpp, no LDS round trip): row_ror:D rotates every 16-lane
 // row, lane l <- lane (l - D) mod 16 of its row; row_bcast15/31 pass lane 15 /
@@ -62,13 +66,32 @@ __device__ __forceinline__ void split8(const f4 &a, const f4 &b, int s, h8 &hi, 
     }
 }
 
-// acc + M S over the first NSL K slabs; M is this lane's float4 of an FP32 block
+// acc + M S over the first NSL K slabs; M is this lane's float4 of an FP32
+// block (slab m = states 4 m .. 4 m + 3)
 template <int NSL>
 __device__ __forceinline__ f4 apply(const f4 &M, const f4 &S, f4 acc)
 {
 #pragma unroll
     for (int m = 0; m < NSL; ++m) acc = mfma_f(M[m], S[m], acc);
     return acc;
+}
+
+// the same on the FP64 matrix core: FP32 operands, FP64 accumulation
+template <int NSL>
+__device__ __forceinline__ d4 apply_d(const f4 &M, const d4 &S, d4 acc)
+{
+#pragma unroll
+    for (int m = 0; m < NSL; ++m) acc = mfma_d((double)M[m], S[m], acc);
+    return acc;
+}
+
+template <int D>
+__device__ __forceinline__ double row_ror_d(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x120 + D, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x120 + D, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
 // S(c, k) = Z(c, k - D) (0 for k < D) over NB column blocks of 16 tiles;
@@ -93,6 +116,32 @@ __device__ __forceinline__ void shift_tiles(const f4 (&Z)[NB], f4 (&S)[NB])
                     S[nb][j] = wrap ? r1 : r0;
                 } else {
                     S[nb][j] = 0.0f;
+                }
+            }
+        }
+    }
+}
+
+// shift_tiles on FP64 state blocks
+template <int D, int NSL, int NB>
+__device__ __forceinline__ void shift_tiles_d(const d4 (&Z)[NB], d4 (&S)[NB])
+{
+    if constexpr (D >= 16) {
+        constexpr int dn = D >> 4;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) S[nb] = (nb >= dn) ? Z[nb >= dn ? nb - dn : 0] : d4{0.0, 0.0, 0.0, 0.0};
+    } else {
+        const bool wrap = (threadIdx.x & 15) < D;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j < NSL) {
+                    const double r0 = row_ror_d<D>(Z[nb][j]);
+                    const double r1 = (nb > 0) ? row_ror_d<D>(Z[nb > 0 ? nb - 1 : 0][j]) : 0.0;
+                    S[nb][j] = wrap ? r1 : r0;
+                } else {
+                    S[nb][j] = 0.0;
                 }
             }
         }
@@ -136,8 +185,10 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
             for (int q = 0; q < KQ; ++q) split8(xr.a[nb][q], xr.b[nb][q], eX, xh[nb][q], xl[nb][q]);
     }
 
-    // 2. V = Gam X (zero chunk-incoming state); Y0 = T X follows the partial
-    f4 Z[NB];
+    // 2. V = Gam X (zero chunk-incoming state); Gam's rows are ordered so that
+    //    lane group g, register j holds state g + 4 j, the FP64 MFMA's layout.
+    //    Y0 = T X follows the partial.
+    d4 Z[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -149,25 +200,26 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
             acc = mfma_h(gh, xh[nb][q], acc);
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) Z[nb][j] = ldexpf(acc[j], -(sG + eX));
+        for (int j = 0; j < 4; ++j) Z[nb][j] = ldexp((double)acc[j], -(sG + eX));
     }
-    // 3. tile end states, log2(G)-level scan (levels D >= 16 skip the
+    // 3. tile end states, log2(G)-level scan on the FP64 matrix core (FP32
+    //    operands Phi^(2^j), FP64 accumulation; levels D >= 16 skip the
     //    all-zero blocks)
     static_for<0, bs_levels<G>(), 1>([&](auto LV) {
         constexpr int D = 1 << decltype(LV)::value;
-        f4 S[NB];
-        shift_tiles<D, NSL, NB>(Z, S);
+        d4 S[NB];
+        shift_tiles_d<D, NSL, NB>(Z, S);
         const f4 P = ops.pow2(decltype(LV)::value);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-            if (D < 16 || nb >= (D >> 4)) Z[nb] = apply<NSL>(P, S[nb], Z[nb]);
+            if (D < 16 || nb >= (D >> 4)) Z[nb] = apply_d<NSL>(P, S[nb], Z[nb]);
     });
 
-    // 4. chunk partial = Z(:, G-1): lanes 15, 31, 47, 63 hold components 4 g + j
+    // 4. chunk partial = Z(:, G-1): lanes 15, 31, 47, 63 hold states g + 4 j
     uint64_t *rec = lb + (size_t)chunk * 32;
     if (col == 15) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) rec_store(rec + 4 * g + j, part_flag, Z[NB - 1][j]);
+        for (int j = 0; j < 4; ++j) rec_store(rec + g + 4 * j, part_flag, (float)Z[NB - 1][j]);
     }
     // the predecessor's record is in flight while Y0 = T X runs on the matrix cores
     const uint64_t poll0 = (chunk > first) ? lookback_first_poll<16>(lb, 32, chunk) : 0ull;
@@ -199,40 +251,46 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) Ob[mb] = ops.obs(mb);
     asm volatile("" ::: "memory");
-    // X_in in B layout on column 0 (component 4 g + m lives on lane 4 g + m)
+    // X_in in B layout on column 0 (state g + 4 m lives on lane g + 4 m)
     f4 xin;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) xin[j] = (j < NSL) ? __shfl(Xj, 4 * g + j, 64) : 0.0f;
+    for (int j = 0; j < 4; ++j) xin[j] = (j < NSL) ? __shfl(Xj, g + 4 * j, 64) : 0.0f;
 
     // injection W_k = Phi^(k+1) X_in: Phi X_in in column 0 of one block, a
-    // 4-level scan inside the block, then Phi^(16 nb) for blocks 1..NB-1
-    f4 U[1];
+    // 4-level scan inside the block, then Phi^(16 nb) for blocks 1..NB-1,
+    // all accumulated in FP64
+    d4 U[1];
     {
-        f4 b0;
+        d4 b0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) b0[j] = (col == 0) ? xin[j] : 0.0f;
-        U[0] = apply<NSL>(ops.phi(), b0, f4{0.f, 0.f, 0.f, 0.f});
+        for (int j = 0; j < 4; ++j) b0[j] = (col == 0) ? (double)xin[j] : 0.0;
+        U[0] = apply_d<NSL>(ops.phi(), b0, d4{0.0, 0.0, 0.0, 0.0});
     }
     static_for<0, 4, 1>([&](auto LV) {
         constexpr int D = 1 << decltype(LV)::value;
-        f4 S[1];
-        shift_tiles<D, NSL, 1>(U, S);
-        U[0] = apply<NSL>(ops.pow2(decltype(LV)::value), S[0], U[0]);
+        d4 S[1];
+        shift_tiles_d<D, NSL, 1>(U, S);
+        U[0] = apply_d<NSL>(ops.pow2(decltype(LV)::value), S[0], U[0]);
     });
 #pragma unroll
-    for (int nb = 1; nb < NB; ++nb) Z[nb] = apply<NSL>(ops.pow16(nb), U[0], Z[nb]);
+    for (int nb = 1; nb < NB; ++nb) Z[nb] = apply_d<NSL>(ops.pow16(nb), U[0], Z[nb]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) Z[0][j] += U[0][j];
 
-    // chunk full = Z(:, G-1)
+    // the states, rounded once to FP32; chunk full = Z(:, G-1)
+    f4 Zf[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Zf[nb][j] = (float)Z[nb][j];
     if (col == 15) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) rec_store(rec + 16 + 4 * g + j, full_flag, Z[NB - 1][j]);
+        for (int j = 0; j < 4; ++j) rec_store(rec + 16 + g + 4 * j, full_flag, Zf[NB - 1][j]);
     }
 
     // 5. S_k = state entering tile k (X_in for tile 0); Y = Y0 + O S
     f4 S[NB];
-    shift_tiles<1, NSL, NB>(Z, S);
+    shift_tiles<1, NSL, NB>(Zf, S);
     if (col == 0) S[0] = xin;
     // a store covers 64 B of each of 16 tiles; the two halves of a 128-byte
     // line (row blocks mb, mb + 1) are issued back to back and through the L2

@@ -4,8 +4,7 @@
 // No counterpart in the reference (its precompute is float32 host code,
 // ph_kernels.cuh:151-200).  One workgroup of 256 threads builds one table:
 // the cascade's state space (A, B, C) from the sections, the balancing by the
-// controllability Gramian, the state permutation onto 4 (s / nsl) + s % nsl,
-// the impulse response h, the observability rows C A^i, the controllability
+// controllability Gramian, the impulse response h, the observability rows C A^i, the controllability
 // columns A^i B, Phi = A^L with Phi^(2^j), Phi^(16 nb) and Phi^G, the
 // power-of-two operand scales, the FP16 hi/lo and FP32 operand blocks and the
 // lookback depth.  Every FP64 operation is the one precompute.cpp: bs_tables
@@ -69,7 +68,6 @@ __global__ __launch_bounds__(kSetupThreads) void bs_setup_kernel(const BsSetupJo
     __shared__ double A[256], W[256], Ak[256], T1[256], T2[256], Pm[6][256], Qm[4][256], Mc[256], Pw[256];
     __shared__ double Bv[16], Cv[16], dsc[16], h[L], powB[L][16], Om[L][16];
     __shared__ double red[kSetupThreads];
-    __shared__ int rho[16];
 
     for (int i = t; i < Y.stride; i += kSetupThreads) out[i] = 0.0f;
 
@@ -146,30 +144,7 @@ __global__ __launch_bounds__(kSetupThreads) void bs_setup_kernel(const BsSetupJo
         __syncthreads();
     }
 
-    // state s onto row 4 (s / nsl) + s % nsl
-    {
-        const int nsl = bs_slabs(ns);
-        if (t < 16) rho[t] = (t < p) ? 4 * (t / nsl) + t % nsl : -1;
-        T1[t] = 0.0;
-        if (t < 16) {
-            dsc[t] = 0.0;  // reused: B2
-            red[t] = 0.0;  // reused: C2
-        }
-        __syncthreads();
-        const int r = t >> 4, c = t & 15;
-        if (r < p && c < p) T1[rho[r] * 16 + rho[c]] = A[t];
-        if (t < p) {
-            dsc[rho[t]] = Bv[t];
-            red[rho[t]] = Cv[t];
-        }
-        __syncthreads();
-        A[t] = T1[t];
-        if (t < 16) {
-            Bv[t] = dsc[t];
-            Cv[t] = red[t];
-        }
-        __syncthreads();
-    }
+    // states stay in order 0..p-1 (K slab m of the state products holds states 4 m .. 4 m + 3)
 
     // impulse response, observability rows C A^i, controllability columns A^i B
     {
@@ -275,7 +250,7 @@ __global__ __launch_bounds__(kSetupThreads) void bs_setup_kernel(const BsSetupJo
             } else {
                 hl = (blk - nT) / Y.KQ;
                 q = (blk - nT) % Y.KQ;
-                r = l & 15;
+                r = bs_state_of_row(l & 15);  // row 4 g + j of the product = state g + 4 j
                 const int c = 32 * q + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
                 s = sG;
                 v = powB[L - 1 - c][r];  // Gam[r][c] = (A^(L-1-c) B)[r]
@@ -288,9 +263,9 @@ __global__ __launch_bounds__(kSetupThreads) void bs_setup_kernel(const BsSetupJo
             dst[l * 8 + j] = half_bits_d(hl == 0 ? hi : lo);
         }
     }
-    // FP32 lane-order blocks: word 4 l + k = M[r0 + l % 16][4 (l / 16) + k]
+    // FP32 lane-order blocks: word 4 l + k = M[r0 + l % 16][l / 16 + 4 k]
     {
-        const int l = t >> 2, k = t & 3, rr = l & 15, cc = 4 * (l >> 4) + k;
+        const int l = t >> 2, k = t & 3, rr = l & 15, cc = (l >> 4) + 4 * k;
         if (t < 256) {
             for (int j = 0; j < 6; ++j) out[bs_P(Y, j) + t] = (float)Pm[j][rr * 16 + cc];
             for (int nb = 1; nb < 4; ++nb) out[bs_Q(Y, nb) + t] = (float)Qm[nb][rr * 16 + cc];

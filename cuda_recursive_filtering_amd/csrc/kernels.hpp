@@ -208,6 +208,11 @@ constexpr int bs_P(const BsLayout &y, int j) { return y.kP + j * kBsBlockWords; 
 constexpr int bs_Q(const BsLayout &y, int nb) { return y.kQ + (nb - 1) * kBsBlockWords; }
 constexpr int bs_O(const BsLayout &y, int mb) { return y.kO + mb * kBsBlockWords; }
 constexpr int bs_slabs(int ns) { return (ns + 1) / 2; }
+// state order of the block-state products: lane group g, register j of a
+// 16 x 16 x 4 FP64 MFMA result holds state g + 4 j (row g + 4 j).  The FP16
+// Gam product puts its row 4 g + j on that lane and register, so Gam's
+// operand row i carries state bs_state_of_row(i).
+constexpr int bs_state_of_row(int i) { return (i >> 2) + 4 * (i & 3); }
 // last nonzero K slab of T's row block mb, and the compact index of (mb, 0)
 // among T's nonzero blocks (blocks above the diagonal are zero)
 constexpr int bs_tq(int mb) { return (16 * mb + 15) / 32; }
@@ -223,14 +228,14 @@ constexpr int bs_tidx(int mb)
 constexpr int bs_waves(int L, int G)
 {
     if (L == 128) return G == 16 ? 3 : 2;
-    return G == 64 ? (L == 32 ? 3 : 2) : G == 32 ? (L == 32 ? 5 : 3) : (L == 32 ? 6 : 4);
+    return G == 64 ? (L == 32 ? 3 : 2) : G == 32 ? (L == 32 ? 4 : 3) : (L == 32 ? 5 : 4);
 }
 // L = 128: T alone takes 40 KB of LDS, so the per-wave input buffers (8 KB
 // per 16 tiles) cap the workgroup at 11 waves (G = 16) or 6 (G = 32)
 constexpr int bs_wg_waves(int L, int G)
 {
     if (L == 128) return G == 16 ? 11 : 6;
-    return G == 64 ? (L == 32 ? 12 : 8) : G == 32 ? (L == 32 ? 10 : 12) : (L == 32 ? 12 : 16);
+    return G == 64 ? (L == 32 ? 12 : 8) : G == 32 ? (L == 32 ? 8 : 12) : (L == 32 ? 10 : 16);
 }
 // scale slots: 0 T, 1 Gam
 

@@ -270,13 +270,15 @@ uint16_t half_bits(_Float16 h)
 
 // FP16 A operand of v_mfma_f32_16x16x32_f16 (kernels.hpp: bs_layout): rows
 // r0..r0+15 and K slab q of matrix m (rows x cols, row major, zero outside),
-// scaled by 2^s, hi (hl = 0) or lo (1) part
-void put_block32(float *dst, const double *m, int rows, int cols, int r0, int q, int s, int hl)
+// scaled by 2^s, hi (hl = 0) or lo (1) part.  With perm, operand row i holds
+// matrix row r0 + bs_state_of_row(i): the product's row 4 g + j then lands on
+// lane group g, register j as state g + 4 j, the FP64 MFMA's order.
+void put_block32(float *dst, const double *m, int rows, int cols, int r0, int q, int s, int hl, bool perm = false)
 {
     uint16_t *h = reinterpret_cast<uint16_t *>(dst);
     for (int l = 0; l < 64; ++l)
         for (int j = 0; j < 8; ++j) {
-            const int g = l / 16, r = r0 + (l % 16);
+            const int g = l / 16, r = r0 + (perm ? bs_state_of_row(l % 16) : l % 16);
             const int c = 32 * q + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
             // hi = v rounded once to FP16, lo = the exact remainder rounded once
             const double v = (r < rows && c < cols) ? std::ldexp(m[(size_t)r * cols + c], s) : 0.0;
@@ -286,12 +288,13 @@ void put_block32(float *dst, const double *m, int rows, int cols, int r0, int q,
         }
 }
 
-// FP32 A operand of v_mfma_f32_16x16x4f32: word 4 l + m = M[l%16][4 (l/16) + m]
-// of the 16 x 16 block at row r0 of m (rows x 16)
+// FP32 A operand of a 16 x 16 x 4 MFMA (f32 or f64) whose K slab m holds
+// states g + 4 m: word 4 l + m = M[r0 + l%16][l/16 + 4 m] of the 16 x 16
+// block at row r0 of m (rows x 16)
 void put_f32(float *dst, const double *m, int r0)
 {
     for (int l = 0; l < 64; ++l)
-        for (int k = 0; k < 4; ++k) dst[4 * l + k] = (float)m[(size_t)(r0 + l % 16) * 16 + 4 * (l / 16) + k];
+        for (int k = 0; k < 4; ++k) dst[4 * l + k] = (float)m[(size_t)(r0 + l % 16) * 16 + (l / 16) + 4 * k];
 }
 
 }  // namespace
@@ -356,22 +359,8 @@ int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out)
             C[c] *= dsc[c];
         }
     }
-    // state s onto row 4 (s / nsl) + s % nsl (only nsl FP32 K slabs are nonzero)
-    {
-        const int nsl = bs_slabs(ns);
-        int rho[16];
-        for (int st = 0; st < 16; ++st) rho[st] = (st < p) ? 4 * (st / nsl) + st % nsl : -1;
-        Mat A2(256, 0.0);
-        std::vector<double> B2(16, 0.0), C2(16, 0.0);
-        for (int r = 0; r < p; ++r) {
-            B2[rho[r]] = B[r];
-            C2[rho[r]] = C[r];
-            for (int c = 0; c < p; ++c) A2[rho[r] * 16 + rho[c]] = A[r * 16 + c];
-        }
-        A.swap(A2);
-        B.swap(B2);
-        C.swap(C2);
-    }
+    // states stay in order 0..p-1: K slab m of the state products holds states
+    // 4 m .. 4 m + 3, so only ceil(p / 4) slabs are nonzero
     // impulse response, observability, controllability
     std::vector<double> h((size_t)L), Om((size_t)L * 16), Gm((size_t)16 * L), Tm((size_t)L * L, 0.0);
     {
@@ -420,7 +409,7 @@ int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out)
     for (int hl = 0; hl < 2; ++hl) {
         for (int mb = 0; mb < Y.MB; ++mb)
             for (int q = 0; q < Y.KQ; ++q) put_block32(out.data() + bs_T(Y, hl, mb, q), Tm.data(), L, L, 16 * mb, q, sT, hl);
-        for (int q = 0; q < Y.KQ; ++q) put_block32(out.data() + bs_G(Y, hl, q), Gm.data(), 16, L, 0, q, sG, hl);
+        for (int q = 0; q < Y.KQ; ++q) put_block32(out.data() + bs_G(Y, hl, q), Gm.data(), 16, L, 0, q, sG, hl, true);
     }
     for (int j = 0; j < 6; ++j) put_f32(out.data() + bs_P(Y, j), P[j].data(), 0);
     for (int nb = 1; nb < 4; ++nb) put_f32(out.data() + bs_Q(Y, nb), Q[nb].data(), 0);

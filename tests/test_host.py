@@ -156,11 +156,13 @@ def _bs_tables(sos, L, G=64):
         return m
 
     def blockf(word_off):
-        """16 x 16 FP32 A operand: word 4 l + m = M[l%16][4(l/16)+m]."""
+        """16 x 16 FP32 A operand, K slab m = states 4m..4m+3: word 4 l + m =
+        M[l%16][l/16 + 4m]."""
         w = tab[word_off: word_off + 256].astype(np.float64).reshape(64, 4)
         m = np.zeros((16, 16))
         for l in range(64):
-            m[l % 16, 4 * (l // 16): 4 * (l // 16) + 4] = w[l]
+            for k in range(4):
+                m[l % 16, l // 16 + 4 * k] = w[l, k]
         return m
 
     T = np.zeros((L, L))
@@ -169,10 +171,12 @@ def _bs_tables(sos, L, G=64):
             off = kT + (mb * KQ + q) * 256
             T[16 * mb:16 * mb + 16, 32 * q:32 * q + 32] = block32(off) + block32(off + MB * KQ * 256)
     T *= 2.0 ** (-tab[kS])
+    # Gam's operand row i carries state (i >> 2) + 4 (i & 3) (kernels.hpp: bs_state_of_row)
     Gm = np.zeros((16, L))
+    rows = [(i >> 2) + 4 * (i & 3) for i in range(16)]
     for q in range(KQ):
         off = kG + q * 256
-        Gm[:, 32 * q:32 * q + 32] = block32(off) + block32(off + KQ * 256)
+        Gm[rows, 32 * q:32 * q + 32] = block32(off) + block32(off + KQ * 256)
     Gm *= 2.0 ** (-tab[kS + 1])
     Om = np.vstack([blockf(kO + mb * 256) for mb in range(MB)])
     P0 = blockf(kP)
@@ -192,9 +196,8 @@ def test_blockstate_tables_reproduce_the_cascade(order, L, G):
     sos, g = iirx.butterworth_sos(order, 0.1)
     T, Om, Gm, P0, Phi, Mc, depth = _bs_tables(sos, L, G)
     assert np.array_equal(P0, Phi)
-    # state s sits on row 4 (s // nsl) + s % nsl: rows with r % 4 >= nsl are empty
-    nsl = (order // 2 + 1) // 2
-    dead = [r for r in range(16) if r % 4 >= nsl]
+    # states 0..order-1 in order: the rows past them are empty
+    dead = list(range(order, 16))
     assert not Phi[dead, :].any() and not Phi[:, dead].any()
     assert not Gm[dead, :].any() and not Om[:, dead].any()
     assert np.allclose(np.linalg.matrix_power(Phi, G), Mc, rtol=1e-4, atol=1e-30 + 1e-6 * np.abs(Mc).max())
