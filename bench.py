@@ -137,6 +137,18 @@ def cpu_baseline(iirx, args):
                       f"{reps} passes in {el:.1f} s, single thread"}
 
 
+def ragged_traffic(args):
+    """HBM bytes per replay of the default ragged configuration, from the
+    committed PMC summary (profiles/traffic.json), or None for other settings."""
+    if args.batch_n != 32 or args.batch_bs_min_sections not in (0, 8) or args.batch_bs_tiles != 32:
+        return None
+    try:
+        with open(os.path.join(HERE, "profiles", "traffic.json")) as f:
+            return json.load(f).get("ragged4096_default")
+    except (OSError, ValueError):
+        return None
+
+
 def bench_ragged(args):
     """The ragged batch: 4096 channels, lengths log-uniform in [2^10, 2^20],
     order uniform over {2, 4, 8, 16}, Butterworth Wn uniform in [0.02, 0.45]
@@ -196,7 +208,7 @@ def bench_ragged(args):
                    "chunks": info["num_chunks"], "grid": info["grid"], "schedule": args.schedule},
         "eager_ms_per_step": round(ms_eager, 4),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": ragged_traffic(args),
                      "algorithmic_bytes": BYTES_PER_SAMPLE * n},
     }
     if args.check:
