@@ -14,17 +14,19 @@
 //      them; blocks of T above the diagonal are skipped;
 //   2. the tile end states Z_k = sum_{j<=k} Phi^(k-j) V_j by a log2(G)-level
 //      Kogge-Stone scan over the G tiles, each level Phi^(2^j) * shift(Z)
-//      on v_mfma_f32_16x16x4f32 (plain FP32 operands: no splitting, no data
-//      scaling), over ceil(p/4) K slabs only;
+//      on v_mfma_f64_16x16x4f64 (FP32 operands Phi^(2^j), FP64 states and
+//      accumulation), over ceil(p/4) K slabs only;
 //   3. the chunk's incoming state X_in from the lookback (16-word records,
-//      chunk transition Phi^G, bounded depth), injected as Phi^(k+1) X_in:
-//      Phi X_in in tile 0, a 4-level scan inside one 16-tile block and
+//      chunk transition Phi^G, bounded depth), injected as Phi^(k+1) X_in in
+//      FP64: Phi X_in in tile 0, a 4-level scan inside one 16-tile block and
 //      Phi^(16 nb) products for the other blocks;
-//   4. Y = Y0 + O S, S_k = state entering tile k, accumulated onto Y0 by the
-//      FP32 matrix core.
-// Both MFMA shapes put D[4 (l/16) + j][l%16] on lane l, register j: tiles are
-// MFMA columns, states rows 4 g + j, so every product feeds the next from
-// registers and the scan's tile shifts are DPP row rotations.
+//   4. Y = Y0 + O S, S_k = state entering tile k rounded to FP32,
+//      accumulated onto Y0 by the FP32 matrix core.
+// MFMA columns are tiles, so every product feeds the next from registers and
+// the scan's tile shifts are DPP row rotations.  The FP64 shape puts row
+// l/16 + 4 j of its result on lane l, register j: states are kept in that
+// order (lane group g, register j = state g + 4 j) by the operand packing,
+// including Gam's row order in the FP16 product.
 #include "bs_core.hpp"
 
 namespace iirx {

@@ -164,14 +164,15 @@ namespace iirx {
 //   y_tile = T x_tile + O s,   s_next = Phi s + Gam x_tile
 // T: L x L lower-triangular Toeplitz of the impulse response, O: L x 16
 // (rows C A^i), Gam: 16 x L (columns A^(L-1-j) B), Phi = A^L.  The p = 2 NS
-// states sit on rows 4 (s / nsl) + s % nsl of 16, nsl = ceil(p / 4), so that
-// the FP32 products need only nsl of the four K = 4 slabs.
+// states keep their order 0..p-1; the state products' K slab m holds states
+// 4 m .. 4 m + 3, so they need only nsl = ceil(p / 4) of the four slabs.
 // FP16 blocks (T, Gam; hi/lo of the matrix scaled by 2^s) are the A operand
 // of v_mfma_f32_16x16x32_f16 in lane order: block word 4 lane + {0..3} = the
 // 8 halves A[l%16][8 (l/16) + j] of K slab q (samples 32 q ..); K row
 // 8 g + j is sample 32 q + 4 g + j (j < 4) or 32 q + 16 + 4 g + j - 4 (j >= 4).
-// FP32 blocks are the A operands of v_mfma_f32_16x16x4f32: word 4 lane + m =
-// M[l%16][4 (l/16) + m], slab m.
+// Gam's operand row i carries state bs_state_of_row(i).
+// FP32 blocks are the A operands of the 16 x 16 x 4 state products (FP64 or
+// FP32 MFMA): word 4 lane + m = M[l%16][l/16 + 4 m], slab m.
 //   [kT]  T hi/lo x MB row blocks x KQ slabs   [kG] Gam hi/lo x KQ slabs
 //   [kP]  Phi^(2^j), j < 6      [kQ] Phi^(16 nb), nb = 1..3     [kPhi] Phi
 //   [kO]  O row blocks          [kScale] exponents of T and Gam
