@@ -111,6 +111,35 @@ def test_batch_edge_lengths(device, n_blocks, schedule, algo):
 
 
 @pytest.mark.gpu
+def test_batch_many_tiny_channels(device):
+    """20000 channels of 0-300 samples (mostly one partial chunk each, some
+    empty), orders 2-16 with order 16 on the block-state form: every channel
+    starts from zero state and no sample of a neighbour leaks in."""
+    import torch
+    print("case batch 20000 tiny channels", flush=True)
+    rng = np.random.default_rng(12)
+    n_ch = 20000
+    lens = rng.integers(0, 301, n_ch)
+    offsets = np.zeros(n_ch + 1, np.int64)
+    offsets[1:] = np.cumsum(lens)
+    designs = {o: iirx.butterworth_sos(o, 0.1) for o in (2, 4, 8, 16)}
+    orders = rng.choice([2, 4, 8, 16], n_ch)
+    sections = [designs[o][0] for o in orders]
+    gains = np.array([designs[o][1] for o in orders])
+    x = ragged_input(offsets, gains, seed=1200)
+    xin = torch.from_numpy(x).to(device)
+    out = torch.full_like(xin, float("nan"))
+    with iirx.Batch(offsets, sections) as b:
+        b.execute(xin, out)
+        b.check()
+        info = b.info()
+    assert info["n_blockstate"] == int((orders == 16).sum()) and info["n_launches"] == 2
+    y = out.cpu().numpy()
+    assert np.isfinite(y).all()
+    _check_channels(y, x, offsets, sections, chans=range(0, n_ch, 37))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("bs_tiles", [16, 64])
 def test_batch_device_tables_equal_host(device, bs_tiles):
     """Block-state channel tables are built in FP64 on the GPU, one workgroup
