@@ -111,11 +111,11 @@ int validate(const iirx_config *cfg, int64_t n)
 // clamped to [1, X_MAX] and rounded down to an instantiated power of two.
 // PLR geometry, the reference's rule (plr_kernels.cuh:49-112: a per-order
 // PLR_BLOCK_SIZE table, PLR_X = clamp(N / (nextpow2(#SM) * BLOCK), 1,
-// PLR_X_MAX)) with an MI355X table: one wave per chunk (no barriers in the
-// segment chain) and X_MAX = clamp(16 / K, 2, 8), measured at 2^25 samples
-// (64 x 8, 64 x 4, 64 x 2, 64 x 2 at orders 2, 4, 8, 16: 226 / 157 / 74 / 19
-// GS/s, against 120 / 48 / 28 / 11 with 256-thread blocks and X = 16); the
-// spill calibration below may still halve X.
+// PLR_X_MAX)) with an MI355X table: one wave per chunk (no barriers; the
+// merge and the segment chain stay in registers) and X_MAX = 16 at order 2, 8
+// above, measured at 2^25 samples (64 x 16, 64 x 8, 64 x 8, 64 x 8 at orders
+// 2, 4, 8, 16: 245 / 176 / 94 / 22 GS/s, against 120 / 48 / 28 / 11 with
+// 256-thread blocks and X = 16); the spill calibration below may still halve X.
 void plr_geometry(const iirx_config *cfg, int64_t n, int *block, int *x)
 {
     const int k = 2 * cfg->n_sections;
@@ -123,7 +123,7 @@ void plr_geometry(const iirx_config *cfg, int64_t n, int *block, int *x)
     if (B == 0) B = 64;
     int X = cfg->plr_x;
     if (X == 0) {
-        const int cap = std::min(8, std::max(2, 16 / k));
+        const int cap = (k == 2) ? 16 : 8;
         const int xmax = (cfg->plr_x_max > 0) ? std::min(cfg->plr_x_max, 16) : cap;
         int64_t raw = n / (256LL * B);
         raw = std::max<int64_t>(1, std::min<int64_t>(raw, xmax));

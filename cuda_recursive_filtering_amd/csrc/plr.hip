@@ -24,6 +24,27 @@
 
 namespace iirx {
 
+// one-wave chunks (B = 64): value of lane (lane & ~(2H-1)) + H - K + M, the
+// merge level H source of the upper half of every 2H-lane group, without LDS
+// traffic: DPP for H = 1 (row_shr:1) and H = 2 (quad_perm), ds_swizzle bit
+// mode ((lane & and) | or within 32 lanes) for H = 4..16, readlane for H = 32
+template <int H, int K, int M>
+__device__ __forceinline__ float plr_bcast(float v)
+{
+    const int u = __float_as_int(v);
+    if constexpr (H == 1) {
+        return __int_as_float(__builtin_amdgcn_update_dpp(u, u, 0x111, 0xF, 0xF, false));
+    } else if constexpr (H == 2) {
+        constexpr int q = M - K + 2;  // quad lane 0 or 1
+        return __int_as_float(__builtin_amdgcn_update_dpp(u, u, q * 0x55, 0xF, 0xF, false));
+    } else if constexpr (H < 32) {
+        constexpr int and_mask = (~(2 * H - 1)) & 0x1F, or_mask = (H - K + M) & 0x1F;
+        return __int_as_float(__builtin_amdgcn_ds_swizzle(u, and_mask | (or_mask << 5)));
+    } else {
+        return __int_as_float(__builtin_amdgcn_readlane(u, 32 - K + M));
+    }
+}
+
 template <int K, int B, int X>
 __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
 {
@@ -65,6 +86,22 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
     }
 
     // 2a. merge levels inside the wave
+    if constexpr (B == 64) {
+        static_for<0, 6, 1>([&](auto LV) {
+            constexpr int H = 1 << decltype(LV)::value;
+            const bool up = (lane & H) != 0;
+            const int i = lane & (H - 1);
+            static_for<(K > H ? K - H : 0), K, 1>([&](auto MI) {
+                constexpr int M = decltype(MI)::value;
+                const float f = fac[M * CH + i];
+#pragma unroll
+                for (int v = 0; v < X; ++v) {
+                    const float sv_ = plr_bcast<H, K, M>(val[v]);
+                    if (up) val[v] = fmaf(f, sv_, val[v]);
+                }
+            });
+        });
+    } else {
 #pragma unroll
     for (int h = 1; h < 64 && h < B; h <<= 1) {
         const bool up = (lane & h) != 0;
@@ -80,6 +117,7 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
                 if (up) val[v] = fmaf(f, s, val[v]);
             }
         }
+    }
     }
 
     // 2b. merge levels across waves (h >= 64 >= K: all K terms)
@@ -103,31 +141,50 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
         }
     }
 
-    // 3. serial chain over the segments
+    // 3. serial chain over the segments (one wave: the last K values of the
+    //    previous segment by readlane)
 #pragma unroll
     for (int v = 1; v < X; ++v) {
-        __syncthreads();
-        if (t >= B - K) sv[t - (B - K)] = val[v - 1];
-        __syncthreads();
+        if constexpr (B == 64) {
 #pragma unroll
-        for (int m = 0; m < K; ++m) val[v] = fmaf(fac[m * CH + t], sv[m], val[v]);
+            for (int m = 0; m < K; ++m) {
+                const float pm = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(val[v - 1]), B - K + m));
+                val[v] = fmaf(fac[m * CH + t], pm, val[v]);
+            }
+        } else {
+            __syncthreads();
+            if (t >= B - K) sv[t - (B - K)] = val[v - 1];
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < K; ++m) val[v] = fmaf(fac[m * CH + t], sv[m], val[v]);
+        }
     }
 
     // 4. chunk partial = last K values; lookback by wave 0
     uint64_t *rec = a.lb + (size_t)chunk * (2 * K);
     if (t >= B - K) rec_store(rec + (t - (B - K)), part_flag, val[X - 1]);
-    __syncthreads();  // everyone is done with sv
-    if (w == 0) {
+    float Xv[K];  // the chunk carry, uniform
+    if constexpr (B == 64) {
         const float Xj = lookbackK<K>(a.lb, 2 * K, chunk, part_flag, full_flag, tab + Y.kF, (int)ctab[Y.kDepth], lp,
                                       a.spin_limit, a.err);
-        if (lane < K) sv[lane] = Xj;
+#pragma unroll
+        for (int m = 0; m < K; ++m) Xv[m] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Xj), m));
+    } else {
+        __syncthreads();  // everyone is done with sv
+        if (w == 0) {
+            const float Xj = lookbackK<K>(a.lb, 2 * K, chunk, part_flag, full_flag, tab + Y.kF, (int)ctab[Y.kDepth],
+                                          lp, a.spin_limit, a.err);
+            if (lane < K) sv[lane] = Xj;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < K; ++m) Xv[m] = sv[m];
     }
-    __syncthreads();
 
     // carry injection (plr:891-899)
 #pragma unroll
     for (int m = 0; m < K; ++m) {
-        const float Xm = sv[m];
+        const float Xm = Xv[m];
 #pragma unroll
         for (int v = 0; v < X; ++v) val[v] = fmaf(fac[m * CH + t + v * B], Xm, val[v]);
     }
