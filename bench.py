@@ -195,6 +195,44 @@ def cpu_baseline(iirx, args):
                       f"{reps} passes in {el:.1f} s, single thread"}
 
 
+def cpu_baselines_more(iirx, args):
+    """The other CPU references the workload names, each timed single-threaded
+    on the host over a bounded sample of the order-2 (and order-16) input:
+    scipy.signal.sosfilt in float64 (the golden) and the oracle's float32
+    restatements of the reference's STCR, DTCR and PLR emulators
+    (tools/stcr_emulator.py, dtcr_emulator.py, plr_emulator.py) at their
+    published geometries."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import numpy as np
+    from scipy import signal
+    import oracle_lib as O
+    n = 1 << 22
+    budget = max(1.0, args.cpu_seconds / 4)
+    out = []
+    for order in (2, 16):
+        sos, g = iirx.butterworth_sos(order, args.wn)
+        x = (g * iirx.fill_normal(1, n).astype(np.float64)).astype(np.float32)
+        bt, at = (np.asarray(v, np.float64) for v in iirx.direct_form_taps(sos))
+        cases = [("scipy.signal.sosfilt float64", "library",
+                  lambda: signal.sosfilt(sos.astype(np.float64), x.astype(np.float64))),
+                 ("oracle STCR float32 restatement 32x64", "port", lambda: O.stcr(sos, x, 32, 64)),
+                 ("oracle DTCR float32 restatement 32x64", "port", lambda: O.dtcr(sos, x, 32, 64))]
+        if order == 2:
+            cases.append(("oracle PLR float32 restatement 256x16", "port", lambda: O.plr(bt, at, x, 256, 16)))
+        for name, kind, fn in cases:
+            reps, t0 = 0, time.perf_counter()
+            while True:
+                fn()
+                reps += 1
+                el = time.perf_counter() - t0
+                if el >= budget / len(cases):
+                    break
+            out.append({"value": round(reps * n / el / 1e9, 4), "unit": "GSamples/s", "cores": 1, "kind": kind,
+                        "sample": f"{name}, order {order}, Wn={args.wn}, 2^22 N(0,1) samples, {reps} passes in "
+                                  f"{el:.1f} s, single thread"})
+    return out
+
+
 def ragged_traffic(args):
     """HBM bytes per replay of the default ragged configuration, from the
     committed PMC summary (profiles/traffic.json), or None for other settings."""
@@ -424,6 +462,7 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(iirx, args)
+        line["cpu_baselines_more"] = cpu_baselines_more(iirx, args)
     print(json.dumps(line), flush=True)
 
 
