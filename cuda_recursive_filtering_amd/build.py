@@ -61,6 +61,15 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     with open(os.path.join(OBJ, "resource_usage.txt"), "w") as f:
         f.write("\n".join(reports))
+    # the C++ driver of the reference's test_accuracy / test_performance, linked to the library
+    drv = os.path.join(HERE, "iirx_driver")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-Wno-unused-result", "-I", os.path.join(os.path.dirname(HERE), "include"),
+           os.path.join(HERE, "harness", "iirx_driver.cpp"), "-L", HERE, "-liirx", "-Wl,-rpath,$ORIGIN",
+           "-o", drv + ".tmp"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"driver build failed\n{' '.join(cmd)}\n{p.stdout}\n{p.stderr}")
+    os.replace(drv + ".tmp", drv)
     if verbose:
         print(f"built {LIB}")
     return LIB
