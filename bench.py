@@ -396,7 +396,9 @@ def main():
         gs = N_SAMPLES / (r["ms"] * 1e-3) / 1e9
         gbs = BYTES_PER_SAMPLE * N_SAMPLES / (r["ms"] * 1e-3) / 1e9
         entry = {"gsamples_per_s": round(gs, 3), "ms": round(r["ms"], 5), "achieved_gbs": round(gbs, 1),
-                 "roof_frac": round(gbs / HBM_PEAK_GBS, 4), "vgprs": r["info"]["vgprs"],
+                 "roof_frac": round(gbs / HBM_PEAK_GBS, 4),
+                 # textbook cascade: 4 FMA = 8 flop per sample and section (reported, not the roof)
+                 "algorithmic_gflops": round(gs * 4 * order, 1), "vgprs": r["info"]["vgprs"],
                  "scratch": r["info"]["scratch_bytes"], "chunks": r["info"]["num_chunks"],
                  "grid": r["info"]["grid"]}
         if args.check:
@@ -460,6 +462,7 @@ def main():
         "copy_ceiling": ceiling,
         "mfma_vs_fp32": mfma or None,
     }
+    line["host_cores"] = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(iirx, args)
         line["cpu_baselines_more"] = cpu_baselines_more(iirx, args)
