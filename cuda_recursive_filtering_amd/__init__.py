@@ -217,6 +217,23 @@ class Plan:
 
     __call__ = execute
 
+    def execute_state(self, d_in, d_out, zi=None, zf=None, stream=None):
+        """PH plans: one launch with the filter state carried in and out (a long
+        signal filtered in calls of n_samples).  zi / zf: device float32
+        tensors (or pointers) of 2 + 2 * n_sections values: x[-2], x[-1] of the
+        input, then y_s[-2], y_s[-1] of every section's output; None = zero
+        state in / no state out."""
+        if self._h is None:
+            raise RuntimeError("plan is closed")
+        for t in (d_in, d_out):
+            if hasattr(t, "numel") and t.numel() != self.n_samples:
+                raise ValueError(f"tensor has {t.numel()} samples, plan expects {self.n_samples}")
+        for t in (zi, zf):
+            if t is not None and hasattr(t, "numel") and t.numel() < 2 + 2 * self.n_sections:
+                raise ValueError(f"state needs {2 + 2 * self.n_sections} values")
+        check(lib().iirx_execute_state(self._h, _ptr(d_in), _ptr(d_out), None if zi is None else _ptr(zi),
+                                       None if zf is None else _ptr(zf), _stream_ptr(stream)), "execute_state")
+
     def device_tables(self):
         """The coefficient tables this plan holds on the device, as float32."""
         n = ctypes.c_int64()

@@ -212,7 +212,8 @@ __device__ __forceinline__ void publish2(uint64_t *r, bool full, uint32_t flag, 
 }
 
 // Decoupled lookback over a 64-chunk window, one predecessor per lane.
-// Returns the end state of chunk `chunk - 1` (zero state for chunk 0).
+// Returns the end state of chunk `chunk - 1`; the chain starts at chunk
+// `first` from the state (s2, s1) (zero unless the caller streams a state in).
 // M is the chunk transition: X_c = M X_{c-1} + partial_c (row-major 2x2).
 //
 // `depth` bounds the fold: the plan sets it to the smallest d with
@@ -231,7 +232,8 @@ __device__ __forceinline__ void publish2(uint64_t *r, bool full, uint32_t flag, 
 __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int sec, int ns,
                                             uint32_t part_flag, uint32_t full_flag,
                                             float m0, float m1, float m2, float m3, int depth,
-                                            uint32_t spin_limit, uint32_t *err, uint32_t first = 0)
+                                            uint32_t spin_limit, uint32_t *err, uint32_t first = 0,
+                                            float s2 = 0.0f, float s1 = 0.0f)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const long pred = (long)chunk - 1 - lane;
@@ -241,6 +243,10 @@ __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int se
     float v2 = 0.0f, v1 = 0.0f;
     // 0 not ready, 1 partial, 2 full, 3 before the signal start, 4 outside the window
     int st = !active ? 4 : (pred < (long)first ? 3 : 0);
+    if (st == 3) {  // the state before the chain start, a full record
+        v2 = s2;
+        v1 = s1;
+    }
     int D = 0;
     bool from_full = false;
     uint32_t spins = 0;

@@ -374,9 +374,10 @@ int iirx_plan_set_plr_taps(iirx_plan *p, const double *b, const double *a, int k
     return IIRX_OK;
 }
 
-static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
+static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, const float *d_zi = nullptr,
+                  float *d_zf = nullptr)
 {
-    LaunchArgs a;
+    LaunchArgs a{};
     a.in = d_in;
     a.out = d_out;
     a.coef = p->d_coef;
@@ -393,6 +394,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
     a.ns = (p->cfg.kernel == IIRX_KERNEL_PLR) ? 2 * p->cfg.n_sections : p->cfg.n_sections;
     a.coef_stride = p->coef_stride;
     a.unrolled = p->cfg.unrolled;
+    a.zi = d_zi;
     hipError_t e = hipErrorInvalidValue;
     switch (p->cfg.kernel) {
     case IIRX_KERNEL_PH: e = launch_ph(p->lanes, p->n_blocks, a, s); break;
@@ -402,6 +404,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s)
     case IIRX_KERNEL_BLOCKSTATE: e = launch_bs(p->n_blocks, p->cfg.block_size, p->cfg.n_sections, a, s); break;
     default: break;
     }
+    if (e == hipSuccess && d_zf) e = launch_ph_end_state(a, (int)p->chunk, d_zf, s);
     if (e != hipSuccess) return IIRX_ERR_HIP;
     p->launches++;
     return IIRX_OK;
@@ -415,6 +418,21 @@ int iirx_execute(iirx_plan *p, const float *d_in, float *d_out, void *stream)
     const uintptr_t ob = (uintptr_t)d_out, oe = ob + (uintptr_t)p->n_samples * 4;
     if (ib < oe && ob < ie) return IIRX_ERR_INVALID;  // overlap
     return launch(p, d_in, d_out, (hipStream_t)stream);
+}
+
+int iirx_execute_state(iirx_plan *p, const float *d_in, float *d_out, const float *d_zi, float *d_zf, void *stream)
+{
+    if (!p || !d_in || !d_out) return IIRX_ERR_INVALID;
+    if (p->cfg.kernel != IIRX_KERNEL_PH) return IIRX_ERR_INVALID;  // streaming state: PH plans
+    if (((uintptr_t)d_in & 15) || ((uintptr_t)d_out & 15)) return IIRX_ERR_INVALID;
+    const uintptr_t ib = (uintptr_t)d_in, ie = ib + (uintptr_t)p->n_samples * 4;
+    const uintptr_t ob = (uintptr_t)d_out, oe = ob + (uintptr_t)p->n_samples * 4;
+    if (ib < oe && ob < ie) return IIRX_ERR_INVALID;  // overlap
+    if (d_zf) {
+        const uintptr_t zb = (uintptr_t)d_zf, ze = zb + (uintptr_t)(2 + 2 * p->cfg.n_sections) * 4;
+        if ((zb < ie && ib < ze) || (zb < oe && ob < ze)) return IIRX_ERR_INVALID;
+    }
+    return launch(p, d_in, d_out, (hipStream_t)stream, d_zi, d_zf);
 }
 
 int iirx_plan_info_get(const iirx_plan *p, iirx_plan_info *info)

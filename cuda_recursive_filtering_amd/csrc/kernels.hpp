@@ -25,6 +25,9 @@ struct LaunchArgs {
     int32_t ns;                  // sections (PH/STCR/DTCR) or order k (PLR)
     int32_t coef_stride;
     int32_t unrolled;
+    // PH streaming state (null: zero state): x[-2], x[-1] of the input, then
+    // y_s[-2], y_s[-1] of every section's output (2 + 2 ns floats)
+    const float *zi;
 };
 
 // plan control block: ticket, epoch, watchdog flag, then 64 arrival slots
@@ -287,6 +290,9 @@ hipError_t launch_batch(int algo, int n, const BatchArgs &a, hipStream_t s);
 const void *kernel_batch(int algo, int n);
 
 hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s);
+// end state of a PH launch in the layout of LaunchArgs::zi, from the last
+// chunk's full records and the last two input samples
+hipError_t launch_ph_end_state(const LaunchArgs &a, int chunk_samples, float *zf, hipStream_t s);
 
 // one block-state table built on the GPU in FP64 (bs_setup.hip): the
 // sections of one cascade and where its table goes

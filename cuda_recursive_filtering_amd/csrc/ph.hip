@@ -47,6 +47,9 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
     } else if (chunk > 0) {
         xi2 = a.in[base - 2];
         xi1 = a.in[base - 1];
+    } else if (a.zi) {
+        xi2 = a.zi[0];
+        xi1 = a.zi[1];
     } else {
         xi2 = 0.0f;
         xi1 = 0.0f;
@@ -93,8 +96,10 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
         uint64_t *rec = rec2(a.lb, chunk, sec, a.ns);
         if (lane == 0) publish2(rec, false, part_flag, p2, p1);
         const int depth = (int)c[Y.kDepth];
+        // the chain starts from the streamed-in state of this section (zero by default)
+        const float z2 = a.zi ? a.zi[2 + 2 * sec] : 0.0f, z1 = a.zi ? a.zi[3 + 2 * sec] : 0.0f;
         const Carry2 X = lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
-                                   a.spin_limit, a.err);
+                                   a.spin_limit, a.err, 0u, z2, z1);
         if (lane == 0) {
             const float f2 = fmaf(m1, X.y1, fmaf(m0, X.y2, p2));
             const float f1 = fmaf(m3, X.y1, fmaf(m2, X.y2, p1));
@@ -130,6 +135,25 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
     __syncthreads();
     stage_out<L, N>(lds, a.out + base);
     });
+}
+
+// zf = x[n-2], x[n-1], then (y2, y1) of every section from the last chunk's
+// full records (the launch that wrote them has finished: same stream)
+__global__ __launch_bounds__(64) void ph_end_state_kernel(LaunchArgs a, int chunk_samples, float *zf)
+{
+    const int t = threadIdx.x;
+    const uint32_t last = a.num_chunks - 1u;
+    if (t < 2) zf[t] = a.in[(long)last * chunk_samples + chunk_samples - 2 + t];
+    if (t < 2 * a.ns) {
+        const uint64_t *r = rec2(a.lb, last, t >> 1, a.ns);
+        zf[2 + t] = rec_val(__hip_atomic_load(r + 2 + (t & 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+}
+
+hipError_t launch_ph_end_state(const LaunchArgs &a, int chunk_samples, float *zf, hipStream_t s)
+{
+    hipLaunchKernelGGL(ph_end_state_kernel, dim3(1), dim3(64), 0, s, a, chunk_samples, zf);
+    return hipGetLastError();
 }
 
 template <int N>
