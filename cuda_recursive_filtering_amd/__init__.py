@@ -218,7 +218,7 @@ class Plan:
     __call__ = execute
 
     def execute_state(self, d_in, d_out, zi=None, zf=None, stream=None):
-        """PH plans: one launch with the filter state carried in and out (a long
+        """PH, STCR and DTCR plans: one launch with the filter state carried in and out (a long
         signal filtered in calls of n_samples).  zi / zf: device float32
         tensors (or pointers) of 2 + 2 * n_sections values: x[-2], x[-1] of the
         input, then y_s[-2], y_s[-1] of every section's output; None = zero

@@ -48,6 +48,9 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
     if (chunk > 0) {
         g1 = a.in[base - 1];
         g2 = a.in[base - 2];
+    } else if (a.zi) {  // streamed-in input history
+        g1 = a.zi[1];
+        g2 = a.zi[0];
     }
     // boundary state carried between sections for tx == 0 (dtcr:337-371)
     float xi1 = 0.0f, xi2 = 0.0f, xi3 = 0.0f;
@@ -179,8 +182,10 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
         if (ty == 0) {
             const float m0 = c[Y.kM + 0], m1 = c[Y.kM + 1], m2 = c[Y.kM + 2], m3 = c[Y.kM + 3];
             const int depth = (int)c[Y.kDepth];
+            // the chain starts from the streamed-in state of this section (zero by default)
+            const float z2 = a.zi ? a.zi[2 + 2 * sec] : 0.0f, z1 = a.zi ? a.zi[3 + 2 * sec] : 0.0f;
             const Carry2 X = lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
-                                       a.spin_limit, a.err);
+                                       a.spin_limit, a.err, 0u, z2, z1);
             if (tx == 0) {
                 xch[1] = X.y2;
                 xch[2] = X.y1;

@@ -423,7 +423,9 @@ int iirx_execute(iirx_plan *p, const float *d_in, float *d_out, void *stream)
 int iirx_execute_state(iirx_plan *p, const float *d_in, float *d_out, const float *d_zi, float *d_zf, void *stream)
 {
     if (!p || !d_in || !d_out) return IIRX_ERR_INVALID;
-    if (p->cfg.kernel != IIRX_KERNEL_PH) return IIRX_ERR_INVALID;  // streaming state: PH plans
+    // streaming state: the cascades whose chunk carry is the direct-form output history
+    if (p->cfg.kernel != IIRX_KERNEL_PH && p->cfg.kernel != IIRX_KERNEL_STCR && p->cfg.kernel != IIRX_KERNEL_DTCR)
+        return IIRX_ERR_INVALID;
     if (((uintptr_t)d_in & 15) || ((uintptr_t)d_out & 15)) return IIRX_ERR_INVALID;
     const uintptr_t ib = (uintptr_t)d_in, ie = ib + (uintptr_t)p->n_samples * 4;
     const uintptr_t ob = (uintptr_t)d_out, oe = ob + (uintptr_t)p->n_samples * 4;

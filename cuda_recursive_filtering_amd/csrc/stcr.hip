@@ -77,6 +77,9 @@ __global__ __launch_bounds__(64) void stcr_kernel(LaunchArgs a)
     } else if (chunk > 0) {
         xi2 = a.in[base - 2];
         xi1 = a.in[base - 1];
+    } else if (a.zi) {  // streamed-in input history
+        xi2 = a.zi[0];
+        xi1 = a.zi[1];
     }
 
     for (int sec = 0; sec < a.ns; ++sec) {
@@ -164,8 +167,10 @@ __global__ __launch_bounds__(64) void stcr_kernel(LaunchArgs a)
         uint64_t *rec = rec2(a.lb, chunk, sec, a.ns);
         if (lane == 0) publish2(rec, false, part_flag, p2, p1);
         const int depth = (int)c[Y.kDepth];
+        // the chain starts from the streamed-in state of this section (zero by default)
+        const float z2 = a.zi ? a.zi[2 + 2 * sec] : 0.0f, z1 = a.zi ? a.zi[3 + 2 * sec] : 0.0f;
         const Carry2 Xc = lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
-                                   a.spin_limit, a.err);
+                                   a.spin_limit, a.err, 0u, z2, z1);
         const float X0 = Xc.y2, X1 = Xc.y1;
 
         // terminal correction (stcr:480-491)

@@ -169,14 +169,15 @@ int iirx_plan_set_plr_taps(iirx_plan *plan, const double *b, const double *a,
 int iirx_execute(iirx_plan *plan, const float *d_in, float *d_out,
                  void *stream);
 
-/* Streaming state (PH plans): iirx_execute with the filter state carried in
+/* Streaming state (PH, STCR and DTCR plans): iirx_execute with the filter state carried in
  * and out, so a long signal can be filtered in calls of n_samples each.  The
  * reference has only zero-state hooks (ph_kernels.cuh:45, 162-165).  A state
  * is 2 + 2 N_SECTIONS floats in device memory: x[-2], x[-1] of the input,
  * then y_s[-2], y_s[-1] of the output of every section s (direct-form
  * histories).  d_zi NULL: zero state; d_zf NULL: not written.  d_zi is read
  * by the launch and d_zf written by a small kernel after it, both on
- * `stream`; d_zf may equal d_zi of the next call.  Other kernels: INVALID. */
+ * `stream`; d_zf may equal d_zi of the next call.  PLR and the block-state
+ * form: INVALID. */
 int iirx_execute_state(iirx_plan *plan, const float *d_in, float *d_out,
                        const float *d_zi, float *d_zf, void *stream);
 

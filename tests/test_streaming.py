@@ -15,7 +15,7 @@ import oracle_lib as O
 from workloads import normal_workload
 
 
-def _stream(device, sos, x, calls, schedule="persistent"):
+def _stream(device, sos, x, calls, schedule="persistent", kernel="PH", geom=(64, 32), unrolled=False):
     import torch
     n = x.size // calls
     ns = sos.shape[0]
@@ -23,7 +23,7 @@ def _stream(device, sos, x, calls, schedule="persistent"):
     out = torch.empty_like(xin)
     za = torch.zeros(2 + 2 * ns, dtype=torch.float32, device=device)
     zb = torch.empty_like(za)
-    with iirx.Plan("PH", sos, n, block_size=64, n_blocks=32, schedule=schedule) as p:
+    with iirx.Plan(kernel, sos, n, block_size=geom[0], n_blocks=geom[1], unrolled=unrolled, schedule=schedule) as p:
         for c in range(calls):
             p.execute_state(xin[c * n:(c + 1) * n], out[c * n:(c + 1) * n], zi=za, zf=zb)
             za, zb = zb, za
@@ -55,6 +55,22 @@ def test_stream_many_calls_narrow_filter(device, schedule):
     y, _ = _stream(device, sos, x, 16, schedule)
     ok, err, mref, rel = O.gate(y, O.golden(sos, x))
     assert ok, rel
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel,geom,unrolled", [("STCR", (32, 64), False), ("STCR", (64, 64), True),
+                                                  ("DTCR", (64, 64), False), ("DTCR", (32, 32), False)])
+@pytest.mark.parametrize("order", [4, 16])
+def test_stream_cr_kernels(device, kernel, geom, unrolled, order):
+    """STCR and DTCR carry the same direct-form history between chunks, so the
+    state streams the same way."""
+    print(f"case stream {kernel} {geom} unrolled={unrolled} order={order}", flush=True)
+    sos, g, x = normal_workload(order, 1 << 20, seed=35, wn=0.1)
+    y, _ = _stream(device, sos, x, 4, kernel=kernel, geom=geom, unrolled=unrolled)
+    ok, err, mref, rel = O.gate(y, O.golden(sos, x))
+    assert ok, rel
+    y1, _ = _stream(device, sos, x, 1, kernel=kernel, geom=geom, unrolled=unrolled)
+    assert np.max(np.abs(y - y1)) <= 64 * np.finfo(np.float32).eps * mref
 
 
 @pytest.mark.gpu
@@ -97,11 +113,12 @@ def test_stream_state_out_is_the_direct_form_history(device):
 
 
 @pytest.mark.gpu
-def test_stream_state_only_for_ph(device):
+def test_stream_state_rejected_by_plr_and_blockstate(device):
     import torch
     sos, g = iirx.butterworth_sos(4, 0.1)
     xin = torch.zeros(1 << 16, dtype=torch.float32, device=device)
     out = torch.empty_like(xin)
-    with iirx.Plan("STCR", sos, xin.numel(), block_size=64, n_blocks=32) as p:
-        with pytest.raises(iirx.IirxError):
-            p.execute_state(xin, out)
+    for kernel, kw in (("PLR", {}), ("BS", {"block_size": 64, "n_blocks": 64})):
+        with iirx.Plan(kernel, sos, xin.numel(), **kw) as p:
+            with pytest.raises(iirx.IirxError):
+                p.execute_state(xin, out)
