@@ -24,7 +24,7 @@ from . import _lib
 from ._lib import IirxError, KERNELS, check, lib
 
 __all__ = [
-    "Plan", "Batch", "IirxError", "KERNELS", "butterworth_sos", "direct_form_taps", "fill_normal",
+    "Plan", "Batch", "IirxError", "KERNELS", "butterworth_sos", "normalize_sos", "direct_form_taps", "fill_normal",
     "plan_tables", "config_check", "version", "tune_candidates", "autotune",
 ]
 
@@ -70,6 +70,17 @@ def butterworth_sos(order, wn):
     g = ctypes.c_double()
     check(lib().iirx_butterworth_sos(int(order), float(wn), sos.ctypes.data, ctypes.byref(g)), "butterworth")
     return sos, g.value
+
+
+def normalize_sos(sos):
+    """General sections (rows b0 b1 b2 a0 a1 a2, float64, any scaling; scipy's
+    cheby1/ellip/... output) into the filter contract: (float32 rows with
+    b0 == a0 == 1, gain) with gain the factor to apply to the input."""
+    s = np.ascontiguousarray(np.asarray(sos, np.float64).reshape(-1, 6))
+    out = np.zeros(s.shape, np.float32)
+    g = ctypes.c_double()
+    check(lib().iirx_normalize_sos(s.ctypes.data, s.shape[0], out.ctypes.data, ctypes.byref(g)), "normalize_sos")
+    return out, g.value
 
 
 def direct_form_taps(sos):

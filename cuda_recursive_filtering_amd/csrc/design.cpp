@@ -51,6 +51,31 @@ int iirx_fill_normal(uint64_t seed, int64_t n, float *out)
     return IIRX_OK;
 }
 
+int iirx_normalize_sos(const double *in, int ns, float *out, double *gain)
+{
+    if (!in || !out || !gain || ns < 1) return IIRX_ERR_INVALID;
+    double g = 1.0;
+    for (int s = 0; s < ns; ++s) {
+        const double *r = in + 6 * s;
+        for (int k = 0; k < 6; ++k)
+            if (!std::isfinite(r[k])) return IIRX_ERR_INVALID;
+        if (r[0] == 0.0 || r[3] == 0.0) return IIRX_ERR_INVALID;
+    }
+    for (int s = 0; s < ns; ++s) {
+        const double *r = in + 6 * s;
+        float *o = out + 6 * s;
+        o[0] = 1.0f;
+        o[1] = (float)(r[1] / r[0]);
+        o[2] = (float)(r[2] / r[0]);
+        o[3] = 1.0f;
+        o[4] = (float)(r[4] / r[3]);
+        o[5] = (float)(r[5] / r[3]);
+        g *= r[0] / r[3];
+    }
+    *gain = g;
+    return IIRX_OK;
+}
+
 int iirx_butterworth_sos(int order, double wn, float *sos, double *gain)
 {
     if (order < 2 || order > 64 || (order & 1) || !(wn > 0.0 && wn < 1.0) || !sos) return IIRX_ERR_INVALID;

@@ -122,6 +122,8 @@ int iirx_batch_create_ex(const iirx_batch_config *cfg, int64_t n_channels, const
     }
     for (int64_t i = 0; i < 6 * total_sec; ++i)
         if (!std::isfinite(sos[i])) return IIRX_ERR_INVALID;
+    for (int64_t s = 0; s < total_sec; ++s)  // b0 == a0 == 1 (iirx_normalize_sos)
+        if (sos[6 * s] != 1.0f || sos[6 * s + 3] != 1.0f) return IIRX_ERR_INVALID;
     if (offsets[n_channels] == offsets[0]) return IIRX_ERR_INVALID;  // no samples at all
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IIRX_ERR_NO_DEVICE;

@@ -65,6 +65,18 @@ namespace {
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// rows of the filter contract: finite, b0 == 1 and a0 == 1 (iirx_normalize_sos makes them)
+bool sos_ok(const float *sos, int ns)
+{
+    for (int s = 0; s < ns; ++s) {
+        const float *r = sos + 6 * s;
+        for (int k = 0; k < 6; ++k)
+            if (!std::isfinite(r[k])) return false;
+        if (r[0] != 1.0f || r[3] != 1.0f) return false;
+    }
+    return true;
+}
+
 int validate(const iirx_config *cfg, int64_t n)
 {
     if (!cfg) return IIRX_ERR_INVALID;
@@ -242,6 +254,7 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     *out = nullptr;
     int st = validate(cfg, n_samples);
     if (st != IIRX_OK) return st;
+    if (!sos_ok(sos, cfg->n_sections)) return IIRX_ERR_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IIRX_ERR_NO_DEVICE;
 
@@ -334,6 +347,7 @@ int iirx_plan_tables(const iirx_config *cfg, const float *sos, int64_t n_samples
     if (!sos) return IIRX_ERR_INVALID;
     int st = validate(cfg, n_samples);
     if (st != IIRX_OK) return st;
+    if (!sos_ok(sos, cfg->n_sections)) return IIRX_ERR_INVALID;
     std::vector<float> t;
     Shape sh;
     st = make_shape(cfg, sos, n_samples, nullptr, nullptr, sh, t);

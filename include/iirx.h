@@ -226,6 +226,15 @@ const char *iirx_version(void);
  */
 int iirx_butterworth_sos(int order, double wn, float *sos, double *gain);
 
+/* General sections into the filter contract: rows {b0, b1, b2, a0, a1, a2}
+ * (float64, any scaling, b2 = a2 = 0 for a first-order section) become
+ * float32 rows {1, b1/b0, b2/b0, 1, a1/a0, a2/a0} and *gain = prod(b0/a0),
+ * the factor the caller applies to the input, as ref_generate.py:61-67 does
+ * with the Butterworth gain.  The reference takes only its own Butterworth
+ * designs (ref_generate.py:50).  INVALID if a b0 or a0 is 0 or a value is
+ * not finite.  Plans reject rows with b0 != 1 or a0 != 1. */
+int iirx_normalize_sos(const double *sos_in, int ns, float *sos_out, double *gain);
+
 /* Direct-form taps of the cascade (the PLR input of ref_generate.py:74-81):
  * FP64 polynomial product of the sections; b[0..2ns) FIR taps past b0 == 1,
  * a[0..2ns) feedback taps in the recurrence sign. */
