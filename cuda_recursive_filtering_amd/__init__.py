@@ -17,6 +17,7 @@ Device memory and streams come from PyTorch (tensors' data pointers and
 torch.cuda streams); the computation is libiirx.so's HIP kernels only.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -25,6 +26,7 @@ from ._lib import IirxError, KERNELS, check, lib
 
 __all__ = [
     "Plan", "Batch", "IirxError", "KERNELS", "butterworth_sos", "normalize_sos", "direct_form_taps", "fill_normal",
+    "filter_taps_design", "write_filter_taps", "read_filter_taps",
     "plan_tables", "config_check", "version", "tune_candidates", "autotune",
 ]
 
@@ -91,6 +93,48 @@ def direct_form_taps(sos):
     a = np.zeros(2 * ns, np.float64)
     check(lib().iirx_direct_form_taps(s.ctypes.data, ns, b.ctypes.data, a.ctypes.data), "direct_form_taps")
     return b, a
+
+
+def filter_taps_design(n_sections, cutoff_wn=0.2):
+    """The reference's test filter as its generated header holds it
+    (ref_generate.py:56-81): a dict with n_sections, plr_order, gain,
+    input_fs, input_freq, cutoff_wn, sos (float32 [ns, 6]), plr_b, plr_a."""
+    t = _lib.FilterTaps()
+    check(lib().iirx_filter_taps_design(int(n_sections), float(cutoff_wn), ctypes.byref(t)), "filter_taps_design")
+    return _taps_dict(t)
+
+
+def write_filter_taps(taps, path):
+    """Write a taps dict as the reference's build/filter_taps.hpp."""
+    ns, k = int(taps["n_sections"]), int(taps["plr_order"])
+    t = _lib.FilterTaps(n_sections=ns, plr_order=k, gain=taps["gain"], input_fs=taps["input_fs"],
+                        input_freq=taps["input_freq"], cutoff_wn=taps.get("cutoff_wn", 0.0))
+    sos = np.asarray(taps["sos"], np.float32).reshape(-1)
+    if sos.size != 6 * ns or len(taps["plr_b"]) != k or len(taps["plr_a"]) != k:
+        raise ValueError("taps: array sizes disagree with n_sections / plr_order")
+    for i, v in enumerate(sos):
+        t.sos[i] = v
+    for i in range(k):
+        t.plr_b[i] = taps["plr_b"][i]
+        t.plr_a[i] = taps["plr_a"][i]
+    check(lib().iirx_filter_taps_write(ctypes.byref(t), os.fsencode(path)), "filter_taps_write")
+
+
+def read_filter_taps(path):
+    """Read a build/filter_taps.hpp (the reference's or one written here)."""
+    t = _lib.FilterTaps()
+    check(lib().iirx_filter_taps_read(os.fsencode(path), ctypes.byref(t)), "filter_taps_read")
+    return _taps_dict(t)
+
+
+def _taps_dict(t):
+    ns, k = t.n_sections, t.plr_order
+    return {
+        "n_sections": ns, "plr_order": k, "gain": t.gain, "input_fs": t.input_fs, "input_freq": t.input_freq,
+        "cutoff_wn": t.cutoff_wn,
+        "sos": np.array(t.sos[:6 * ns], np.float32).reshape(ns, 6),
+        "plr_b": np.array(t.plr_b[:k], np.float64), "plr_a": np.array(t.plr_a[:k], np.float64),
+    }
 
 
 def fill_normal(seed, n):

@@ -100,6 +100,24 @@ class BatchInfo(ctypes.Structure):
     ]
 
 
+TAPS_MAX_SECTIONS = 8
+
+
+class FilterTaps(ctypes.Structure):
+    """iirx_filter_taps: the reference's generated filter header as data."""
+    _fields_ = [
+        ("n_sections", ctypes.c_int32),
+        ("plr_order", ctypes.c_int32),
+        ("gain", ctypes.c_double),
+        ("input_fs", ctypes.c_double),
+        ("input_freq", ctypes.c_double),
+        ("cutoff_wn", ctypes.c_double),
+        ("sos", ctypes.c_float * (TAPS_MAX_SECTIONS * 6)),
+        ("plr_b", ctypes.c_double * (2 * TAPS_MAX_SECTIONS)),
+        ("plr_a", ctypes.c_double * (2 * TAPS_MAX_SECTIONS)),
+    ]
+
+
 # every symbol include/iirx.h declares, with its ctypes signature
 _P = ctypes.c_void_p
 _SIGS = {
@@ -138,6 +156,9 @@ _SIGS = {
     "iirx_normalize_sos": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_double)]),
     "iirx_batch_read_tables": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "iirx_plan_read_tables": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
+    "iirx_filter_taps_design": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.POINTER(FilterTaps)]),
+    "iirx_filter_taps_write": (ctypes.c_int, [ctypes.POINTER(FilterTaps), ctypes.c_char_p]),
+    "iirx_filter_taps_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(FilterTaps)]),
 }
 
 _lib = None

@@ -9,12 +9,19 @@
 //   iirx_driver accuracy|performance --kernel PH|STCR|DTCR|PLR|BS
 //       [--block-size B] [--n-blocks N] [--sections NS] [--log2n L]
 //       [--unrolled] [--plr-x-max X] [--schedule persistent|ticket]
+//       [--taps FILE] [--write-taps FILE]
 //       [--reference FILE] [--warmup W] [--batches K] [--batch M]
+//   iirx_driver taps --sections NS --write-taps FILE    (header only, no GPU)
 //
 // Both modes filter the reference workload (ref_generate.py:56-99): a
 // Butterworth lowpass of order 2 NS at 0.2 x Nyquist in b0-normalised SOS
-// form, input FILTER_GAIN * cos(2 pi 120 i / 1000) computed in float64 and
-// rounded to float32 once.  accuracy makes one launch (test_accuracy.cu:
+// form, input FILTER_GAIN * cos(2 pi INPUT_FREQ i / INPUT_FS) computed in
+// float64 and rounded to float32 once.  --taps takes the filter, gain, signal
+// and PLR taps from a generated header instead (the reference's
+// build/filter_taps.hpp, which its drivers compile in), with its
+// FILTER_N_SECTIONS == N_SECTIONS check (test_accuracy.cu:107-109) made at
+// run time; --write-taps writes the header of the filter used, in the
+// reference's declarations, so the reference's drivers can be built on it.  accuracy makes one launch (test_accuracy.cu:
 // 146-160); performance times P20 of batch means over rotating input buffers
 // (test_performance.cu:144-198, measure.cuh:38-110).  With --reference, the
 // output is checked against that file (float32, N samples: the float64
@@ -41,16 +48,18 @@ struct Options {
     int kernel = IIRX_KERNEL_PH;
     int block_size = -1, n_blocks = 64, sections = 2, log2n = 20, unrolled = 0, plr_x_max = 0;
     int schedule = IIRX_SCHEDULE_PERSISTENT;
-    std::string reference;
+    std::string reference, taps, write_taps;
+    bool sections_given = false;
     int warmup = 100, batches = 20, batch = 200;
 };
 
 int usage(const char *prog)
 {
-    std::printf("usage: %s accuracy|performance --kernel PH|STCR|DTCR|PLR|BS [--block-size B] [--n-blocks N]\n"
+    std::printf("usage: %s taps --sections NS --write-taps FILE\n"
+                "       %s accuracy|performance --kernel PH|STCR|DTCR|PLR|BS [--block-size B] [--n-blocks N]\n"
                 "       [--sections NS] [--log2n L] [--unrolled] [--plr-x-max X] [--schedule persistent|ticket]\n"
-                "       [--reference FILE] [--warmup W] [--batches K] [--batch M]\n",
-                prog);
+                "       [--taps FILE] [--write-taps FILE] [--reference FILE] [--warmup W] [--batches K] [--batch M]\n",
+                prog, prog);
     return 0;
 }
 
@@ -58,7 +67,7 @@ bool parse(int argc, char **argv, Options &o)
 {
     if (argc < 2) return false;
     o.mode = argv[1];
-    if (o.mode != "accuracy" && o.mode != "performance") return false;
+    if (o.mode != "accuracy" && o.mode != "performance" && o.mode != "taps") return false;
     for (int i = 2; i < argc; ++i) {
         const std::string a = argv[i];
         auto next = [&]() -> const char * { return (i + 1 < argc) ? argv[++i] : nullptr; };
@@ -78,11 +87,16 @@ bool parse(int argc, char **argv, Options &o)
             else return false;
         } else if (a == "--block-size") o.block_size = std::atoi(v);
         else if (a == "--n-blocks") o.n_blocks = std::atoi(v);
-        else if (a == "--sections") o.sections = std::atoi(v);
+        else if (a == "--sections") {
+            o.sections = std::atoi(v);
+            o.sections_given = true;
+        }
         else if (a == "--log2n") o.log2n = std::atoi(v);
         else if (a == "--plr-x-max") o.plr_x_max = std::atoi(v);
         else if (a == "--schedule") o.schedule = std::string(v) == "ticket" ? IIRX_SCHEDULE_TICKET : IIRX_SCHEDULE_PERSISTENT;
         else if (a == "--reference") o.reference = v;
+        else if (a == "--taps") o.taps = v;
+        else if (a == "--write-taps") o.write_taps = v;
         else if (a == "--warmup") o.warmup = std::atoi(v);
         else if (a == "--batches") o.batches = std::atoi(v);
         else if (a == "--batch") o.batch = std::atoi(v);
@@ -142,7 +156,29 @@ int main(int argc, char **argv)
         return -1;
     }
     const int64_t n = 1LL << o.log2n;
-    const int order = 2 * o.sections;
+
+    // filter and input of the reference workload (ref_generate.py:56-99), or
+    // those of a generated header
+    iirx_filter_taps taps{};
+    if (!o.taps.empty()) {
+        if (iirx_filter_taps_read(o.taps.c_str(), &taps) != IIRX_OK) {
+            std::fprintf(stderr, "ERROR: cannot read filter header %s\n", o.taps.c_str());
+            return -1;
+        }
+        if (o.sections_given && taps.n_sections != o.sections) {
+            std::fprintf(stderr, "ERROR: FILTER_N_SECTIONS (%d) in %s != N_SECTIONS (%d)\n", taps.n_sections,
+                         o.taps.c_str(), o.sections);
+            return -1;
+        }
+        o.sections = taps.n_sections;
+    } else if (iirx_filter_taps_design(o.sections, 0.2, &taps) != IIRX_OK) {
+        return -1;
+    }
+    if (!o.write_taps.empty() && iirx_filter_taps_write(&taps, o.write_taps.c_str()) != IIRX_OK) {
+        std::fprintf(stderr, "ERROR: cannot write %s\n", o.write_taps.c_str());
+        return -1;
+    }
+    if (o.mode == "taps") return 0;  // header only, no device
     iirx_config cfg = {o.kernel, o.block_size, o.n_blocks, o.sections, o.unrolled, 0, o.plr_x_max, o.schedule};
     int st = iirx_config_check(&cfg, n);
     if (st != IIRX_OK) {
@@ -150,17 +186,18 @@ int main(int argc, char **argv)
         return -1;
     }
 
-    // filter and input of the reference workload (ref_generate.py:56-99)
-    std::vector<float> sos(6 * o.sections);
-    double gain = 0.0;
-    if (iirx_butterworth_sos(order, 0.2, sos.data(), &gain) != IIRX_OK) return -1;
     std::vector<float> h_in((size_t)n), h_out((size_t)n);
-    for (int64_t i = 0; i < n; ++i) h_in[(size_t)i] = (float)(gain * std::cos(2.0 * M_PI * 120.0 * (double)i / 1000.0));
+    for (int64_t i = 0; i < n; ++i)
+        h_in[(size_t)i] = (float)(taps.gain * std::cos(2.0 * M_PI * taps.input_freq * (double)i / taps.input_fs));
 
     iirx_plan *plan = nullptr;
-    st = iirx_plan_create(&cfg, sos.data(), n, &plan);
+    st = iirx_plan_create(&cfg, taps.sos, n, &plan);
+    // PLR runs on the header's direct-form taps, as plr_kernels.cuh:416-427
+    if (st == IIRX_OK && o.kernel == IIRX_KERNEL_PLR && !o.taps.empty())
+        st = iirx_plan_set_plr_taps(plan, taps.plr_b, taps.plr_a, taps.plr_order);
     if (st != IIRX_OK) {
         std::fprintf(stderr, "ERROR: plan: %s\n", iirx_status_string(st));
+        iirx_plan_destroy(plan);
         return -1;
     }
     iirx_plan_info info{};

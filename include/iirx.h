@@ -244,6 +244,51 @@ int iirx_direct_form_taps(const float *sos, int ns, double *b, double *a);
  * Box-Muller in float64, rounded to float32). */
 int iirx_fill_normal(uint64_t seed, int64_t n, float *out);
 
+/* ---- the reference harness's artefacts ------------------------------- */
+
+/*
+ * The generated header the reference's drivers compile in
+ * (build/filter_taps.hpp, written by ref_generate.py:101-131 and read by
+ * test_accuracy.cu:107-109, ph_kernels.cuh:151, plr_kernels.cuh:416-427),
+ * as data: FILTER_N_SECTIONS, FILTER_GAIN, INPUT_FS, INPUT_FREQ,
+ * FILTER_SOS[ns][6] (b0-normalised, float32), PLR_FILTER_ORDER and the
+ * direct-form taps PLR_FILTER_B / PLR_FILTER_A (b0-normalised, FIR taps
+ * past b0 == 1, feedback taps in the sign y[n] += A[j] y[n-j-1]).
+ * cutoff_wn appears only in the header's comment (0 if not known).
+ */
+#define IIRX_TAPS_MAX_SECTIONS 8
+typedef struct iirx_filter_taps {
+    int32_t n_sections;
+    int32_t plr_order;
+    double gain;
+    double input_fs;
+    double input_freq;
+    double cutoff_wn;
+    float sos[IIRX_TAPS_MAX_SECTIONS * 6];
+    double plr_b[2 * IIRX_TAPS_MAX_SECTIONS];
+    double plr_a[2 * IIRX_TAPS_MAX_SECTIONS];
+} iirx_filter_taps;
+
+/* The reference's test filter and signal (ref_generate.py:56-81): the
+ * Butterworth lowpass of order 2 n_sections at cutoff_wn (the reference
+ * uses 0.2), input_fs = 1000, input_freq = 120; the direct-form taps are
+ * the FP64 product of the unrounded sections. n_sections in 1..8. */
+int iirx_filter_taps_design(int n_sections, double cutoff_wn, iirx_filter_taps *taps);
+
+/* Writes `taps` as a header with the declarations of ref_generate.py:
+ * 106-129 (same names, types and text of every value: floats as %.9e with
+ * an f suffix, doubles as the shortest text that reads back exactly), so
+ * the reference's drivers compile against it. IO error: INVALID. */
+int iirx_filter_taps_write(const iirx_filter_taps *taps, const char *path);
+
+/* Reads such a header (one written by ref_generate.py or by
+ * iirx_filter_taps_write): comments are skipped, float values are read as
+ * a compiler reads an f-suffixed literal (one rounding to float32).
+ * INVALID if the file cannot be read, a declaration is missing, a count
+ * disagrees with FILTER_N_SECTIONS / PLR_FILTER_ORDER or n_sections is
+ * outside 1..8. */
+int iirx_filter_taps_read(const char *path, iirx_filter_taps *taps);
+
 /*
  * Host copy of the coefficient tables a plan of this configuration would
  * upload (FP64 precompute rounded to float32).  Writes min(cap, size)
