@@ -8,8 +8,10 @@
 // the channel's first chunk (zero initial state per channel).  A channel's
 // last chunk is partial: samples past the channel end are read as zero (the
 // filter is causal, so they cannot reach earlier outputs) and not written.
-// Channel offsets are arbitrary (CSR packing), so HBM is accessed with
-// 4-byte coalesced loads and stores: 64 consecutive floats per instruction.
+// Channel offsets are arbitrary (CSR packing): full chunks move with 16-byte
+// accesses shifted by the chunk's offset mod 4 (stage_in_any below), a
+// channel's last chunk with 4-byte coalesced loads and stores (as do full
+// chunks of buffers that are not 16-byte aligned).
 // A channel may instead run the block-state form (bs_core.hpp): tiles of 64
 // samples, G tiles per chunk, its tables read from global memory.  The two
 // forms are separate kernels, each with its own register budget, launched one
@@ -22,11 +24,63 @@ typedef const __attribute__((address_space(4))) BatchChannel cchannel;
 typedef const __attribute__((address_space(4))) uint32_t cu32;
 
 // waves per SIMD the register budgets are set for: PH at N = 32 four waves
-// (a 20-byte/lane spill) measured 5% faster than three without one
+// (a 40-byte/lane spill) measured faster than three without one (ragged
+// 4096-channel workload: 457 vs 446 GS/s)
 template <int N>
 constexpr int batch_waves() { return N == 16 ? 5 : N == 32 ? 4 : 1; }
 template <int G>
 constexpr int batch_bs_waves() { return G == 16 ? 5 : G == 32 ? 4 : 2; }
+
+// Full chunk of CH floats at any float offset `base`, moved with 16-byte
+// accesses (`in`/`out` themselves 16-byte aligned): float4 k (k < CH/4, over
+// the wave's 64 threads) covers [base - m + 4k, +4), m = base % 4, so the
+// chunk's element e = 4k - m + t lands at LDS index map(e).  Loads: the m
+// elements before the chunk that float4 0 also reads lie inside the buffer
+// and are dropped; the m elements after the last whole float4 come in scalar.
+// Stores: every float4 but the first lies wholly inside the chunk; the first
+// (when m != 0) and the last m elements are stored one by one, so no
+// neighbour's samples are written.
+template <int CH, class Map>
+__device__ __forceinline__ void stage_in_any(const float *__restrict__ in, long base, float *lds, const Map &map)
+{
+    constexpr int V = CH / 4, PER = V / 64;
+    static_assert(V % 64 == 0, "chunk must split evenly over the wave");
+    const int tid = threadIdx.x & 63, m = (int)(base & 3);
+    const f4 *s4 = reinterpret_cast<const f4 *>(in + (base - m));
+    f4 v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) v[i] = __builtin_nontemporal_load(s4 + i * 64 + tid);
+    const float tail = (tid < m) ? in[base + CH - m + tid] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e0 = 4 * (i * 64 + tid) - m;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (e0 + t >= 0) lds[map(e0 + t)] = v[i][t];
+    }
+    if (tid < m) lds[map(CH - m + tid)] = tail;
+}
+
+template <int CH, class Map>
+__device__ __forceinline__ void stage_out_any(const float *lds, float *__restrict__ out, long base, const Map &map)
+{
+    constexpr int V = CH / 4, PER = V / 64;
+    const int tid = threadIdx.x & 63, m = (int)(base & 3);
+    f4 *d4 = reinterpret_cast<f4 *>(out + (base - m));
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int k = i * 64 + tid, e0 = 4 * k - m;
+        if (k == 0 && m != 0) {
+            for (int t = m; t < 4; ++t) out[base + e0 + t] = lds[map(e0 + t)];
+        } else {
+            f4 w;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) w[t] = lds[map(e0 + t)];
+            __builtin_nontemporal_store(w, d4 + k);
+        }
+    }
+    if (tid < m) out[base + CH - m + tid] = lds[map(CH - m + tid)];
+}
 
 // block-state operands of one channel, from its tables in global memory
 struct BsGlobalOps {
@@ -62,10 +116,13 @@ __device__ __forceinline__ void batch_bs_chunk(const BatchArgs &a, uint32_t chun
     const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
     const cfloat *ctab = as_const(coef);
 
+    const auto tile_map = [](int e) { return (e >> 6) * kBsTilePitch + (e & 63); };
     __syncthreads();  // LDS of the previous chunk fully consumed
     {
         const float *src = a.in + base + lane;
-        if (r == CH) {
+        if (r == CH && a.vec16) {
+            stage_in_any<CH>(a.in, base, lds, tile_map);
+        } else if (r == CH) {
             float v[G];
 #pragma unroll
             for (int t = 0; t < G; ++t) v[t] = src[t * L];
@@ -100,7 +157,9 @@ __device__ __forceinline__ void batch_bs_chunk(const BatchArgs &a, uint32_t chun
                       a.err, first, (int)ctab[Y.kScale + 0], (int)ctab[Y.kScale + 1], ops, store);
     __syncthreads();
     float *dst = a.out + base + lane;
-    if (r == CH) {
+    if (r == CH && a.vec16) {
+        stage_out_any<CH>(lds, a.out, base, tile_map);
+    } else if (r == CH) {
 #pragma unroll
         for (int t = 0; t < G; ++t) dst[t * L] = lds[t * kBsTilePitch + lane];
     } else {
@@ -161,14 +220,19 @@ void batch_ph_kernel(BatchArgs a)
     const long rem = ch_len - j * CH;
     const int r = rem < CH ? (int)rem : CH;  // valid samples in this chunk
 
-    // stage in: chunk element e = 64 i + lane (coalesced), lane t owns
-    // [t N, t N + N).  Full chunks use one base address and immediate offsets;
-    // the channel's last chunk clamps the index to the channel's last sample
-    // (always in bounds) and stores zero past it, one load at a time.
+    // stage in: lane t owns chunk elements [t N, t N + N).  Full chunks of
+    // 16-byte aligned buffers move with 16-byte accesses; otherwise element
+    // e = 64 i + lane is read with coalesced 4-byte loads, from one base
+    // address with immediate offsets in a full chunk; the channel's last chunk
+    // clamps the index to the channel's last sample (always in bounds) and
+    // stores zero past it, one load at a time.
+    const auto row_map = [](int e) { return (e / N) * (N + 1) + e % N; };
     __syncthreads();  // LDS of the previous chunk fully consumed
     {
         const float *src = a.in + base + lane;
-        if (r == CH) {
+        if (r == CH && a.vec16) {
+            stage_in_any<CH>(a.in, base, lds, row_map);
+        } else if (r == CH) {
             float v[N];
 #pragma unroll
             for (int i = 0; i < N; ++i) v[i] = src[i * L];
@@ -277,7 +341,9 @@ void batch_ph_kernel(BatchArgs a)
     for (int n = 0; n < N; ++n) lds[lane * (N + 1) + n] = x[n];
     __syncthreads();
     float *dst = a.out + base;
-    if (r == CH) {
+    if (r == CH && a.vec16) {
+        stage_out_any<CH>(lds, a.out, base, row_map);
+    } else if (r == CH) {
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             const int e = i * L + lane;

@@ -79,6 +79,7 @@ BatchArgs args(const iirx_batch *b, const BatchGroup &g, const float *d_in, floa
     a.grid = g.grid;
     a.coef_stride = b->coef_stride;
     a.chunk_base = g.base;
+    a.vec16 = (((uintptr_t)d_in | (uintptr_t)d_out) & 15u) == 0;
     return a;
 }
 

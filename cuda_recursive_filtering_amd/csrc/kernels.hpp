@@ -283,6 +283,7 @@ struct BatchArgs {
     uint32_t grid;
     int32_t coef_stride;
     uint32_t chunk_base;  // this launch's chunks: chunk_base .. chunk_base + num_chunks - 1
+    int32_t vec16;        // in and out 16-byte aligned: full chunks move with 16-byte accesses
 };
 
 // algo kBatchPH: n = samples per lane; kBatchBS: n = tiles per chunk

@@ -188,6 +188,37 @@ def test_batch_algorithms_agree(device):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("algo", ["ph", "blockstate"])
+def test_batch_16_byte_staging_any_offset(device, algo):
+    """Full chunks of 16-byte aligned buffers move with 16-byte accesses
+    shifted by the chunk's offset mod 4; buffers that are not 16-byte aligned
+    take 4-byte accesses.  Channels start at every offset mod 4 and span
+    several full chunks; both paths give the same output bit for bit, and
+    nothing outside the channels is written."""
+    import torch
+    print(f"case batch 16-byte staging {algo}", flush=True)
+    lengths = [3 * 2048 + 1, 2 * 2048 + 2, 4 * 2048 + 3, 2048 + 7, 5 * 2048, 2 * 2048 + 5, 3 * 2048 + 6, 4096]
+    offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    assert {int(o) % 4 for o in offsets[:-1]} == {0, 1, 2, 3}
+    sections = [iirx.butterworth_sos([2, 4, 8, 16][c % 4], 0.05 + 0.04 * c)[0] for c in range(len(lengths))]
+    n = int(offsets[-1])
+    x = (np.random.default_rng(7).standard_normal(n) * 0.01).astype(np.float32)
+    outs = {}
+    for shift in (0, 1):  # float offset 1: not 16-byte aligned
+        xin = torch.zeros(n + 4, dtype=torch.float32, device=device)
+        xin[shift:shift + n] = torch.from_numpy(x).to(device)
+        out = torch.full((n + 8,), 7.0, dtype=torch.float32, device=device)
+        with iirx.Batch(offsets, sections, algo=algo, n_blocks=32, bs_tiles=32) as b:
+            b.execute(xin[shift:shift + n], out[4 + shift:4 + shift + n])
+            b.check()
+        o = out.cpu().numpy()
+        assert np.all(o[:4 + shift] == 7.0) and np.all(o[4 + shift + n:] == 7.0), "write outside the channels"
+        outs[shift] = o[4 + shift:4 + shift + n]
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    _check_channels(outs[0], x, offsets, sections)
+
+
+@pytest.mark.gpu
 def test_batch_ragged_workload(device):
     """256 channels of the ragged workload's distribution (lengths 2^10..2^16)."""
     import torch
