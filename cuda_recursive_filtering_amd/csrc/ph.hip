@@ -98,6 +98,9 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
         const int depth = (int)c[Y.kDepth];
         // the chain starts from the streamed-in state of this section (zero by default)
         const float z2 = a.zi ? a.zi[2 + 2 * sec] : 0.0f, z1 = a.zi ? a.zi[3 + 2 * sec] : 0.0f;
+        // C^(lane+1), the per-lane carry injection: its load is in flight
+        // during the lookback instead of after it
+        const float4 cf = *reinterpret_cast<const float4 *>(cv + Y.kCfin + 4 * lane);
         const Carry2 X = lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
                                    a.spin_limit, a.err, 0u, z2, z1);
         if (lane == 0) {
@@ -107,7 +110,6 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
         }
 
         // per-lane end state with the chunk carry injected, C^(lane+1) X
-        const float4 cf = *reinterpret_cast<const float4 *>(cv + Y.kCfin + 4 * lane);
         const float e2 = fmaf(cf.y, X.y1, fmaf(cf.x, X.y2, s2));
         const float e1 = fmaf(cf.w, X.y1, fmaf(cf.z, X.y2, s1));
 
