@@ -12,8 +12,9 @@
 // library (the reference's other artefact, reference.bin, is raw float32
 // and needs no reader).
 //
-// Number text: floats as printf's %.9e plus an "f" suffix (Python's .9e of
-// the float32 value, the same correctly rounded digits); doubles as the
+// Number text (locale-independent, std::to_chars / std::from_chars): floats
+// as %.9e plus an "f" suffix (Python's .9e of the float32 value, the same
+// correctly rounded digits); doubles as the
 // shortest decimal that reads back to the same double (the closest such if
 // several), laid out as Python's repr lays it out: positional for decimal
 // exponents -5 < e < 16 with at least one digit after the point, else
@@ -27,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <system_error>
 #include <vector>
 
 namespace {
@@ -77,11 +79,24 @@ std::string repr_double(double v)
     return out;
 }
 
+// %.9e of the float's value (the same correctly rounded digits as printf,
+// independent of the C locale), plus the f suffix
 std::string f32_text(float v)
 {
     char buf[64];
-    std::snprintf(buf, sizeof buf, "%.9ef", (double)v);
+    const std::to_chars_result r = std::to_chars(buf, buf + sizeof buf - 2, (double)v, std::chars_format::scientific, 9);
+    *r.ptr = 'f';
+    r.ptr[1] = '\0';
     return buf;
+}
+
+// locale-independent, correctly rounded number reads
+template <typename T>
+const char *read_number(const char *p, const char *end, T &v)
+{
+    if (p < end && *p == '+') ++p;  // from_chars takes no leading '+'
+    const std::from_chars_result r = std::from_chars(p, end, v, std::chars_format::general);
+    return r.ec == std::errc() ? r.ptr : nullptr;
 }
 
 // the header text with comments removed (// to end of line, /* ... */)
@@ -117,11 +132,11 @@ size_t find_ident(const std::string &s, const char *name)
 
 bool read_define(const std::string &s, const char *name, long &v)
 {
-    const size_t p = find_ident(s, name);
+    size_t p = find_ident(s, name);
     if (p == std::string::npos) return false;
-    char *end = nullptr;
-    v = std::strtol(s.c_str() + p, &end, 10);
-    return end != s.c_str() + p;
+    while (p < s.size() && std::isspace((unsigned char)s[p])) ++p;
+    const std::from_chars_result r = std::from_chars(s.data() + p, s.data() + s.size(), v, 10);
+    return r.ec == std::errc();
 }
 
 // "NAME = value;" -> value
@@ -131,14 +146,14 @@ bool read_scalar(const std::string &s, const char *name, double &v)
     if (p == std::string::npos) return false;
     p = s.find('=', p);
     if (p == std::string::npos) return false;
-    char *end = nullptr;
-    v = std::strtod(s.c_str() + p + 1, &end);
-    return end != s.c_str() + p + 1;
+    ++p;
+    while (p < s.size() && std::isspace((unsigned char)s[p])) ++p;
+    return read_number(s.data() + p, s.data() + s.size(), v) != nullptr;
 }
 
 // the numbers between the braces of "NAME[...]... = { ... };" in order; for
 // T = float each is read as a compiler reads an f-suffixed literal (one
-// rounding from the decimal text)
+// rounding from the decimal text to float)
 template <typename T>
 bool read_array(const std::string &s, const char *name, std::vector<T> &out)
 {
@@ -158,11 +173,9 @@ bool read_array(const std::string &s, const char *name, std::vector<T> &out)
         } else if (ch == '}') {
             if (--depth == 0) return true;
         } else if (std::isdigit((unsigned char)ch) || ch == '-' || ch == '+' || ch == '.') {
-            char *end = nullptr;
             T v;
-            if constexpr (sizeof(T) == 4) v = std::strtof(c + i, &end);
-            else v = std::strtod(c + i, &end);
-            if (end == c + i) return false;
+            const char *end = read_number(c + i, c + s.size(), v);
+            if (!end) return false;
             out.push_back(v);
             i = (size_t)(end - c);
             if (i < s.size() && (c[i] == 'f' || c[i] == 'F' || c[i] == 'l' || c[i] == 'L')) ++i;
@@ -232,7 +245,7 @@ int iirx_filter_taps_read(const char *path, iirx_filter_taps *t)
     // the cutoff is only in the design comment ("cutoff <wn> x Nyquist")
     double wn = 0.0;
     const size_t cp = src.find("cutoff ");
-    if (cp != std::string::npos) wn = std::strtod(src.c_str() + cp + 7, nullptr);
+    if (cp != std::string::npos && !read_number(src.data() + cp + 7, src.data() + src.size(), wn)) wn = 0.0;
 
     const std::string s = strip_comments(src);
     long ns = 0, k = 0;
