@@ -179,6 +179,8 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
         uint64_t *rec = rec2(a.lb, chunk, sec, a.ns);
         const float pside = readlane_f(yv, L - 1);
         if (tx == 0) rec_store(rec + ty, part_flag, pside);
+        // the per-lane terminal coefficients: their load is in flight during the lookback
+        const float4 tm = *reinterpret_cast<const float4 *>(cv + Y.kTerm + 4 * tx);
         if (ty == 0) {
             const float m0 = c[Y.kM + 0], m1 = c[Y.kM + 1], m2 = c[Y.kM + 2], m3 = c[Y.kM + 3];
             const int depth = (int)c[Y.kDepth];
@@ -196,7 +198,6 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
 
         // terminal correction and this side's word of the full record (dtcr:534-553)
         {
-            const float4 tm = *reinterpret_cast<const float4 *>(cv + Y.kTerm + 4 * tx);
             const float c2 = (ty == 0) ? tm.x : tm.z;  // he2 | hb2
             const float c1 = (ty == 0) ? tm.y : tm.w;  // he1 | hb1
             reg[HALF - 1] = fmaf(-c1, X1, fmaf(-c2, X0, yv));

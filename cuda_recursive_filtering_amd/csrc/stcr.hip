@@ -169,13 +169,14 @@ __global__ __launch_bounds__(64) void stcr_kernel(LaunchArgs a)
         const int depth = (int)c[Y.kDepth];
         // the chain starts from the streamed-in state of this section (zero by default)
         const float z2 = a.zi ? a.zi[2 + 2 * sec] : 0.0f, z1 = a.zi ? a.zi[3 + 2 * sec] : 0.0f;
+        // the per-lane terminal coefficients: their load is in flight during the lookback
+        const float4 tm = *reinterpret_cast<const float4 *>(cv + Y.kTerm + 4 * lane);
         const Carry2 Xc = lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
                                    a.spin_limit, a.err, 0u, z2, z1);
         const float X0 = Xc.y2, X1 = Xc.y1;
 
         // terminal correction (stcr:480-491)
         {
-            const float4 tm = *reinterpret_cast<const float4 *>(cv + Y.kTerm + 4 * lane);
             r[N - 2] = fmaf(-tm.y, X1, fmaf(-tm.x, X0, ye));
             r[N - 1] = fmaf(-tm.w, X1, fmaf(-tm.z, X0, yo));
         }
