@@ -272,20 +272,25 @@ class Plan:
 
     __call__ = execute
 
+    def state_size(self):
+        """Floats in a streaming state: 2 + 2 * n_sections (PH, STCR, DTCR) or
+        4 * n_sections (PLR: the order-K direct-form history, K = 2 n_sections)."""
+        return 4 * self.n_sections if self.kernel in ("PLR", KERNELS["PLR"]) else 2 + 2 * self.n_sections
+
     def execute_state(self, d_in, d_out, zi=None, zf=None, stream=None):
-        """PH, STCR and DTCR plans: one launch with the filter state carried in and out (a long
-        signal filtered in calls of n_samples).  zi / zf: device float32
-        tensors (or pointers) of 2 + 2 * n_sections values: x[-2], x[-1] of the
-        input, then y_s[-2], y_s[-1] of every section's output; None = zero
-        state in / no state out."""
+        """One launch with the filter state carried in and out (a long signal
+        filtered in calls of n_samples).  zi / zf: device float32 tensors (or
+        pointers) of state_size() values, oldest first: PH, STCR, DTCR x[-2],
+        x[-1] of the input, then y_s[-2], y_s[-1] of every section's output;
+        PLR x[-K..-1], then y[-K..-1].  None = zero state in / no state out."""
         if self._h is None:
             raise RuntimeError("plan is closed")
         for t in (d_in, d_out):
             if hasattr(t, "numel") and t.numel() != self.n_samples:
                 raise ValueError(f"tensor has {t.numel()} samples, plan expects {self.n_samples}")
         for t in (zi, zf):
-            if t is not None and hasattr(t, "numel") and t.numel() < 2 + 2 * self.n_sections:
-                raise ValueError(f"state needs {2 + 2 * self.n_sections} values")
+            if t is not None and hasattr(t, "numel") and t.numel() < self.state_size():
+                raise ValueError(f"state needs {self.state_size()} values")
         check(lib().iirx_execute_state(self._h, _ptr(d_in), _ptr(d_out), None if zi is None else _ptr(zi),
                                        None if zf is None else _ptr(zf), _stream_ptr(stream)), "execute_state")
 

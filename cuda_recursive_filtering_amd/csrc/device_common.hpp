@@ -313,6 +313,8 @@ __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int se
 // `lp` is LDS scratch of SLOTS*K floats: at most SLOTS partials are folded;
 // an unbounded lookback (depth 0) or one deeper than SLOTS takes a full
 // record in its last slot.  Returns X_j on lane j < K (0 elsewhere).
+// The chain starts at chunk `first` from the state `start[j]` (a full record's
+// words, device memory; nullptr = zero state).
 // The predecessor record's first poll, issued early so that its round trip
 // overlaps independent work (lanes [0, 2K), 0 elsewhere and for chunk 0).
 template <int K>
@@ -326,7 +328,7 @@ template <int K, int SLOTS = kWave>
 __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint32_t chunk, uint32_t part_flag,
                                           uint32_t full_flag, const float *F, int depth, float *lp,
                                           uint32_t spin_limit, uint32_t *err, const uint64_t *first_poll = nullptr,
-                                          uint32_t first = 0)
+                                          uint32_t first = 0, const float *start = nullptr)
 {
     static_assert(2 * K <= kWave, "a record's two halves must fit on one wave");
     const int lane = threadIdx.x & (kWave - 1);
@@ -339,7 +341,8 @@ __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint
     for (int d = 0; d < maxd && !done; ++d) {
         const long pred = (long)chunk - 1 - d;
         if (pred < (long)first) {
-            D = d;  // signal (or channel) start: zero state
+            D = d;  // signal (or channel) start: the start state, zero by default
+            if (start && lane < K) Xj = start[lane];
             done = true;
             break;
         }

@@ -418,16 +418,24 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
     case IIRX_KERNEL_BLOCKSTATE: e = launch_bs(p->n_blocks, p->cfg.block_size, p->cfg.n_sections, a, s); break;
     default: break;
     }
-    if (e == hipSuccess && d_zf) e = launch_ph_end_state(a, (int)p->chunk, d_zf, s);
+    if (e == hipSuccess && d_zf)
+        e = (p->cfg.kernel == IIRX_KERNEL_PLR) ? launch_plr_end_state(a, d_zf, s)
+                                               : launch_ph_end_state(a, (int)p->chunk, d_zf, s);
     if (e != hipSuccess) return IIRX_ERR_HIP;
     p->launches++;
     return IIRX_OK;
 }
 
+// 16-byte accesses need 16-byte aligned buffers; PLR moves one float at a time
+static bool io_aligned(const iirx_plan *p, const float *d_in, const float *d_out)
+{
+    return p->cfg.kernel == IIRX_KERNEL_PLR || !(((uintptr_t)d_in & 15) || ((uintptr_t)d_out & 15));
+}
+
 int iirx_execute(iirx_plan *p, const float *d_in, float *d_out, void *stream)
 {
     if (!p || !d_in || !d_out) return IIRX_ERR_INVALID;
-    if (((uintptr_t)d_in & 15) || ((uintptr_t)d_out & 15)) return IIRX_ERR_INVALID;
+    if (!io_aligned(p, d_in, d_out)) return IIRX_ERR_INVALID;
     const uintptr_t ib = (uintptr_t)d_in, ie = ib + (uintptr_t)p->n_samples * 4;
     const uintptr_t ob = (uintptr_t)d_out, oe = ob + (uintptr_t)p->n_samples * 4;
     if (ib < oe && ob < ie) return IIRX_ERR_INVALID;  // overlap
@@ -437,15 +445,16 @@ int iirx_execute(iirx_plan *p, const float *d_in, float *d_out, void *stream)
 int iirx_execute_state(iirx_plan *p, const float *d_in, float *d_out, const float *d_zi, float *d_zf, void *stream)
 {
     if (!p || !d_in || !d_out) return IIRX_ERR_INVALID;
-    // streaming state: the cascades whose chunk carry is the direct-form output history
-    if (p->cfg.kernel != IIRX_KERNEL_PH && p->cfg.kernel != IIRX_KERNEL_STCR && p->cfg.kernel != IIRX_KERNEL_DTCR)
-        return IIRX_ERR_INVALID;
-    if (((uintptr_t)d_in & 15) || ((uintptr_t)d_out & 15)) return IIRX_ERR_INVALID;
+    // streaming state: the kernels whose chunk carry is a direct-form output
+    // history (per section for the cascades, of the whole recurrence for PLR)
+    if (p->cfg.kernel == IIRX_KERNEL_BLOCKSTATE) return IIRX_ERR_INVALID;
+    const int state_floats = (p->cfg.kernel == IIRX_KERNEL_PLR) ? 4 * p->cfg.n_sections : 2 + 2 * p->cfg.n_sections;
+    if (!io_aligned(p, d_in, d_out)) return IIRX_ERR_INVALID;
     const uintptr_t ib = (uintptr_t)d_in, ie = ib + (uintptr_t)p->n_samples * 4;
     const uintptr_t ob = (uintptr_t)d_out, oe = ob + (uintptr_t)p->n_samples * 4;
     if (ib < oe && ob < ie) return IIRX_ERR_INVALID;  // overlap
     if (d_zf) {
-        const uintptr_t zb = (uintptr_t)d_zf, ze = zb + (uintptr_t)(2 + 2 * p->cfg.n_sections) * 4;
+        const uintptr_t zb = (uintptr_t)d_zf, ze = zb + (uintptr_t)state_floats * 4;
         if ((zb < ie && ib < ze) || (zb < oe && ob < ze)) return IIRX_ERR_INVALID;
     }
     return launch(p, d_in, d_out, (hipStream_t)stream, d_zi, d_zf);

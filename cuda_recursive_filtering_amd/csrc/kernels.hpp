@@ -294,6 +294,8 @@ hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s);
 // end state of a PH launch in the layout of LaunchArgs::zi, from the last
 // chunk's full records and the last two input samples
 hipError_t launch_ph_end_state(const LaunchArgs &a, int chunk_samples, float *zf, hipStream_t s);
+// end state of a PLR launch: the direct-form history x[n-K..n-1], y[n-K..n-1]
+hipError_t launch_plr_end_state(const LaunchArgs &a, float *zf, hipStream_t s);
 
 // one block-state table built on the GPU in FP64 (bs_setup.hip): the
 // sections of one cascade and where its table goes

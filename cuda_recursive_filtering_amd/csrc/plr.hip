@@ -73,8 +73,9 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
         xs[K + t + v * B] = val[v];
     }
     if (t < K) {
+        // input history; before sample 0 the streamed-in x[-K..-1] (zero by default)
         const long hi = cs - K + t;
-        xs[t] = (hi >= 0) ? a.in[hi] : 0.0f;
+        xs[t] = (hi >= 0) ? a.in[hi] : (a.zi ? a.zi[K + hi] : 0.0f);
     }
     __syncthreads();
 
@@ -166,14 +167,14 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
     float Xv[K];  // the chunk carry, uniform
     if constexpr (B == 64) {
         const float Xj = lookbackK<K>(a.lb, 2 * K, chunk, part_flag, full_flag, tab + Y.kF, (int)ctab[Y.kDepth], lp,
-                                      a.spin_limit, a.err);
+                                      a.spin_limit, a.err, nullptr, 0u, a.zi ? a.zi + K : nullptr);
 #pragma unroll
         for (int m = 0; m < K; ++m) Xv[m] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Xj), m));
     } else {
         __syncthreads();  // everyone is done with sv
         if (w == 0) {
             const float Xj = lookbackK<K>(a.lb, 2 * K, chunk, part_flag, full_flag, tab + Y.kF, (int)ctab[Y.kDepth],
-                                          lp, a.spin_limit, a.err);
+                                          lp, a.spin_limit, a.err, nullptr, 0u, a.zi ? a.zi + K : nullptr);
             if (lane < K) sv[lane] = Xj;
         }
         __syncthreads();
@@ -196,6 +197,30 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
         if (idx < a.n_samples) a.out[idx] = val[v];
     }
     });
+}
+
+// zf = the direct-form history after the call: x[n-K..n-1], then
+// y[n-K..n-1] (from zi where the call is shorter than K).  Every value is read
+// before any is written, so zf may be the zi of this call.
+__global__ __launch_bounds__(64) void plr_end_state_kernel(LaunchArgs a, float *zf)
+{
+    const int t = threadIdx.x, K = a.ns;
+    float v = 0.0f;
+    if (t < 2 * K) {
+        const int m = t % K;
+        const long j = a.n_samples - K + m;  // sample index, < 0: history before the call
+        const float *src = (t < K) ? a.in : a.out;
+        if (j >= 0) v = src[j];
+        else if (a.zi) v = a.zi[(t < K ? 0 : K) + K + j];
+    }
+    __syncthreads();
+    if (t < 2 * K) zf[t] = v;
+}
+
+hipError_t launch_plr_end_state(const LaunchArgs &a, float *zf, hipStream_t s)
+{
+    hipLaunchKernelGGL(plr_end_state_kernel, dim3(1), dim3(64), 0, s, a, zf);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

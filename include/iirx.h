@@ -14,7 +14,8 @@
  * b0 == 1 and a0 == 1; the removed cascade gain is applied to the input by
  * the caller.  Each section computes
  *     y[n] = u[n] + b1 u[n-1] + b2 u[n-2] - a1 y[n-1] - a2 y[n-2]
- * with zero initial state, float32 in and float32 out.
+ * with zero initial state (iirx_execute_state carries a state in and out),
+ * float32 in and float32 out.
  *
  * All device pointers are plain device addresses (hipMalloc or any
  * allocator of the same device); `stream` is a hipStream_t passed as void*
@@ -68,8 +69,9 @@ typedef enum {
     IIRX_KERNEL_BLOCKSTATE = 4  /* no counterpart in the reference: the whole
                                    cascade as a state-space block filter on the
                                    matrix cores (FP16 hi/lo operands for the
-                                   Toeplitz and input products, FP32 for the
-                                   tile-state scan); n_blocks = tile length L
+                                   Toeplitz and input products, FP32 operands
+                                   with FP64 accumulation for the tile-state
+                                   scan); n_blocks = tile length L
                                    (32, 64 or 128), block_size = tiles per
                                    chunk G (16, 32 or 64; 16 or 32 at L =
                                    128), chunk = G * L samples */
@@ -164,20 +166,25 @@ int iirx_plan_set_plr_taps(iirx_plan *plan, const double *b, const double *a,
  * counter under IIRX_SCHEDULE_TICKET), so the caller passes none and a
  * captured graph replays correctly.  Calls on one plan must
  * be ordered (one stream, or explicit dependencies); distinct plans are
- * independent.
+ * independent.  d_in and d_out must be 16-byte aligned (INVALID otherwise),
+ * except for PLR plans, which move one float at a time.
  */
 int iirx_execute(iirx_plan *plan, const float *d_in, float *d_out,
                  void *stream);
 
-/* Streaming state (PH, STCR and DTCR plans): iirx_execute with the filter state carried in
- * and out, so a long signal can be filtered in calls of n_samples each.  The
- * reference has only zero-state hooks (ph_kernels.cuh:45, 162-165).  A state
- * is 2 + 2 N_SECTIONS floats in device memory: x[-2], x[-1] of the input,
- * then y_s[-2], y_s[-1] of the output of every section s (direct-form
- * histories).  d_zi NULL: zero state; d_zf NULL: not written.  d_zi is read
- * by the launch and d_zf written by a small kernel after it, both on
- * `stream`; d_zf may equal d_zi of the next call.  PLR and the block-state
- * form: INVALID. */
+/* Streaming state: iirx_execute with the filter state carried in and out, so
+ * a long signal can be filtered in calls of n_samples each.  The reference
+ * has only zero-state hooks (ph_kernels.cuh:45, 162-165; plr_kernels.cuh:
+ * 134-142).  States are direct-form histories in device memory, oldest first:
+ *   PH, STCR, DTCR: 2 + 2 N_SECTIONS floats, x[-2], x[-1] of the input, then
+ *     y_s[-2], y_s[-1] of the output of every section s;
+ *   PLR: 2 K floats (K = 2 N_SECTIONS, the recurrence order), x[-K..-1] of
+ *     the input, then y[-K..-1] of the output.
+ * d_zi NULL: zero state; d_zf NULL: not written.  d_zi is read by the launch
+ * and d_zf written by a small kernel after it, both on `stream`; d_zf may
+ * equal d_zi (of this call or the next).  The block-state form: INVALID (its
+ * carry is the balanced state vector, from which the sections' histories do
+ * not follow). */
 int iirx_execute_state(iirx_plan *plan, const float *d_in, float *d_out,
                        const float *d_zi, float *d_zf, void *stream);
 

@@ -1,7 +1,7 @@
-"""Streaming state (iirx_execute_state, PH plans): a long signal filtered in
-calls, the state of the cascade carried from one call to the next.  No
-counterpart in the reference, whose initial-state hooks are always zero
-(ph_kernels.cuh:45, 162-165).  Checked by what streaming must preserve:
+"""Streaming state (iirx_execute_state; PH, STCR, DTCR and PLR plans): a long
+signal filtered in calls, the filter state carried from one call to the
+next.  No counterpart in the reference, whose initial-state hooks are always
+zero (ph_kernels.cuh:45, 162-165; plr_kernels.cuh:134-142).  Checked by what streaming must preserve:
 filtering in calls equals filtering the whole signal (against the FP64
 cascade with the reference's gate, and against one call on the whole
 signal), a zero state in equals the plain launch bit for bit, and the state
@@ -113,12 +113,95 @@ def test_stream_state_out_is_the_direct_form_history(device):
 
 
 @pytest.mark.gpu
-def test_stream_state_rejected_by_plr_and_blockstate(device):
+def test_stream_state_rejected_by_blockstate(device):
     import torch
     sos, g = iirx.butterworth_sos(4, 0.1)
     xin = torch.zeros(1 << 16, dtype=torch.float32, device=device)
     out = torch.empty_like(xin)
-    for kernel, kw in (("PLR", {}), ("BS", {"block_size": 64, "n_blocks": 64})):
-        with iirx.Plan(kernel, sos, xin.numel(), **kw) as p:
-            with pytest.raises(iirx.IirxError):
-                p.execute_state(xin, out)
+    with iirx.Plan("BS", sos, xin.numel(), block_size=64, n_blocks=64) as p:
+        with pytest.raises(iirx.IirxError):
+            p.execute_state(xin, out)
+
+
+def _plr_fp64(sos, x, x_hist=None, y_hist=None):
+    """FP64 direct form of the cascade with the history x[-K..-1], y[-K..-1]
+    (scipy's lfiltic turns a history into lfilter's state)."""
+    b, a = iirx.direct_form_taps(sos)
+    bb, aa = np.r_[1.0, b], np.r_[1.0, -a]
+    if x_hist is None:
+        return signal.lfilter(bb, aa, x.astype(np.float64))
+    zi = signal.lfiltic(bb, aa, y=np.asarray(y_hist, np.float64)[::-1], x=np.asarray(x_hist, np.float64)[::-1])
+    return signal.lfilter(bb, aa, x.astype(np.float64), zi=zi)[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [2, 4])
+def test_stream_plr_calls_equal_one(device, order):
+    """PLR in 3 calls of a length that is no multiple of its chunk (nor of 4:
+    the slices are not 16-byte aligned): the output passes the gate against
+    the FP64 cascade, as does one call on the whole signal, and the two agree
+    to the gate's bound."""
+    import torch
+    print(f"case stream PLR order={order}", flush=True)
+    calls, n = 3, 100003
+    sos, g, x = normal_workload(order, calls * n, seed=40 + order, wn=0.1)
+    xin = torch.from_numpy(x).to(device)
+    out = torch.empty_like(xin)
+    whole = torch.empty_like(xin)
+    K = 2 * sos.shape[0]
+    za = torch.zeros(2 * K, dtype=torch.float32, device=device)
+    zb = torch.empty_like(za)
+    with iirx.Plan("PLR", sos, n) as p:
+        assert p.state_size() == 2 * K
+        for c in range(calls):
+            p.execute_state(xin[c * n:(c + 1) * n], out[c * n:(c + 1) * n], zi=za, zf=zb)
+            za, zb = zb, za
+        p.check()
+    with iirx.Plan("PLR", sos, calls * n) as p:
+        p.execute(xin, whole)
+        p.check()
+    y, yw = out.cpu().numpy(), whole.cpu().numpy()
+    ref = O.golden(sos, x)
+    assert O.gate(y, ref)[0] and O.gate(yw, ref)[0]
+    # the float32 direct form's own error (about 5e-5 of the peak at order 4)
+    # differs between the two geometries; both stay within the gate
+    assert np.max(np.abs(y - yw)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
+def test_stream_plr_history_in_and_out(device):
+    """A non-zero history in (x[-K..-1], y[-K..-1]) against scipy's lfilter
+    with lfiltic's state; the history out is the call's last K inputs and
+    outputs bit for bit; a call shorter than K takes the rest from the history
+    in, also when the state is updated in place (zf = zi)."""
+    import torch
+    print("case stream PLR history", flush=True)
+    sos, g = iirx.butterworth_sos(4, 0.1)
+    K = 4
+    rng = np.random.default_rng(44)
+    x = (g * rng.standard_normal(50000)).astype(np.float32)
+    x_hist = (g * rng.standard_normal(K)).astype(np.float32)
+    y_hist = (0.5 * rng.standard_normal(K)).astype(np.float32)
+    xin = torch.from_numpy(x).to(device)
+    out = torch.empty_like(xin)
+    zi = torch.from_numpy(np.r_[x_hist, y_hist].astype(np.float32)).to(device)
+    zf = torch.empty_like(zi)
+    with iirx.Plan("PLR", sos, x.size) as p:
+        p.execute_state(xin, out, zi=zi, zf=zf)
+        p.check()
+    y = out.cpu().numpy()
+    ref = _plr_fp64(sos, x, x_hist, y_hist)
+    assert np.max(np.abs(y - ref)) <= 1e-4 * np.max(np.abs(ref))
+    z = zf.cpu().numpy()
+    assert np.array_equal(z[:K], x[-K:]) and np.array_equal(z[K:], y[-K:])
+    # a call of 3 < K samples, state updated in place
+    xs = torch.from_numpy(x[:3].copy()).to(device)
+    ys = torch.empty_like(xs)
+    zs = torch.from_numpy(np.r_[x_hist, y_hist].astype(np.float32)).to(device)
+    with iirx.Plan("PLR", sos, 3) as p:
+        p.execute_state(xs, ys, zi=zs, zf=zs)
+        p.check()
+    ys, zs = ys.cpu().numpy(), zs.cpu().numpy()
+    assert np.max(np.abs(ys - ref[:3])) <= 1e-4 * np.max(np.abs(ref))
+    assert np.array_equal(zs[:K], np.r_[x_hist[3:], x[:3]])
+    assert np.array_equal(zs[K:], np.r_[y_hist[3:], ys])
