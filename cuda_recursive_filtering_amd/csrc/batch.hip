@@ -290,18 +290,10 @@ void batch_ph_kernel(BatchArgs a)
             }
         }
 
-        // lane scan of the end states with C^(2^k)
+        // lane scan of the end states (as ph.hip: DPP row scan, then the
+        // rows' carries from lanes 15, 31, 47)
         float s2 = x[N - 2], s1 = x[N - 1];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const int d = 1 << k;
-            const cfloat *P = c + Y.kCpow + 4 * k;
-            const float t2 = lane_up(s2, d), t1 = lane_up(s1, d);
-            if (lane >= d) {
-                s2 = fmaf(P[1], t1, fmaf(P[0], t2, s2));
-                s1 = fmaf(P[3], t1, fmaf(P[2], t2, s1));
-            }
-        }
+        ph_lane_scan(c + Y.kCpow, cv + Y.kCfin, lane, s2, s1);
 
         // chunk partial, lookback within the channel, chunk full
         const float m0 = c[Y.kM + 0], m1 = c[Y.kM + 1], m2 = c[Y.kM + 2], m3 = c[Y.kM + 3];

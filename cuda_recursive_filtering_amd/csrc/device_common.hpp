@@ -73,6 +73,15 @@ __device__ __forceinline__ int readlane_i(int v, int lane)
 // value of lane (l - d) for l >= d, own value otherwise (CUDA shfl_up rule)
 __device__ __forceinline__ float lane_up(float v, int d) { return __shfl_up(v, (unsigned)d, kWave); }
 
+// value of lane l - D inside the same row of 16 lanes (DPP row_shr:D), 0 for
+// the first D lanes of every row; D in 1..15
+template <int D>
+__device__ __forceinline__ float row_shr_zero(float v)
+{
+    static_assert(D >= 1 && D <= 15, "row_shr takes 1..15");
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + D, 0xF, 0xF, false));
+}
+
 // ------------------------------------------------------------------------
 // compile-time loops: every index a literal (the "hand-unrolled" forms)
 // ------------------------------------------------------------------------
@@ -87,6 +96,37 @@ __device__ __forceinline__ void static_for(F &&f)
         f(ic<B>{});
         static_for<B + S, E, S>(f);
     }
+}
+
+// Inclusive scan over the wave's 64 lanes of 2-vectors s with one uniform
+// 2x2 operator C (row-major): lane l ends with sum_{i<=l} C^(l-i) s_i.
+// Cpow: C^(2^k), k = 0..4 (uniform, 4 floats each); Cfin: C^(t+1) for
+// t = 0..15 on 16-byte boundaries (per lane).  Inside each row of 16 lanes a
+// Kogge-Stone scan over DPP row shifts (no LDS round trip), then the rows'
+// carries K_r = S(16 r - 1) + C^16 K_(r-1), from lanes 15, 31 and 47, enter
+// lane j of row r as C^(j+1) K_r.  The PH lane scan (ph_kernels.cuh:289-382)
+// with the same operators in another order.
+__device__ __forceinline__ void ph_lane_scan(const cfloat *Cpow, const float *Cfin, int lane, float &s2, float &s1)
+{
+    const float4 cj = *reinterpret_cast<const float4 *>(Cfin + 4 * (lane & 15));
+    static_for<0, 4, 1>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        const auto *P = Cpow + 4 * k;
+        const float t2 = row_shr_zero<1 << k>(s2), t1 = row_shr_zero<1 << k>(s1);
+        s2 = fmaf(P[1], t1, fmaf(P[0], t2, s2));
+        s1 = fmaf(P[3], t1, fmaf(P[2], t2, s1));
+    });
+    const auto *P = Cpow + 16;  // C^16
+    const float a2 = readlane_f(s2, 15), a1 = readlane_f(s1, 15);
+    const float b2 = fmaf(P[1], a1, fmaf(P[0], a2, readlane_f(s2, 31)));
+    const float b1 = fmaf(P[3], a1, fmaf(P[2], a2, readlane_f(s1, 31)));
+    const float e2 = fmaf(P[1], b1, fmaf(P[0], b2, readlane_f(s2, 47)));
+    const float e1 = fmaf(P[3], b1, fmaf(P[2], b2, readlane_f(s1, 47)));
+    const int row = lane >> 4;
+    const float k2 = row == 1 ? a2 : row == 2 ? b2 : row == 3 ? e2 : 0.0f;
+    const float k1 = row == 1 ? a1 : row == 2 ? b1 : row == 3 ? e1 : 0.0f;
+    s2 = fmaf(cj.y, k1, fmaf(cj.x, k2, s2));
+    s1 = fmaf(cj.w, k1, fmaf(cj.z, k2, s1));
 }
 
 // descending: f(ic<I>) for I = B, B-1, ... > E

@@ -77,18 +77,9 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
             }
         }
 
-        // lane scan of the end states with C^(2^k)
+        // lane scan of the end states
         float s2 = r[N - 2], s1 = r[N - 1];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const int d = 1 << k;
-            const cfloat *P = c + Y.kCpow + 4 * k;
-            const float t2 = lane_up(s2, d), t1 = lane_up(s1, d);
-            if (lane >= d) {
-                s2 = fmaf(P[1], t1, fmaf(P[0], t2, s2));
-                s1 = fmaf(P[3], t1, fmaf(P[2], t2, s1));
-            }
-        }
+        ph_lane_scan(c + Y.kCpow, cv + Y.kCfin, lane, s2, s1);
 
         // chunk partial, lookback, chunk full (ph_kernels.cuh:384-469)
         const float m0 = c[Y.kM + 0], m1 = c[Y.kM + 1], m2 = c[Y.kM + 2], m3 = c[Y.kM + 3];
