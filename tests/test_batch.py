@@ -193,8 +193,9 @@ def test_batch_16_byte_staging_any_offset(device, algo):
     """Full chunks of 16-byte aligned buffers move with 16-byte accesses
     shifted by the chunk's offset mod 4; buffers that are not 16-byte aligned
     take 4-byte accesses.  Channels start at every offset mod 4 and span
-    several full chunks; both paths give the same output bit for bit, and
-    nothing outside the channels is written."""
+    several full chunks; both paths pass the gate per channel and agree to
+    8 float32 ulps of each channel's peak, and nothing outside the channels
+    is written."""
     import torch
     print(f"case batch 16-byte staging {algo}", flush=True)
     lengths = [3 * 2048 + 1, 2 * 2048 + 2, 4 * 2048 + 3, 2048 + 7, 5 * 2048, 2 * 2048 + 5, 3 * 2048 + 6, 4096]
@@ -214,8 +215,14 @@ def test_batch_16_byte_staging_any_offset(device, algo):
         o = out.cpu().numpy()
         assert np.all(o[:4 + shift] == 7.0) and np.all(o[4 + shift + n:] == 7.0), "write outside the channels"
         outs[shift] = o[4 + shift:4 + shift + n]
-    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    # the same arithmetic per chunk; a carry taken from a full record or from
+    # folded partials (timing) may differ in the last bits at lookback depth >= 2
+    for c in range(len(lengths)):
+        a, e = int(offsets[c]), int(offsets[c + 1])
+        peak = float(np.max(np.abs(outs[0][a:e])))
+        assert np.max(np.abs(outs[0][a:e] - outs[1][a:e])) <= 8 * np.finfo(np.float32).eps * peak, c
     _check_channels(outs[0], x, offsets, sections)
+    _check_channels(outs[1], x, offsets, sections)
 
 
 @pytest.mark.gpu
