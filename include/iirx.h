@@ -15,7 +15,11 @@
  * the caller.  Each section computes
  *     y[n] = u[n] + b1 u[n-1] + b2 u[n-2] - a1 y[n-1] - a2 y[n-2]
  * with zero initial state (iirx_execute_state carries a state in and out),
- * float32 in and float32 out.
+ * float32 in and float32 out.  A non-finite input sample at k makes the
+ * output non-finite from k on (the block-state form: from the start of k's
+ * 32-sample slab) until the bounded lookback drops the carry that holds it,
+ * a few chunks later; after that the output is that of the signal with
+ * x[k] = 0 (sequential filtering would stay non-finite).
  *
  * All device pointers are plain device addresses (hipMalloc or any
  * allocator of the same device); `stream` is a hipStream_t passed as void*
