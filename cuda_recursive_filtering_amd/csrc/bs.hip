@@ -139,8 +139,10 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G), bs_waves(L, G)))) void bs_ker
         auto store = [&](int nb, int mb, const f4 &y) {
             *reinterpret_cast<f4 *>(out + (long)(16 * nb + col) * L + 16 * mb + 4 * g) = y;
         };
+        // streaming: a.zi is the balanced state before the signal, a.tail
+        // receives the state entering the last chunk's last tile
         bs_chunk<L, G, NSL>(xr, chunk, part_flag, full_flag, a.lb, tab + Y.kMc, depth, lp, a.spin_limit, a.err, 0u,
-                            sT, sG, ops, store);
+                            sT, sG, ops, store, a.zi, chunk + 1u == a.num_chunks ? a.tail : nullptr);
     };
 
     const uint32_t GW = gridDim.x * W, gw = blockIdx.x * W + wave;  // this wave among the grid's
@@ -183,6 +185,80 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G), bs_waves(L, G)))) void bs_ker
         read_chunk(cur, true);
         process(chunk, pf, pf + 1u, cur);
     }
+}
+
+// ---------------------------------------------------------------------------
+// streaming state.  The realization's state c is the direct-form-II-
+// transposed state of section c / 2 divided by d[c]: after sample n,
+// w1 = b1 u[n] + b2 u[n-1] - a1 y[n] - a2 y[n-1] and w2 = b2 u[n] - a2 y[n]
+// with u the section's input and y its output.  One thread each: the maps
+// run once per call.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void bs_state_in_kernel(const float *zi, const BsStateMap *m, float *z0)
+{
+    if (threadIdx.x != 0) return;
+    float z[16];
+    for (int c = 0; c < 16; ++c) z[c] = 0.0f;
+    double u2 = zi[0], u1 = zi[1];  // x[-2], x[-1]
+    for (int s = 0; s < m->ns; ++s) {
+        const double b1 = m->c[s][0], b2 = m->c[s][1], a1 = m->c[s][2], a2 = m->c[s][3];
+        const double y2 = zi[2 + 2 * s], y1 = zi[3 + 2 * s];
+        const double w1 = b1 * u1 + b2 * u2 - a1 * y1 - a2 * y2;
+        const double w2 = b2 * u1 - a2 * y1;
+        z[2 * s] = (float)(w1 / m->d[2 * s]);
+        z[2 * s + 1] = (float)(w2 / m->d[2 * s + 1]);
+        u2 = y2;  // the next section's input is this section's output
+        u1 = y1;
+    }
+    for (int c = 0; c < 16; ++c) z0[c] = z[c];
+}
+
+// zf = x[n-2], x[n-1], then y_s[n-2], y_s[n-1] of every section: the state
+// entering the last tile (after sample n - L - 1) mapped back to w and the
+// tile's L inputs run through the sections in FP64 (y = u + w1,
+// w1 <- -a1 w1 + w2 + (b1 - a1) u, w2 <- -a2 w1 + (b2 - a2) u).  Everything
+// is read before zf is written, so zf may be the zi of this call.
+__global__ __launch_bounds__(64) void bs_state_out_kernel(const float *in, int64_t n, int L, const float *tail,
+                                                           const BsStateMap *m, float *zf)
+{
+    if (threadIdx.x != 0) return;
+    const int ns = m->ns;
+    double w[16];
+    for (int c = 0; c < 2 * ns; ++c) w[c] = (double)tail[c] * m->d[c];
+    double hist[8][2] = {};
+    for (int64_t t = n - L; t < n; ++t) {
+        double u = in[t];
+        for (int s = 0; s < ns; ++s) {
+            const double b1 = m->c[s][0], b2 = m->c[s][1], a1 = m->c[s][2], a2 = m->c[s][3];
+            const double y = u + w[2 * s];
+            const double w1 = -a1 * w[2 * s] + w[2 * s + 1] + (b1 - a1) * u;
+            const double w2 = -a2 * w[2 * s] + (b2 - a2) * u;
+            w[2 * s] = w1;
+            w[2 * s + 1] = w2;
+            if (t >= n - 2) hist[s][t - (n - 2)] = y;
+            u = y;
+        }
+    }
+    const float x2 = in[n - 2], x1 = in[n - 1];
+    zf[0] = x2;
+    zf[1] = x1;
+    for (int s = 0; s < ns; ++s) {
+        zf[2 + 2 * s] = (float)hist[s][0];
+        zf[3 + 2 * s] = (float)hist[s][1];
+    }
+}
+
+hipError_t launch_bs_state_in(const float *zi, const BsStateMap *m, float *z0, hipStream_t s)
+{
+    hipLaunchKernelGGL(bs_state_in_kernel, dim3(1), dim3(64), 0, s, zi, m, z0);
+    return hipGetLastError();
+}
+
+hipError_t launch_bs_state_out(const float *in, int64_t n, int L, const float *tail, const BsStateMap *m, float *zf,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(bs_state_out_kernel, dim3(1), dim3(64), 0, s, in, n, L, tail, m, zf);
+    return hipGetLastError();
 }
 
 template <int L, int G, int NSL>

@@ -160,7 +160,8 @@ template <int L, int G, int NSL, class Ops, class Store>
 __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32_t chunk, uint32_t part_flag,
                                          uint32_t full_flag, uint64_t *lb, const float *Mc, int depth, float *lp,
                                          uint32_t spin_limit, uint32_t *err, uint32_t first, int sT, int sG,
-                                         const Ops &ops, const Store &store)
+                                         const Ops &ops, const Store &store, const float *start = nullptr,
+                                         float *tail = nullptr)
 {
     constexpr int MB = L / 16;  // output row blocks of T
     constexpr int KQ = L / 32;  // K slabs of 32 samples
@@ -244,7 +245,7 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
     }
 
     const float Xj = lookbackK<16, kBsSlots>(lb, 32, chunk, part_flag, full_flag, Mc, depth, lp, spin_limit, err,
-                                             &poll0, first);
+                                             &poll0, first, start);
     // the output stage's O blocks, fetched before any store of this chunk:
     // vmcnt retires in issue order, so a load issued behind stores waits for them
     f4 Ob[MB];
@@ -292,6 +293,10 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
     f4 S[NB];
     shift_tiles<1, NSL, NB>(Zf, S);
     if (col == 0) S[0] = xin;
+    if (tail && col == 15) {  // the state entering the chunk's last tile
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tail[g + 4 * j] = (j < NSL) ? S[NB - 1][j] : 0.0f;
+    }
     // a store covers 64 B of each of 16 tiles; the two halves of a 128-byte
     // line (row blocks mb, mb + 1) are issued back to back and through the L2
     // (not nontemporal), so that they merge there: measured WRITE_SIZE 1.06x

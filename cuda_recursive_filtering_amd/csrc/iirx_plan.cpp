@@ -59,6 +59,10 @@ struct iirx_plan {
     uint32_t grid;
     const void *fn;
     int64_t launches;
+    // block-state form, streaming state: the map between the sections'
+    // histories and the balanced state, then 16 floats of incoming state and
+    // 16 of the state entering the last tile (one allocation)
+    BsStateMap *d_bsmap;
 };
 
 namespace {
@@ -288,10 +292,12 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     p->coef_stride = sh.coef_stride;
     const size_t lb_count = sh.lb_count;
 
+    p->d_bsmap = nullptr;
     auto fail = [&](int code) {
         if (p->d_coef) hipFree(p->d_coef);
         if (p->d_lb) hipFree(p->d_lb);
         if (p->d_ticket) hipFree(p->d_ticket);
+        if (p->d_bsmap) hipFree(p->d_bsmap);
         delete p;
         return code;
     };
@@ -313,6 +319,20 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
                         hipDeviceSynchronize() == hipSuccess;
         hipFree(d_job);
         if (!ok) return fail(IIRX_ERR_HIP);
+        // the streaming-state map (FP64 realization of precompute.cpp)
+        BsStateMap map{};
+        std::vector<double> A, B, C;
+        bs_realization(p->sos.data(), cfg->n_sections, A, B, C, map.d);
+        for (int s = 0; s < cfg->n_sections; ++s) {
+            const float *r = p->sos.data() + 6 * s;
+            map.c[s][0] = r[1];
+            map.c[s][1] = r[2];
+            map.c[s][2] = r[4];
+            map.c[s][3] = r[5];
+        }
+        map.ns = cfg->n_sections;
+        if (hipMalloc((void **)&p->d_bsmap, sizeof map + 32 * sizeof(float)) != hipSuccess) return fail(IIRX_ERR_OOM);
+        if (hipMemcpy(p->d_bsmap, &map, sizeof map, hipMemcpyHostToDevice) != hipSuccess) return fail(IIRX_ERR_HIP);
     } else if (hipMemcpy(p->d_coef, p->host_coef.data(), p->host_coef.size() * sizeof(float),
                          hipMemcpyHostToDevice) != hipSuccess) {
         return fail(IIRX_ERR_HIP);
@@ -409,7 +429,22 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
     a.coef_stride = p->coef_stride;
     a.unrolled = p->cfg.unrolled;
     a.zi = d_zi;
-    hipError_t e = hipErrorInvalidValue;
+    a.tail = nullptr;
+    hipError_t e = hipSuccess;
+    float *bs_z0 = nullptr, *bs_tail = nullptr;
+    if (p->cfg.kernel == IIRX_KERNEL_BLOCKSTATE && p->d_bsmap) {
+        // the block-state kernel takes the balanced state; it writes the state
+        // entering its last tile for the histories out
+        bs_z0 = reinterpret_cast<float *>(p->d_bsmap + 1);
+        bs_tail = bs_z0 + 16;
+        if (d_zi) {
+            e = launch_bs_state_in(d_zi, p->d_bsmap, bs_z0, s);
+            a.zi = bs_z0;
+        }
+        if (d_zf) a.tail = bs_tail;
+    }
+    if (e != hipSuccess) return IIRX_ERR_HIP;
+    e = hipErrorInvalidValue;
     switch (p->cfg.kernel) {
     case IIRX_KERNEL_PH: e = launch_ph(p->lanes, p->n_blocks, a, s); break;
     case IIRX_KERNEL_STCR: e = launch_stcr(p->lanes, p->n_blocks, a, s); break;
@@ -418,9 +453,12 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
     case IIRX_KERNEL_BLOCKSTATE: e = launch_bs(p->n_blocks, p->cfg.block_size, p->cfg.n_sections, a, s); break;
     default: break;
     }
-    if (e == hipSuccess && d_zf)
-        e = (p->cfg.kernel == IIRX_KERNEL_PLR) ? launch_plr_end_state(a, d_zf, s)
-                                               : launch_ph_end_state(a, (int)p->chunk, d_zf, s);
+    if (e == hipSuccess && d_zf) {
+        if (p->cfg.kernel == IIRX_KERNEL_PLR) e = launch_plr_end_state(a, d_zf, s);
+        else if (p->cfg.kernel == IIRX_KERNEL_BLOCKSTATE)
+            e = launch_bs_state_out(d_in, p->n_samples, p->cfg.n_blocks, bs_tail, p->d_bsmap, d_zf, s);
+        else e = launch_ph_end_state(a, (int)p->chunk, d_zf, s);
+    }
     if (e != hipSuccess) return IIRX_ERR_HIP;
     p->launches++;
     return IIRX_OK;
@@ -445,9 +483,8 @@ int iirx_execute(iirx_plan *p, const float *d_in, float *d_out, void *stream)
 int iirx_execute_state(iirx_plan *p, const float *d_in, float *d_out, const float *d_zi, float *d_zf, void *stream)
 {
     if (!p || !d_in || !d_out) return IIRX_ERR_INVALID;
-    // streaming state: the kernels whose chunk carry is a direct-form output
-    // history (per section for the cascades, of the whole recurrence for PLR)
-    if (p->cfg.kernel == IIRX_KERNEL_BLOCKSTATE) return IIRX_ERR_INVALID;
+    // streaming state: the sections' direct-form histories (PH, STCR, DTCR,
+    // the block-state form through its state map), PLR's of the whole recurrence
     const int state_floats = (p->cfg.kernel == IIRX_KERNEL_PLR) ? 4 * p->cfg.n_sections : 2 + 2 * p->cfg.n_sections;
     if (!io_aligned(p, d_in, d_out)) return IIRX_ERR_INVALID;
     const uintptr_t ib = (uintptr_t)d_in, ie = ib + (uintptr_t)p->n_samples * 4;
@@ -516,6 +553,7 @@ int iirx_plan_destroy(iirx_plan *p)
     if (p->d_coef) hipFree(p->d_coef);
     if (p->d_lb) hipFree(p->d_lb);
     if (p->d_ticket) hipFree(p->d_ticket);
+    if (p->d_bsmap) hipFree(p->d_bsmap);
     delete p;
     return IIRX_OK;
 }

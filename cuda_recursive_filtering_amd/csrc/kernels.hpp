@@ -25,9 +25,14 @@ struct LaunchArgs {
     int32_t ns;                  // sections (PH/STCR/DTCR) or order k (PLR)
     int32_t coef_stride;
     int32_t unrolled;
-    // PH streaming state (null: zero state): x[-2], x[-1] of the input, then
-    // y_s[-2], y_s[-1] of every section's output (2 + 2 ns floats)
+    // streaming state in (null: zero state).  PH/STCR/DTCR: x[-2], x[-1] of
+    // the input, then y_s[-2], y_s[-1] of every section's output (2 + 2 ns
+    // floats); PLR: x[-K..-1], y[-K..-1]; block-state form: the balanced
+    // state vector (16 floats, natural order)
     const float *zi;
+    // block-state form: where the last chunk writes the state entering its
+    // last tile (16 floats; null: not written)
+    float *tail;
 };
 
 // plan control block: ticket, epoch, watchdog flag, then 64 arrival slots
@@ -296,6 +301,21 @@ hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s);
 hipError_t launch_ph_end_state(const LaunchArgs &a, int chunk_samples, float *zf, hipStream_t s);
 // end state of a PLR launch: the direct-form history x[n-K..n-1], y[n-K..n-1]
 hipError_t launch_plr_end_state(const LaunchArgs &a, float *zf, hipStream_t s);
+
+// Streaming state of the block-state form (bs.hip): the realization's states
+// are the sections' direct-form-II-transposed (w1, w2) divided by d; c holds
+// b1, b2, a1, a2 (scipy sign) per section.
+struct BsStateMap {
+    double d[16];
+    double c[8][4];
+    int32_t ns, pad;
+};
+// the DF1 histories zi (2 + 2 ns floats) as the balanced state z0 (16 floats)
+hipError_t launch_bs_state_in(const float *zi, const BsStateMap *m, float *z0, hipStream_t s);
+// the DF1 histories after sample n-1 from the state entering the last tile of
+// L samples (balanced, 16 floats) and that tile's inputs, re-filtered in FP64
+hipError_t launch_bs_state_out(const float *in, int64_t n, int L, const float *tail, const BsStateMap *m, float *zf,
+                               hipStream_t s);
 
 // one block-state table built on the GPU in FP64 (bs_setup.hip): the
 // sections of one cascade and where its table goes

@@ -299,15 +299,21 @@ void put_f32(float *dst, const double *m, int r0)
 
 }  // namespace
 
-int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out)
+// The block-state realization (FP64): the cascade of direct-form-II-
+// transposed sections, z_s' = A_s z_s + B_s u_s with z_s = (w1, w2) of
+// section s at states 2s, 2s+1, u_s = u_0 + sum_{r<s} z_r[0],
+// y = u_0 + sum_r z_r[0]; then every state divided by d_c = sqrt(W_cc), W
+// the controllability Gramian (balancing).  A is 16 x 16 row-major, B, C
+// and d 16 long (d = 1 past the p = 2 ns states).
+void bs_realization(const float *sos, int ns, std::vector<double> &A, std::vector<double> &B, std::vector<double> &C,
+                    double *d)
 {
-    const BsLayout Y = bs_layout(L);
-    out.assign((size_t)Y.stride, 0.0f);
     const int p = 2 * ns;
+    A.assign(256, 0.0);
+    B.assign(16, 0.0);
+    C.assign(16, 0.0);
     // cascade of direct-form-II-transposed sections: z_s' = A_s z_s + B_s u_s,
     // u_s = u_0 + sum_{r<s} z_r[0], y = u_0 + sum_r z_r[0]
-    Mat A(256, 0.0);
-    std::vector<double> B(16, 0.0), C(16, 0.0);
     for (int s = 0; s < ns; ++s) {
         const float *r = sos + 6 * s;
         const double b1 = r[1], b2 = r[2], a1 = r[4], a2 = r[5];  // scipy sign
@@ -347,7 +353,7 @@ int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out)
             Ak = mat_mul16(Ak, Ak);
             if (!(delta > 1e-18 * size)) break;
         }
-        double dsc[16];
+        double *dsc = d;
         for (int c = 0; c < 16; ++c) {
             const double w = W[c * 16 + c];
             dsc[c] = (c < p && w > 0.0 && std::isfinite(w)) ? std::sqrt(w) : 1.0;
@@ -359,6 +365,17 @@ int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out)
             C[c] *= dsc[c];
         }
     }
+}
+
+int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out)
+{
+    const BsLayout Y = bs_layout(L);
+    out.assign((size_t)Y.stride, 0.0f);
+    const int p = 2 * ns;
+    Mat A;
+    std::vector<double> B, C;
+    double dsc[16];
+    bs_realization(sos, ns, A, B, C, dsc);
     // states stay in order 0..p-1: K slab m of the state products holds states
     // 4 m .. 4 m + 3, so only ceil(p / 4) slabs are nonzero
     // impulse response, observability, controllability

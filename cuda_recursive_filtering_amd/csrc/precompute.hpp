@@ -49,4 +49,11 @@ int plr_tables(const double *b, const double *a, int K, int CH, std::vector<floa
 // Block-state tables (layout kernels.hpp:bs_layout) for tiles of L samples.
 int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out);
 
+// The block-state realization in FP64: A (16 x 16), B, C of the cascade of
+// direct-form-II-transposed sections (state (w1, w2) of section s at 2s,
+// 2s+1), every state divided by the balancing scale d[c] (d = 1 past the
+// 2 ns states).
+void bs_realization(const float *sos, int ns, std::vector<double> &A, std::vector<double> &B, std::vector<double> &C,
+                    double *d);
+
 }  // namespace iirx

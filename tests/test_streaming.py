@@ -112,15 +112,77 @@ def test_stream_state_out_is_the_direct_form_history(device):
         assert abs(z[2 + 2 * s] - u[-2]) <= 1e-5 * scale and abs(z[3 + 2 * s] - u[-1]) <= 1e-5 * scale, s
 
 
+def _sos_fp64_with_history(sos, x, zi):
+    """FP64 cascade from the DF1 histories zi (x[-2], x[-1], then y_s[-2],
+    y_s[-1] per section): lfiltic turns each section's history into
+    sosfilt's state."""
+    ns = sos.shape[0]
+    state = np.zeros((ns, 2))
+    u2, u1 = float(zi[0]), float(zi[1])
+    for s in range(ns):
+        y2, y1 = float(zi[2 + 2 * s]), float(zi[3 + 2 * s])
+        state[s] = signal.lfiltic(sos[s, :3].astype(np.float64), sos[s, 3:].astype(np.float64), y=[y1, y2], x=[u1, u2])
+        u2, u1 = y2, y1
+    return signal.sosfilt(sos.astype(np.float64), x.astype(np.float64), zi=state)[0]
+
+
 @pytest.mark.gpu
-def test_stream_state_rejected_by_blockstate(device):
+@pytest.mark.parametrize("order", [8, 16])
+def test_stream_blockstate_calls(device, order):
+    """The block-state form in 4 calls: the histories go in through the
+    balanced state map and come out by re-filtering the last tile in FP64;
+    the output passes the gate and matches one call on the whole signal."""
     import torch
-    sos, g = iirx.butterworth_sos(4, 0.1)
-    xin = torch.zeros(1 << 16, dtype=torch.float32, device=device)
+    print(f"case stream block-state order={order}", flush=True)
+    sos, g, x = normal_workload(order, 1 << 20, seed=60 + order, wn=0.1)
+    y, _ = _stream(device, sos, x, 4, kernel="BS", geom=(64, 64))
+    ref = O.golden(sos, x)
+    assert O.gate(y, ref)[0]
+    xin = torch.from_numpy(x).to(device)
+    whole = torch.empty_like(xin)
+    with iirx.Plan("BS", sos, x.size, block_size=64, n_blocks=64) as p:
+        p.execute(xin, whole)
+        p.check()
+    assert np.max(np.abs(y - whole.cpu().numpy())) <= 1e-5 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
+def test_stream_blockstate_history_in_and_out(device):
+    """A non-zero history in against scipy (lfiltic per section); the
+    histories out against the FP64 sections; a zero history is the plain
+    launch bit for bit."""
+    import torch
+    print("case stream block-state history", flush=True)
+    sos, g, x = normal_workload(8, 1 << 18, seed=66, wn=0.1)
+    ns = sos.shape[0]
+    rng = np.random.default_rng(67)
+    zi_h = (0.3 * rng.standard_normal(2 + 2 * ns)).astype(np.float32)
+    xin = torch.from_numpy(x).to(device)
     out = torch.empty_like(xin)
-    with iirx.Plan("BS", sos, xin.numel(), block_size=64, n_blocks=64) as p:
-        with pytest.raises(iirx.IirxError):
-            p.execute_state(xin, out)
+    plain = torch.empty_like(xin)
+    zi = torch.from_numpy(zi_h).to(device)
+    zf = torch.empty_like(zi)
+    with iirx.Plan("BS", sos, x.size, block_size=64, n_blocks=64) as p:
+        p.execute_state(xin, out, zi=zi, zf=zf)
+        p.check()
+        y = out.cpu().numpy()
+        ref = _sos_fp64_with_history(sos, x, zi_h)
+        assert np.max(np.abs(y - ref)) <= 1e-5 * np.max(np.abs(ref))
+        z = zf.cpu().numpy()
+        assert z[0] == x[-2] and z[1] == x[-1]
+        u = x.astype(np.float64)
+        u2, u1 = float(zi_h[0]), float(zi_h[1])
+        for s in range(ns):
+            st = signal.lfiltic(sos[s, :3].astype(np.float64), sos[s, 3:].astype(np.float64),
+                                y=[float(zi_h[3 + 2 * s]), float(zi_h[2 + 2 * s])], x=[u1, u2])
+            u2, u1 = float(zi_h[2 + 2 * s]), float(zi_h[3 + 2 * s])
+            u = signal.sosfilt(sos[s:s + 1].astype(np.float64), u, zi=st[None, :])[0]  # output of section s
+            scale = np.max(np.abs(u))
+            assert abs(z[2 + 2 * s] - u[-2]) <= 1e-5 * scale and abs(z[3 + 2 * s] - u[-1]) <= 1e-5 * scale, s
+        p.execute(xin, plain)
+        p.execute_state(xin, out, zi=torch.zeros_like(zi))
+        p.check()
+    assert torch.equal(out, plain)
 
 
 def _plr_fp64(sos, x, x_hist=None, y_hist=None):
