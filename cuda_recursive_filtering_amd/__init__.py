@@ -273,16 +273,17 @@ class Plan:
     __call__ = execute
 
     def state_size(self):
-        """Floats in a streaming state: 2 + 2 * n_sections (PH, STCR, DTCR) or
+        """Floats in a streaming state: 2 + 2 * n_sections (PH, STCR, DTCR, BS) or
         4 * n_sections (PLR: the order-K direct-form history, K = 2 n_sections)."""
         return 4 * self.n_sections if self.kernel in ("PLR", KERNELS["PLR"]) else 2 + 2 * self.n_sections
 
     def execute_state(self, d_in, d_out, zi=None, zf=None, stream=None):
         """One launch with the filter state carried in and out (a long signal
         filtered in calls of n_samples).  zi / zf: device float32 tensors (or
-        pointers) of state_size() values, oldest first: PH, STCR, DTCR x[-2],
-        x[-1] of the input, then y_s[-2], y_s[-1] of every section's output;
-        PLR x[-K..-1], then y[-K..-1].  None = zero state in / no state out."""
+        pointers) of state_size() values, oldest first: PH, STCR, DTCR and the
+        block-state form x[-2], x[-1] of the input, then y_s[-2], y_s[-1] of
+        every section's output; PLR x[-K..-1], then y[-K..-1].  None = zero
+        state in / no state out."""
         if self._h is None:
             raise RuntimeError("plan is closed")
         for t in (d_in, d_out):

@@ -180,15 +180,16 @@ int iirx_execute(iirx_plan *plan, const float *d_in, float *d_out,
  * a long signal can be filtered in calls of n_samples each.  The reference
  * has only zero-state hooks (ph_kernels.cuh:45, 162-165; plr_kernels.cuh:
  * 134-142).  States are direct-form histories in device memory, oldest first:
- *   PH, STCR, DTCR: 2 + 2 N_SECTIONS floats, x[-2], x[-1] of the input, then
- *     y_s[-2], y_s[-1] of the output of every section s;
+ *   PH, STCR, DTCR, block-state form: 2 + 2 N_SECTIONS floats, x[-2], x[-1]
+ *     of the input, then y_s[-2], y_s[-1] of the output of every section s;
  *   PLR: 2 K floats (K = 2 N_SECTIONS, the recurrence order), x[-K..-1] of
  *     the input, then y[-K..-1] of the output.
  * d_zi NULL: zero state; d_zf NULL: not written.  d_zi is read by the launch
- * and d_zf written by a small kernel after it, both on `stream`; d_zf may
- * equal d_zi (of this call or the next).  The block-state form: INVALID (its
- * carry is the balanced state vector, from which the sections' histories do
- * not follow). */
+ * (the block-state form maps it to its balanced state in a small kernel
+ * first) and d_zf written by a small kernel after it, all on `stream`; d_zf
+ * may equal d_zi (of this call or the next).  The block-state form computes
+ * d_zf by running the sections in FP64 over the last tile, from the state
+ * the kernel records at its start. */
 int iirx_execute_state(iirx_plan *plan, const float *d_in, float *d_out,
                        const float *d_zi, float *d_zf, void *stream);
 

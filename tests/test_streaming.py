@@ -1,4 +1,4 @@
-"""Streaming state (iirx_execute_state; PH, STCR, DTCR and PLR plans): a long
+"""Streaming state (iirx_execute_state, every kernel): a long
 signal filtered in calls, the filter state carried from one call to the
 next.  No counterpart in the reference, whose initial-state hooks are always
 zero (ph_kernels.cuh:45, 162-165; plr_kernels.cuh:134-142).  Checked by what streaming must preserve:
