@@ -251,6 +251,17 @@ __device__ __forceinline__ void publish2(uint64_t *r, bool full, uint32_t flag, 
     rec_store(r + o + 1, flag, y1);
 }
 
+// End state (y2, y1) of section `sec` before chunk `chunk`, for a launch that
+// runs after the one that filtered the chunks before it (same stream): the
+// full record of chunk - 1, or the streamed-in state (z2, z1) at chunk 0.
+__device__ __forceinline__ Carry2 prior_state2(uint64_t *lb, uint32_t chunk, int sec, int ns, float z2, float z1)
+{
+    if (chunk == 0) return Carry2{z2, z1};
+    uint64_t *r = rec2(lb, chunk - 1u, sec, ns);
+    return Carry2{rec_val(__hip_atomic_load(r + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)),
+                  rec_val(__hip_atomic_load(r + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))};
+}
+
 // Decoupled lookback over a 64-chunk window, one predecessor per lane.
 // Returns the end state of chunk `chunk - 1`; the chain starts at chunk
 // `first` from the state (s2, s1) (zero unless the caller streams a state in).
@@ -499,6 +510,52 @@ __device__ __forceinline__ void stage_out(const float *lds, float *__restrict__ 
         v.z = p[2];
         v.w = p[3];
         __builtin_nontemporal_store(v, d4 + i * T + tid);
+    }
+}
+
+// The last chunk of a signal whose length is no multiple of the chunk holds
+// `valid` samples (1 <= valid < L N).  They move with coalesced 4-byte
+// accesses in the layout of stage_in / stage_out; samples past the end read
+// as zero (the filter is causal: they cannot reach an earlier output; the
+// load index is clamped to the last sample, so every load stays in bounds)
+// and are not written.
+template <int L, int N, int T = L>
+__device__ __forceinline__ void stage_in_part(const float *__restrict__ src, float *lds, int valid)
+{
+    const int tid = threadIdx.x;
+    for (int e = tid; e < L * N; e += T) {
+        const float v = src[e < valid ? e : valid - 1];
+        lds[(e / N) * (N + 1) + e % N] = (e < valid) ? v : 0.0f;
+    }
+}
+
+template <int L, int N, int T = L>
+__device__ __forceinline__ void stage_out_part(const float *lds, float *__restrict__ dst, int valid)
+{
+    const int tid = threadIdx.x;
+    for (int e = tid; e < valid; e += T) dst[e] = lds[(e / N) * (N + 1) + e % N];
+}
+
+// Samples valid - 2 and valid - 1 of a chunk held as r[N] per lane (lane t
+// owns samples t N .. t N + N - 1), written by lane 0 to dst[0], dst[1],
+// through the chunk's LDS tile (row pitch N + 1), which is free between the
+// chunk's staging in and out.  Sample -1 (valid == 1) is lane 0's `before`,
+// the value just ahead of the chunk.  This is the streaming state at the end
+// of a signal that ends inside the chunk.  (Selecting the two samples from
+// registers instead lets the compiler hoist 2 N lane masks out of the
+// section loop, which spills scalar registers.)
+template <int N>
+__device__ __forceinline__ void tail_pair(const float (&r)[N], float *lds, int valid, float before, float *dst)
+{
+    const int lane = threadIdx.x & 63;
+    __syncthreads();  // earlier reads of the tile are done
+#pragma unroll
+    for (int n = 0; n < N; ++n) lds[lane * (N + 1) + n] = r[n];
+    __syncthreads();
+    if (lane == 0) {
+        const int e1 = valid - 1, e2 = valid - 2;
+        dst[0] = e2 >= 0 ? lds[(e2 / N) * (N + 1) + e2 % N] : before;
+        dst[1] = lds[(e1 / N) * (N + 1) + e1 % N];
     }
 }
 

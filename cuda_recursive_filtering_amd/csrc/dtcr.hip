@@ -22,7 +22,13 @@
 
 namespace iirx {
 
-template <int N, bool UNROLLED>
+// LAST = false: chunks 0 .. num_chunks - 1 under the plan's schedule.
+// LAST = true: one workgroup for chunk num_chunks, the partial chunk that ends
+// a length which is no multiple of the chunk, launched after the first launch
+// on the same stream: its incoming state is the full record its predecessor
+// left (no lookback), samples past the end read as zero and are not written,
+// and with a.tail set it records the end state there.
+template <int N, bool UNROLLED, bool LAST>
 __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
 {
     constexpr int L = 64;  // rows (lanes of one side)
@@ -36,11 +42,11 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
     const int tx = threadIdx.x & 63;
     const int ty = threadIdx.x >> 6;  // wave-uniform parity side
 
-    for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err,
-                   [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
+    const auto body = [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
     const long base = (long)chunk * (L * N);
     __syncthreads();  // previous chunk's LDS consumed
-    stage_in<L, N, 128>(a.in + base, inn);
+    if constexpr (LAST) stage_in_part<L, N, 128>(a.in + base, inn, a.last_valid);
+    else stage_in<L, N, 128>(a.in + base, inn);
     __syncthreads();
 
     // raw input history before the chunk (sec 0, tx 0)
@@ -52,6 +58,16 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
         g1 = a.zi[1];
         g2 = a.zi[0];
     }
+    // the end state of the signal: samples last_valid - 2 and - 1 of the input
+    // and of every section's output (sample -1: the value just ahead of the chunk)
+    const int v = a.last_valid;
+    const int t1 = ((v - 1) / N) * RS + (v - 1) % N;
+    const int t2 = v >= 2 ? ((v - 2) / N) * RS + (v - 2) % N : 0;
+    if constexpr (LAST)
+        if (a.tail && threadIdx.x == 0) {
+            a.tail[0] = v >= 2 ? inn[t2] : g1;
+            a.tail[1] = inn[t1];
+        }
     // boundary state carried between sections for tx == 0 (dtcr:337-371)
     float xi1 = 0.0f, xi2 = 0.0f, xi3 = 0.0f;
     float reg[HALF];
@@ -178,7 +194,7 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
         // chunk partial: even side -> word 0, odd side -> word 1; lookback by wave 0
         uint64_t *rec = rec2(a.lb, chunk, sec, a.ns);
         const float pside = readlane_f(yv, L - 1);
-        if (tx == 0) rec_store(rec + ty, part_flag, pside);
+        if (!LAST && tx == 0) rec_store(rec + ty, part_flag, pside);
         // the per-lane terminal coefficients: their load is in flight during the lookback
         const float4 tm = *reinterpret_cast<const float4 *>(cv + Y.kTerm + 4 * tx);
         if (ty == 0) {
@@ -186,8 +202,9 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
             const int depth = (int)c[Y.kDepth];
             // the chain starts from the streamed-in state of this section (zero by default)
             const float z2 = a.zi ? a.zi[2 + 2 * sec] : 0.0f, z1 = a.zi ? a.zi[3 + 2 * sec] : 0.0f;
-            const Carry2 X = lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
-                                       a.spin_limit, a.err, 0u, z2, z1);
+            const Carry2 X = LAST ? prior_state2(a.lb, chunk, sec, a.ns, z2, z1)
+                                  : lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
+                                              a.spin_limit, a.err, 0u, z2, z1);
             if (tx == 0) {
                 xch[1] = X.y2;
                 xch[2] = X.y1;
@@ -202,7 +219,7 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
             const float c1 = (ty == 0) ? tm.y : tm.w;  // he1 | hb1
             reg[HALF - 1] = fmaf(-c1, X1, fmaf(-c2, X0, yv));
             const float fside = readlane_f(reg[HALF - 1], L - 1);
-            if (tx == 0) rec_store(rec + 2 + ty, full_flag, fside);
+            if (!LAST && tx == 0) rec_store(rec + 2 + ty, full_flag, fside);
         }
 
         // deepest companion level (dtcr:555-590)
@@ -296,29 +313,38 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
 #pragma unroll
         for (int k = 0; k < HALF; ++k) wrow[2 * k + ty] = reg[k];
         __syncthreads();
+        if constexpr (LAST)
+            if (a.tail && threadIdx.x == 0) {
+                a.tail[2 + 2 * sec] = v >= 2 ? inn[t2] : X1;
+                a.tail[3 + 2 * sec] = inn[t1];
+            }
     }
 
-    stage_out<L, N, 128>(inn, a.out + base);
-    });
+    if constexpr (LAST) stage_out_part<L, N, 128>(inn, a.out + base, a.last_valid);
+    else stage_out<L, N, 128>(inn, a.out + base);
+    };
+    if constexpr (LAST) body(a.num_chunks, 0u, 0u);
+    else for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err, body);
 }
 
 template <int N, bool U>
-static hipError_t launch_n(const LaunchArgs &a, hipStream_t s)
+static hipError_t launch_n(const LaunchArgs &a, bool last, hipStream_t s)
 {
-    hipLaunchKernelGGL((dtcr_kernel<N, U>), dim3(a.grid), dim3(128), 0, s, a);
+    if (last) hipLaunchKernelGGL((dtcr_kernel<N, U, true>), dim3(1), dim3(128), 0, s, a);
+    else hipLaunchKernelGGL((dtcr_kernel<N, U, false>), dim3(a.grid), dim3(128), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_dtcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s)
+hipError_t launch_dtcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last)
 {
     if (lanes != 64) return hipErrorInvalidValue;
     const bool u = a.unrolled != 0;
     switch (n_blocks) {
-    case 8: return u ? launch_n<8, true>(a, s) : launch_n<8, false>(a, s);
-    case 16: return u ? launch_n<16, true>(a, s) : launch_n<16, false>(a, s);
-    case 32: return u ? launch_n<32, true>(a, s) : launch_n<32, false>(a, s);
-    case 64: return u ? launch_n<64, true>(a, s) : launch_n<64, false>(a, s);
-    case 128: return u ? launch_n<128, true>(a, s) : launch_n<128, false>(a, s);
+    case 8: return u ? launch_n<8, true>(a, last, s) : launch_n<8, false>(a, last, s);
+    case 16: return u ? launch_n<16, true>(a, last, s) : launch_n<16, false>(a, last, s);
+    case 32: return u ? launch_n<32, true>(a, last, s) : launch_n<32, false>(a, last, s);
+    case 64: return u ? launch_n<64, true>(a, last, s) : launch_n<64, false>(a, last, s);
+    case 128: return u ? launch_n<128, true>(a, last, s) : launch_n<128, false>(a, last, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -328,11 +354,11 @@ const void *kernel_dtcr(int lanes, int n_blocks, int unrolled)
     if (lanes != 64) return nullptr;
     const bool u = unrolled != 0;
     switch (n_blocks) {
-    case 8: return u ? (const void *)dtcr_kernel<8, true> : (const void *)dtcr_kernel<8, false>;
-    case 16: return u ? (const void *)dtcr_kernel<16, true> : (const void *)dtcr_kernel<16, false>;
-    case 32: return u ? (const void *)dtcr_kernel<32, true> : (const void *)dtcr_kernel<32, false>;
-    case 64: return u ? (const void *)dtcr_kernel<64, true> : (const void *)dtcr_kernel<64, false>;
-    case 128: return u ? (const void *)dtcr_kernel<128, true> : (const void *)dtcr_kernel<128, false>;
+    case 8: return u ? (const void *)dtcr_kernel<8, true, false> : (const void *)dtcr_kernel<8, false, false>;
+    case 16: return u ? (const void *)dtcr_kernel<16, true, false> : (const void *)dtcr_kernel<16, false, false>;
+    case 32: return u ? (const void *)dtcr_kernel<32, true, false> : (const void *)dtcr_kernel<32, false, false>;
+    case 64: return u ? (const void *)dtcr_kernel<64, true, false> : (const void *)dtcr_kernel<64, false, false>;
+    case 128: return u ? (const void *)dtcr_kernel<128, true, false> : (const void *)dtcr_kernel<128, false, false>;
     default: return nullptr;
     }
 }

@@ -63,6 +63,11 @@ struct iirx_plan {
     // histories and the balanced state, then 16 floats of incoming state and
     // 16 of the state entering the last tile (one allocation)
     BsStateMap *d_bsmap;
+    // PH/STCR/DTCR with a length that is no multiple of the chunk: the last
+    // chunk is partial and runs in a one-workgroup launch of its own after
+    // the main launch, which filters main_chunks = num_chunks - 1 chunks
+    bool split_last;
+    uint32_t main_chunks;
 };
 
 namespace {
@@ -94,9 +99,10 @@ int validate(const iirx_config *cfg, int64_t n)
         const int B = cfg->block_size, N = cfg->n_blocks;
         if (!(B == 32 || B == 64)) return IIRX_ERR_GEOMETRY;
         if (!is_pow2(N) || N < 8 || N > 128) return IIRX_ERR_GEOMETRY;
+        // any length: the last chunk may be partial (the reference requires a
+        // multiple of the chunk, test_accuracy.cu:37-38)
         const int64_t chunk = 64LL * N;
-        if (n % chunk != 0) return IIRX_ERR_GEOMETRY;
-        if (n / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
+        if ((n + chunk - 1) / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
         return IIRX_OK;
     }
     case IIRX_KERNEL_BLOCKSTATE: {
@@ -212,7 +218,7 @@ int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double
         return IIRX_ERR_INVALID;
     }
     sh.chunk = (int64_t)sh.lanes * sh.n_blocks;
-    sh.num_chunks = (uint32_t)(n / sh.chunk);
+    sh.num_chunks = (uint32_t)((n + sh.chunk - 1) / sh.chunk);
     sh.lb_count = (size_t)sh.num_chunks * (size_t)ns * 4u;  // 2 partial + 2 full words per (chunk, section)
     return IIRX_OK;
 }
@@ -240,7 +246,7 @@ const char *iirx_status_string(int s)
     switch (s) {
     case IIRX_OK: return "ok";
     case IIRX_ERR_INVALID: return "invalid argument or unsupported configuration";
-    case IIRX_ERR_GEOMETRY: return "geometry: length not a multiple of the chunk, or no kernel instance";
+    case IIRX_ERR_GEOMETRY: return "geometry: block-state length not a multiple of the chunk, or no kernel instance";
     case IIRX_ERR_CONDITIONING: return "CR factors ill-conditioned for this filter and N_BLOCKS";
     case IIRX_ERR_HIP: return "HIP runtime error";
     case IIRX_ERR_NO_DEVICE: return "no GPU device";
@@ -293,6 +299,10 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     const size_t lb_count = sh.lb_count;
 
     p->d_bsmap = nullptr;
+    p->split_last = (cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR ||
+                     cfg->kernel == IIRX_KERNEL_DTCR) &&
+                    n_samples % sh.chunk != 0;
+    p->main_chunks = p->num_chunks - (p->split_last ? 1u : 0u);
     auto fail = [&](int code) {
         if (p->d_coef) hipFree(p->d_coef);
         if (p->d_lb) hipFree(p->d_lb);
@@ -345,7 +355,7 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     if (!p->fn) return fail(IIRX_ERR_GEOMETRY);
     // workgroups: one per chunk (ticket) or one per resident slot (persistent);
     // a block-state workgroup runs chunks_per_block waves, each its own chunks
-    const int64_t blocks_needed = ((int64_t)p->num_chunks + sh.chunks_per_block - 1) / sh.chunks_per_block;
+    const int64_t blocks_needed = ((int64_t)p->main_chunks + sh.chunks_per_block - 1) / sh.chunks_per_block;
     p->grid = (uint32_t)blocks_needed;
     if (cfg->schedule == IIRX_SCHEDULE_PERSISTENT) {
         int per_cu = 0, cus = 0;
@@ -421,7 +431,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
     a.err = p->d_err;
     a.arrive = p->d_arrive;
     a.n_samples = p->n_samples;
-    a.num_chunks = p->num_chunks;
+    a.num_chunks = p->main_chunks;
     a.spin_limit = p->spin_limit;
     a.persistent = p->cfg.schedule == IIRX_SCHEDULE_PERSISTENT;
     a.grid = p->grid;
@@ -430,6 +440,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
     a.unrolled = p->cfg.unrolled;
     a.zi = d_zi;
     a.tail = nullptr;
+    a.last_valid = (int32_t)(p->n_samples - (int64_t)p->main_chunks * p->chunk);
     hipError_t e = hipSuccess;
     float *bs_z0 = nullptr, *bs_tail = nullptr;
     if (p->cfg.kernel == IIRX_KERNEL_BLOCKSTATE && p->d_bsmap) {
@@ -446,9 +457,23 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
     if (e != hipSuccess) return IIRX_ERR_HIP;
     e = hipErrorInvalidValue;
     switch (p->cfg.kernel) {
-    case IIRX_KERNEL_PH: e = launch_ph(p->lanes, p->n_blocks, a, s); break;
-    case IIRX_KERNEL_STCR: e = launch_stcr(p->lanes, p->n_blocks, a, s); break;
-    case IIRX_KERNEL_DTCR: e = launch_dtcr(p->lanes, p->n_blocks, a, s); break;
+    case IIRX_KERNEL_PH:
+    case IIRX_KERNEL_STCR:
+    case IIRX_KERNEL_DTCR: {
+        // the main launch, then the partial last chunk (which records zf)
+        auto run = [&](const LaunchArgs &la, bool last) {
+            if (p->cfg.kernel == IIRX_KERNEL_PH) return launch_ph(p->lanes, p->n_blocks, la, s, last);
+            if (p->cfg.kernel == IIRX_KERNEL_STCR) return launch_stcr(p->lanes, p->n_blocks, la, s, last);
+            return launch_dtcr(p->lanes, p->n_blocks, la, s, last);
+        };
+        e = p->main_chunks > 0 ? run(a, false) : hipSuccess;
+        if (e == hipSuccess && p->split_last) {
+            LaunchArgs b = a;
+            b.tail = d_zf;
+            e = run(b, true);
+        }
+        break;
+    }
     case IIRX_KERNEL_PLR: e = launch_plr(2 * p->cfg.n_sections, p->shape.plr_block, p->shape.plr_x, a, s); break;
     case IIRX_KERNEL_BLOCKSTATE: e = launch_bs(p->n_blocks, p->cfg.block_size, p->cfg.n_sections, a, s); break;
     default: break;
@@ -457,7 +482,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
         if (p->cfg.kernel == IIRX_KERNEL_PLR) e = launch_plr_end_state(a, d_zf, s);
         else if (p->cfg.kernel == IIRX_KERNEL_BLOCKSTATE)
             e = launch_bs_state_out(d_in, p->n_samples, p->cfg.n_blocks, bs_tail, p->d_bsmap, d_zf, s);
-        else e = launch_ph_end_state(a, (int)p->chunk, d_zf, s);
+        else if (!p->split_last) e = launch_ph_end_state(a, (int)p->chunk, d_zf, s);
     }
     if (e != hipSuccess) return IIRX_ERR_HIP;
     p->launches++;

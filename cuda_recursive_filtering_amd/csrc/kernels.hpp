@@ -30,9 +30,13 @@ struct LaunchArgs {
     // floats); PLR: x[-K..-1], y[-K..-1]; block-state form: the balanced
     // state vector (16 floats, natural order)
     const float *zi;
-    // block-state form: where the last chunk writes the state entering its
-    // last tile (16 floats; null: not written)
+    // where the last chunk writes the end state (null: not written).
+    // PH/STCR/DTCR, partial last chunk: x[n-2], x[n-1], then y_s[n-2],
+    // y_s[n-1] of every section (2 + 2 ns floats); block-state form: the
+    // state entering its last tile (16 floats)
     float *tail;
+    // PH/STCR/DTCR: samples in the partial last chunk
+    int32_t last_valid;
 };
 
 // plan control block: ticket, epoch, watchdog flag, then 64 arrival slots
@@ -78,7 +82,9 @@ constexpr PhLayout ph_layout(int L, int N)
 
 // Launchers (one per algorithm); `lanes` and `n_blocks` pick the template
 // instance.  Return hipErrorInvalidValue when no instance exists.
-hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s);
+// last = true: the one-workgroup launch of the partial last chunk (chunk
+// a.num_chunks, a.last_valid samples), after the launch of the others
+hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last = false);
 const void *kernel_ph(int lanes, int n_blocks);
 
 }  // namespace iirx
@@ -124,9 +130,9 @@ constexpr CrLayout cr_layout(int L)
     return p;
 }
 
-hipError_t launch_stcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s);
+hipError_t launch_stcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last = false);
 const void *kernel_stcr(int lanes, int n_blocks, int unrolled);
-hipError_t launch_dtcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s);
+hipError_t launch_dtcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last = false);
 const void *kernel_dtcr(int lanes, int n_blocks, int unrolled);
 
 }  // namespace iirx
@@ -296,8 +302,9 @@ hipError_t launch_batch(int algo, int n, const BatchArgs &a, hipStream_t s);
 const void *kernel_batch(int algo, int n);
 
 hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s);
-// end state of a PH launch in the layout of LaunchArgs::zi, from the last
-// chunk's full records and the last two input samples
+// end state of a PH, STCR or DTCR launch whose length is a multiple of the
+// chunk, in the layout of LaunchArgs::zi, from the last chunk's full records
+// and the last two input samples (a partial last chunk records it itself)
 hipError_t launch_ph_end_state(const LaunchArgs &a, int chunk_samples, float *zf, hipStream_t s);
 // end state of a PLR launch: the direct-form history x[n-K..n-1], y[n-K..n-1]
 hipError_t launch_plr_end_state(const LaunchArgs &a, float *zf, hipStream_t s);

@@ -19,7 +19,13 @@
 
 namespace iirx {
 
-template <int N>
+// LAST = false: chunks 0 .. num_chunks - 1 under the plan's schedule.
+// LAST = true: one workgroup for chunk num_chunks, the partial chunk that ends
+// a length which is no multiple of the chunk, launched after the first launch
+// on the same stream: its incoming state is the full record its predecessor
+// left (no lookback), samples past the end read as zero and are not written,
+// and with a.tail set it records the end state there.
+template <int N, bool LAST>
 __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
 {
     constexpr int L = 64;
@@ -27,12 +33,12 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
     __shared__ float lds[L * (N + 1)];
 
     const int lane = threadIdx.x;
-    for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err,
-                   [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
+    const auto body = [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
     const long base = (long)chunk * (L * N);
 
     __syncthreads();  // LDS of the previous chunk fully consumed
-    stage_in<L, N>(a.in + base, lds);
+    if constexpr (LAST) stage_in_part<L, N>(a.in + base, lds, a.last_valid);
+    else stage_in<L, N>(a.in + base, lds);
     __syncthreads();
 
     float r[N];
@@ -54,6 +60,9 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
         xi2 = 0.0f;
         xi1 = 0.0f;
     }
+    // the end state of the signal: samples last_valid - 2 and - 1
+    if constexpr (LAST)
+        if (a.tail) tail_pair<N>(r, lds, a.last_valid, xi1, a.tail);
 
     for (int sec = 0; sec < a.ns; ++sec) {
         const float *cv = a.coef + (size_t)sec * a.coef_stride;  // per-lane vector loads
@@ -85,16 +94,17 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
         const float m0 = c[Y.kM + 0], m1 = c[Y.kM + 1], m2 = c[Y.kM + 2], m3 = c[Y.kM + 3];
         const float p2 = readlane_f(s2, L - 1), p1 = readlane_f(s1, L - 1);
         uint64_t *rec = rec2(a.lb, chunk, sec, a.ns);
-        if (lane == 0) publish2(rec, false, part_flag, p2, p1);
+        if (!LAST && lane == 0) publish2(rec, false, part_flag, p2, p1);
         const int depth = (int)c[Y.kDepth];
         // the chain starts from the streamed-in state of this section (zero by default)
         const float z2 = a.zi ? a.zi[2 + 2 * sec] : 0.0f, z1 = a.zi ? a.zi[3 + 2 * sec] : 0.0f;
         // C^(lane+1), the per-lane carry injection: its load is in flight
         // during the lookback instead of after it
         const float4 cf = *reinterpret_cast<const float4 *>(cv + Y.kCfin + 4 * lane);
-        const Carry2 X = lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
-                                   a.spin_limit, a.err, 0u, z2, z1);
-        if (lane == 0) {
+        const Carry2 X = LAST ? prior_state2(a.lb, chunk, sec, a.ns, z2, z1)
+                              : lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
+                                          a.spin_limit, a.err, 0u, z2, z1);
+        if (!LAST && lane == 0) {
             const float f2 = fmaf(m1, X.y1, fmaf(m0, X.y2, p2));
             const float f1 = fmaf(m3, X.y1, fmaf(m2, X.y2, p1));
             publish2(rec, true, full_flag, f2, f1);
@@ -120,14 +130,19 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
         // the next section's input history is this section's output
         xi2 = u2;
         xi1 = u1;
+        if constexpr (LAST)
+            if (a.tail) tail_pair<N>(r, lds, a.last_valid, X.y1, a.tail + 2 + 2 * sec);
     }
 
     __syncthreads();
 #pragma unroll
     for (int n = 0; n < N; ++n) lds[lane * (N + 1) + n] = r[n];
     __syncthreads();
-    stage_out<L, N>(lds, a.out + base);
-    });
+    if constexpr (LAST) stage_out_part<L, N>(lds, a.out + base, a.last_valid);
+    else stage_out<L, N>(lds, a.out + base);
+    };
+    if constexpr (LAST) body(a.num_chunks, 0u, 0u);
+    else for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err, body);
 }
 
 // zf = x[n-2], x[n-1], then (y2, y1) of every section from the last chunk's
@@ -150,21 +165,22 @@ hipError_t launch_ph_end_state(const LaunchArgs &a, int chunk_samples, float *zf
 }
 
 template <int N>
-static hipError_t launch_n(const LaunchArgs &a, hipStream_t s)
+static hipError_t launch_n(const LaunchArgs &a, bool last, hipStream_t s)
 {
-    hipLaunchKernelGGL(ph_kernel<N>, dim3(a.grid), dim3(64), 0, s, a);
+    if (last) hipLaunchKernelGGL((ph_kernel<N, true>), dim3(1), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((ph_kernel<N, false>), dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s)
+hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last)
 {
     if (lanes != 64) return hipErrorInvalidValue;
     switch (n_blocks) {
-    case 8: return launch_n<8>(a, s);
-    case 16: return launch_n<16>(a, s);
-    case 32: return launch_n<32>(a, s);
-    case 64: return launch_n<64>(a, s);
-    case 128: return launch_n<128>(a, s);
+    case 8: return launch_n<8>(a, last, s);
+    case 16: return launch_n<16>(a, last, s);
+    case 32: return launch_n<32>(a, last, s);
+    case 64: return launch_n<64>(a, last, s);
+    case 128: return launch_n<128>(a, last, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -173,11 +189,11 @@ const void *kernel_ph(int lanes, int n_blocks)
 {
     if (lanes != 64) return nullptr;
     switch (n_blocks) {
-    case 8: return (const void *)ph_kernel<8>;
-    case 16: return (const void *)ph_kernel<16>;
-    case 32: return (const void *)ph_kernel<32>;
-    case 64: return (const void *)ph_kernel<64>;
-    case 128: return (const void *)ph_kernel<128>;
+    case 8: return (const void *)ph_kernel<8, false>;
+    case 16: return (const void *)ph_kernel<16, false>;
+    case 32: return (const void *)ph_kernel<32, false>;
+    case 64: return (const void *)ph_kernel<64, false>;
+    case 128: return (const void *)ph_kernel<128, false>;
     default: return nullptr;
     }
 }

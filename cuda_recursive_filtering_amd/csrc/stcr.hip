@@ -44,7 +44,13 @@ __device__ __forceinline__ void cr_round_loop(float (&r)[N], int ro_dyn, float f
     }
 }
 
-template <int N, bool UNROLLED>
+// LAST = false: chunks 0 .. num_chunks - 1 under the plan's schedule.
+// LAST = true: one workgroup for chunk num_chunks, the partial chunk that ends
+// a length which is no multiple of the chunk, launched after the first launch
+// on the same stream: its incoming state is the full record its predecessor
+// left (no lookback), samples past the end read as zero and are not written,
+// and with a.tail set it records the end state there.
+template <int N, bool UNROLLED, bool LAST>
 __global__ __launch_bounds__(64) void stcr_kernel(LaunchArgs a)
 {
     constexpr int L = 64;
@@ -54,12 +60,12 @@ __global__ __launch_bounds__(64) void stcr_kernel(LaunchArgs a)
     __shared__ float lds[L * (N + 1)];
 
     const int lane = threadIdx.x;
-    for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err,
-                   [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
+    const auto body = [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
     const long base = (long)chunk * (L * N);
 
     __syncthreads();  // LDS of the previous chunk fully consumed
-    stage_in<L, N>(a.in + base, lds);
+    if constexpr (LAST) stage_in_part<L, N>(a.in + base, lds, a.last_valid);
+    else stage_in<L, N>(a.in + base, lds);
     __syncthreads();
 
     float r[N];
@@ -81,6 +87,9 @@ __global__ __launch_bounds__(64) void stcr_kernel(LaunchArgs a)
         xi2 = a.zi[0];
         xi1 = a.zi[1];
     }
+    // the end state of the signal: samples last_valid - 2 and - 1
+    if constexpr (LAST)
+        if (a.tail) tail_pair<N>(r, lds, a.last_valid, xi1, a.tail);
 
     for (int sec = 0; sec < a.ns; ++sec) {
         const float *cv = a.coef + (size_t)sec * a.coef_stride;  // per-lane vector loads
@@ -165,14 +174,15 @@ __global__ __launch_bounds__(64) void stcr_kernel(LaunchArgs a)
         const float m0 = c[Y.kM + 0], m1 = c[Y.kM + 1], m2 = c[Y.kM + 2], m3 = c[Y.kM + 3];
         const float p2 = readlane_f(ye, L - 1), p1 = readlane_f(yo, L - 1);
         uint64_t *rec = rec2(a.lb, chunk, sec, a.ns);
-        if (lane == 0) publish2(rec, false, part_flag, p2, p1);
+        if (!LAST && lane == 0) publish2(rec, false, part_flag, p2, p1);
         const int depth = (int)c[Y.kDepth];
         // the chain starts from the streamed-in state of this section (zero by default)
         const float z2 = a.zi ? a.zi[2 + 2 * sec] : 0.0f, z1 = a.zi ? a.zi[3 + 2 * sec] : 0.0f;
         // the per-lane terminal coefficients: their load is in flight during the lookback
         const float4 tm = *reinterpret_cast<const float4 *>(cv + Y.kTerm + 4 * lane);
-        const Carry2 Xc = lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
-                                   a.spin_limit, a.err, 0u, z2, z1);
+        const Carry2 Xc = LAST ? prior_state2(a.lb, chunk, sec, a.ns, z2, z1)
+                               : lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
+                                           a.spin_limit, a.err, 0u, z2, z1);
         const float X0 = Xc.y2, X1 = Xc.y1;
 
         // terminal correction (stcr:480-491)
@@ -182,7 +192,7 @@ __global__ __launch_bounds__(64) void stcr_kernel(LaunchArgs a)
         }
         {
             const float f2 = readlane_f(r[N - 2], L - 1), f1 = readlane_f(r[N - 1], L - 1);
-            if (lane == 0) publish2(rec, true, full_flag, f2, f1);
+            if (!LAST && lane == 0) publish2(rec, true, full_flag, f2, f1);
         }
 
         // deepest companion level (stcr:493-523)
@@ -273,33 +283,39 @@ __global__ __launch_bounds__(64) void stcr_kernel(LaunchArgs a)
         // hand-off: the next section's input history is this section's output
         w4 = lane_up(r[N - 4], 1);
         w3 = lane_up(r[N - 3], 1);
+        if constexpr (LAST)
+            if (a.tail) tail_pair<N>(r, lds, a.last_valid, X1, a.tail + 2 + 2 * sec);
     }
 
     __syncthreads();
 #pragma unroll
     for (int n = 0; n < N; ++n) lds[lane * (N + 1) + n] = r[n];
     __syncthreads();
-    stage_out<L, N>(lds, a.out + base);
-    });
+    if constexpr (LAST) stage_out_part<L, N>(lds, a.out + base, a.last_valid);
+    else stage_out<L, N>(lds, a.out + base);
+    };
+    if constexpr (LAST) body(a.num_chunks, 0u, 0u);
+    else for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err, body);
 }
 
 template <int N, bool U>
-static hipError_t launch_n(const LaunchArgs &a, hipStream_t s)
+static hipError_t launch_n(const LaunchArgs &a, bool last, hipStream_t s)
 {
-    hipLaunchKernelGGL((stcr_kernel<N, U>), dim3(a.grid), dim3(64), 0, s, a);
+    if (last) hipLaunchKernelGGL((stcr_kernel<N, U, true>), dim3(1), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((stcr_kernel<N, U, false>), dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_stcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s)
+hipError_t launch_stcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last)
 {
     if (lanes != 64) return hipErrorInvalidValue;
     const bool u = a.unrolled != 0;
     switch (n_blocks) {
-    case 8: return u ? launch_n<8, true>(a, s) : launch_n<8, false>(a, s);
-    case 16: return u ? launch_n<16, true>(a, s) : launch_n<16, false>(a, s);
-    case 32: return u ? launch_n<32, true>(a, s) : launch_n<32, false>(a, s);
-    case 64: return u ? launch_n<64, true>(a, s) : launch_n<64, false>(a, s);
-    case 128: return u ? launch_n<128, true>(a, s) : launch_n<128, false>(a, s);
+    case 8: return u ? launch_n<8, true>(a, last, s) : launch_n<8, false>(a, last, s);
+    case 16: return u ? launch_n<16, true>(a, last, s) : launch_n<16, false>(a, last, s);
+    case 32: return u ? launch_n<32, true>(a, last, s) : launch_n<32, false>(a, last, s);
+    case 64: return u ? launch_n<64, true>(a, last, s) : launch_n<64, false>(a, last, s);
+    case 128: return u ? launch_n<128, true>(a, last, s) : launch_n<128, false>(a, last, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -309,11 +325,11 @@ const void *kernel_stcr(int lanes, int n_blocks, int unrolled)
     if (lanes != 64) return nullptr;
     const bool u = unrolled != 0;
     switch (n_blocks) {
-    case 8: return u ? (const void *)stcr_kernel<8, true> : (const void *)stcr_kernel<8, false>;
-    case 16: return u ? (const void *)stcr_kernel<16, true> : (const void *)stcr_kernel<16, false>;
-    case 32: return u ? (const void *)stcr_kernel<32, true> : (const void *)stcr_kernel<32, false>;
-    case 64: return u ? (const void *)stcr_kernel<64, true> : (const void *)stcr_kernel<64, false>;
-    case 128: return u ? (const void *)stcr_kernel<128, true> : (const void *)stcr_kernel<128, false>;
+    case 8: return u ? (const void *)stcr_kernel<8, true, false> : (const void *)stcr_kernel<8, false, false>;
+    case 16: return u ? (const void *)stcr_kernel<16, true, false> : (const void *)stcr_kernel<16, false, false>;
+    case 32: return u ? (const void *)stcr_kernel<32, true, false> : (const void *)stcr_kernel<32, false, false>;
+    case 64: return u ? (const void *)stcr_kernel<64, true, false> : (const void *)stcr_kernel<64, false, false>;
+    case 128: return u ? (const void *)stcr_kernel<128, true, false> : (const void *)stcr_kernel<128, false, false>;
     default: return nullptr;
     }
 }

@@ -43,8 +43,9 @@ extern "C" {
 typedef enum {
     IIRX_OK = 0,
     IIRX_ERR_INVALID = 1,        /* bad argument or unsupported configuration */
-    IIRX_ERR_GEOMETRY = 2,       /* length not a multiple of the chunk, or
-                                    no kernel instance for this geometry */
+    IIRX_ERR_GEOMETRY = 2,       /* block-state length not a multiple of
+                                    the chunk, or no kernel instance for
+                                    this geometry */
     IIRX_ERR_CONDITIONING = 3,   /* CR factors ill-conditioned for this
                                     (filter, N_BLOCKS): |e_n| too small */
     IIRX_ERR_HIP = 4,            /* a HIP runtime call failed */
@@ -86,9 +87,9 @@ typedef enum {
  * (ph_kernels.cuh:6-14, stcr_kernels.cuh:7-22, dtcr_kernels.cuh:11-26,
  * plr_kernels.cuh:14-21) as runtime fields.
  *
- *   block_size  BLOCK_SIZE: lanes per chunk for PH/STCR/DTCR (32 or 64; the
- *               wave64 build also accepts 128 and 256: several waves per
- *               workgroup).  PLR: threads per block, 0 = per-order default.
+ *   block_size  BLOCK_SIZE: lanes per chunk for PH/STCR/DTCR (32 or 64;
+ *               both run as 64-lane chunks, two 32-lane chunks per wave).
+ *               PLR: threads per block, 0 = per-order default.
  *   n_blocks    N_BLOCKS: samples per lane, a power of two (PH: 8..128,
  *               STCR/DTCR: 8..128).  Ignored by PLR.
  *   n_sections  N_SECTIONS = order / 2 (1..8).
@@ -171,7 +172,11 @@ int iirx_plan_set_plr_taps(iirx_plan *plan, const double *b, const double *a,
  * captured graph replays correctly.  Calls on one plan must
  * be ordered (one stream, or explicit dependencies); distinct plans are
  * independent.  d_in and d_out must be 16-byte aligned (INVALID otherwise),
- * except for PLR plans, which move one float at a time.
+ * except for PLR plans, which move one float at a time.  PH, STCR, DTCR and
+ * PLR take any n_samples: the last chunk may be partial, its missing samples
+ * read as zero and not written (the reference's cascades require a multiple
+ * of the chunk, test_accuracy.cu:37-38); the block-state form needs a
+ * multiple of its G x L chunk.
  */
 int iirx_execute(iirx_plan *plan, const float *d_in, float *d_out,
                  void *stream);
@@ -187,9 +192,12 @@ int iirx_execute(iirx_plan *plan, const float *d_in, float *d_out,
  * d_zi NULL: zero state; d_zf NULL: not written.  d_zi is read by the launch
  * (the block-state form maps it to its balanced state in a small kernel
  * first) and d_zf written by a small kernel after it, all on `stream`; d_zf
- * may equal d_zi (of this call or the next).  The block-state form computes
- * d_zf by running the sections in FP64 over the last tile, from the state
- * the kernel records at its start. */
+ * may equal d_zi (of this call or the next).  PH, STCR and DTCR record the
+ * end state in their last chunk (samples n-2 and n-1 of the input and of
+ * every section's output, wherever the signal ends in the chunk) and the
+ * small kernel copies it to d_zf.  The block-state form computes d_zf by
+ * running the sections in FP64 over the last tile, from the state the
+ * kernel records at its start. */
 int iirx_execute_state(iirx_plan *plan, const float *d_in, float *d_out,
                        const float *d_zi, float *d_zf, void *stream);
 

@@ -43,12 +43,15 @@ def test_candidates_flags():
 
 
 def test_candidates_length_must_fit_the_chunk():
+    """Only the block-state form needs a multiple of its chunk; the cascades
+    and PLR take a partial last chunk."""
     sos, _ = iirx.butterworth_sos(4, 0.1)
-    n = 64 * 8 * 3  # a multiple of the 64 x 8 chunk only
+    n = 64 * 8 * 3 + 64  # no multiple of any block-state chunk (512 samples and up)
     k = _by_kernel(iirx.tune_candidates(sos, n))
     ph = {e["n_blocks"]: e["status"] for e in k["PH"]}
-    assert ph[8] == 0 and all(ph[N] == 2 for N in (16, 32, 64, 128))
-    assert k["PLR"][0]["status"] == 0  # PLR is tail-safe
+    assert all(ph[N] == 0 for N in (8, 16, 32, 64, 128))
+    assert all(e["status"] == 0 for e in k["STCR"] + k["DTCR"] + k["PLR"])
+    assert k["BS"] and all(e["status"] == 2 for e in k["BS"])
 
 
 def test_candidates_exclude_ill_conditioned_cr():

@@ -118,7 +118,8 @@ def test_lookback_depth(wn, N, lo, hi):
     ("PH", 32, 32, 1 << 20, True),
     ("PH", 48, 64, 1 << 20, False),       # BLOCK_SIZE not 32/64
     ("PH", 64, 48, 1 << 20, False),       # N_BLOCKS not a power of two
-    ("PH", 64, 64, (1 << 20) + 64, False),  # no tail path for cascaded kernels
+    ("PH", 64, 64, (1 << 20) + 64, True),   # a partial last chunk (the reference needs a multiple)
+    ("DTCR", 32, 16, 7, True),            # shorter than one chunk
     ("BS", 16, 128, 1 << 20, True),       # block-state: G = 16 tiles of L = 128
     ("BS", 32, 128, 1 << 20, True),
     ("BS", 64, 128, 1 << 20, False),      # G = 64 at L = 128 does not fit the LDS
