@@ -63,11 +63,17 @@ struct iirx_plan {
     // histories and the balanced state, then 16 floats of incoming state and
     // 16 of the state entering the last tile (one allocation)
     BsStateMap *d_bsmap;
-    // PH/STCR/DTCR with a length that is no multiple of the chunk: the last
-    // chunk is partial and runs in a one-workgroup launch of its own after
-    // the main launch, which filters main_chunks = num_chunks - 1 chunks
+    // a length that is no multiple of the chunk: the last chunk is partial
+    // and runs in a one-workgroup launch of its own after the main launch,
+    // which filters main_chunks = num_chunks - 1 chunks
     bool split_last;
     uint32_t main_chunks;
+    // block-state form, partial last chunk: it runs through the PH last-chunk
+    // kernel (64 x 64 samples) from the direct-form histories at the end of
+    // the whole chunks; its tables and those histories
+    float *d_aux;
+    int aux_stride;
+    float *d_mid;
 };
 
 namespace {
@@ -109,8 +115,9 @@ int validate(const iirx_config *cfg, int64_t n)
         const int L = cfg->n_blocks, G = cfg->block_size;
         if (!(L == 32 || L == 64 || L == 128) || !(G == 16 || G == 32 || G == 64) || (L == 128 && G == 64))
             return IIRX_ERR_GEOMETRY;
+        // any length: the samples past the last whole chunk go through PH (below)
         const int64_t chunk = (int64_t)G * L;
-        if (n % chunk != 0 || n / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
+        if ((n + chunk - 1) / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
         return IIRX_OK;
     }
     case IIRX_KERNEL_PLR: {
@@ -180,7 +187,7 @@ int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double
         sh.chunks_per_block = bs_wg_waves(cfg->n_blocks, cfg->block_size);
         sh.threads = 64 * sh.chunks_per_block;
         sh.chunk = (int64_t)cfg->block_size * cfg->n_blocks;
-        sh.num_chunks = (uint32_t)(n / sh.chunk);
+        sh.num_chunks = (uint32_t)((n + sh.chunk - 1) / sh.chunk);
         sh.lb_count = (size_t)sh.num_chunks * 32u;  // 16 partial + 16 full state words
         return IIRX_OK;
     case IIRX_KERNEL_PLR: {
@@ -300,10 +307,15 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
 
     p->d_bsmap = nullptr;
     p->split_last = (cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR ||
-                     cfg->kernel == IIRX_KERNEL_DTCR) &&
+                     cfg->kernel == IIRX_KERNEL_DTCR || cfg->kernel == IIRX_KERNEL_BLOCKSTATE) &&
                     n_samples % sh.chunk != 0;
     p->main_chunks = p->num_chunks - (p->split_last ? 1u : 0u);
+    p->d_aux = nullptr;
+    p->d_mid = nullptr;
+    p->aux_stride = 0;
     auto fail = [&](int code) {
+        if (p->d_aux) hipFree(p->d_aux);
+        if (p->d_mid) hipFree(p->d_mid);
         if (p->d_coef) hipFree(p->d_coef);
         if (p->d_lb) hipFree(p->d_lb);
         if (p->d_ticket) hipFree(p->d_ticket);
@@ -343,6 +355,17 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
         map.ns = cfg->n_sections;
         if (hipMalloc((void **)&p->d_bsmap, sizeof map + 32 * sizeof(float)) != hipSuccess) return fail(IIRX_ERR_OOM);
         if (hipMemcpy(p->d_bsmap, &map, sizeof map, hipMemcpyHostToDevice) != hipSuccess) return fail(IIRX_ERR_HIP);
+        if (p->split_last) {
+            // the partial last chunk (fewer than G L <= 4096 samples) runs the PH
+            // last-chunk kernel at 64 x 64
+            std::vector<float> aux;
+            p->aux_stride = ph_tables(p->sos.data(), cfg->n_sections, 64, 64, aux);
+            if (hipMalloc((void **)&p->d_aux, aux.size() * sizeof(float)) != hipSuccess) return fail(IIRX_ERR_OOM);
+            if (hipMalloc((void **)&p->d_mid, (2 + 2 * cfg->n_sections) * sizeof(float)) != hipSuccess)
+                return fail(IIRX_ERR_OOM);
+            if (hipMemcpy(p->d_aux, aux.data(), aux.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+                return fail(IIRX_ERR_HIP);
+        }
     } else if (hipMemcpy(p->d_coef, p->host_coef.data(), p->host_coef.size() * sizeof(float),
                          hipMemcpyHostToDevice) != hipSuccess) {
         return fail(IIRX_ERR_HIP);
@@ -452,7 +475,8 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
             e = launch_bs_state_in(d_zi, p->d_bsmap, bs_z0, s);
             a.zi = bs_z0;
         }
-        if (d_zf) a.tail = bs_tail;
+        // a partial last chunk starts from the histories at the end of the whole chunks
+        if (d_zf || p->split_last) a.tail = bs_tail;
     }
     if (e != hipSuccess) return IIRX_ERR_HIP;
     e = hipErrorInvalidValue;
@@ -475,12 +499,35 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
         break;
     }
     case IIRX_KERNEL_PLR: e = launch_plr(2 * p->cfg.n_sections, p->shape.plr_block, p->shape.plr_x, a, s); break;
-    case IIRX_KERNEL_BLOCKSTATE: e = launch_bs(p->n_blocks, p->cfg.block_size, p->cfg.n_sections, a, s); break;
+    case IIRX_KERNEL_BLOCKSTATE: {
+        e = p->main_chunks > 0 ? launch_bs(p->n_blocks, p->cfg.block_size, p->cfg.n_sections, a, s) : hipSuccess;
+        if (e == hipSuccess && p->split_last) {
+            // the partial last chunk: direct-form histories after the whole
+            // chunks (the state-out map), then the PH last-chunk kernel from them
+            const int64_t n_main = (int64_t)p->main_chunks * p->chunk;
+            const float *hist = d_zi;
+            if (p->main_chunks > 0) {
+                e = launch_bs_state_out(d_in, n_main, p->cfg.n_blocks, bs_tail, p->d_bsmap, p->d_mid, s);
+                hist = p->d_mid;
+            }
+            LaunchArgs b = a;
+            b.in = d_in + n_main;
+            b.out = d_out + n_main;
+            b.coef = p->d_aux;
+            b.coef_stride = p->aux_stride;
+            b.num_chunks = 0;  // the chunk at b.in
+            b.last_valid = (int32_t)(p->n_samples - n_main);
+            b.zi = hist;
+            b.tail = d_zf;
+            if (e == hipSuccess) e = launch_ph(64, 64, b, s, true);
+        }
+        break;
+    }
     default: break;
     }
     if (e == hipSuccess && d_zf) {
         if (p->cfg.kernel == IIRX_KERNEL_PLR) e = launch_plr_end_state(a, d_zf, s);
-        else if (p->cfg.kernel == IIRX_KERNEL_BLOCKSTATE)
+        else if (p->cfg.kernel == IIRX_KERNEL_BLOCKSTATE && !p->split_last)
             e = launch_bs_state_out(d_in, p->n_samples, p->cfg.n_blocks, bs_tail, p->d_bsmap, d_zf, s);
         else if (!p->split_last) e = launch_ph_end_state(a, (int)p->chunk, d_zf, s);
     }
@@ -579,6 +626,8 @@ int iirx_plan_destroy(iirx_plan *p)
     if (p->d_lb) hipFree(p->d_lb);
     if (p->d_ticket) hipFree(p->d_ticket);
     if (p->d_bsmap) hipFree(p->d_bsmap);
+    if (p->d_aux) hipFree(p->d_aux);
+    if (p->d_mid) hipFree(p->d_mid);
     delete p;
     return IIRX_OK;
 }

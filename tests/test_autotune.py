@@ -42,16 +42,15 @@ def test_candidates_flags():
     assert all(e["schedule"] == "ticket" for es in k.values() for e in es)
 
 
-def test_candidates_length_must_fit_the_chunk():
-    """Only the block-state form needs a multiple of its chunk; the cascades
-    and PLR take a partial last chunk."""
+def test_candidates_any_length():
+    """Every kernel takes a partial last chunk, so a length that is no
+    multiple of any chunk leaves every candidate eligible."""
     sos, _ = iirx.butterworth_sos(4, 0.1)
     n = 64 * 8 * 3 + 64  # no multiple of any block-state chunk (512 samples and up)
     k = _by_kernel(iirx.tune_candidates(sos, n))
     ph = {e["n_blocks"]: e["status"] for e in k["PH"]}
     assert all(ph[N] == 0 for N in (8, 16, 32, 64, 128))
-    assert all(e["status"] == 0 for e in k["STCR"] + k["DTCR"] + k["PLR"])
-    assert k["BS"] and all(e["status"] == 2 for e in k["BS"])
+    assert k["BS"] and all(e["status"] == 0 for e in k["STCR"] + k["DTCR"] + k["PLR"] + k["BS"])
 
 
 def test_candidates_exclude_ill_conditioned_cr():

@@ -124,7 +124,7 @@ def test_lookback_depth(wn, N, lo, hi):
     ("BS", 32, 128, 1 << 20, True),
     ("BS", 64, 128, 1 << 20, False),      # G = 64 at L = 128 does not fit the LDS
     ("BS", 16, 256, 1 << 20, False),      # no L = 256
-    ("BS", 16, 128, (1 << 20) + 1024, False),  # not a multiple of the 2048-sample chunk
+    ("BS", 16, 128, (1 << 20) + 1024, True),  # not a multiple of the 2048-sample chunk: PH ends it
 ])
 def test_config_check(kernel, B, N, n, ok):
     st = iirx.config_check(kernel, 2, n, block_size=B, n_blocks=N)

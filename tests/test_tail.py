@@ -1,8 +1,8 @@
-"""Cascades (PH, STCR, DTCR) at lengths that are no multiple of the chunk.
-The reference requires a multiple (test_accuracy.cu:37-38: grid =
-N / (BLOCK_SIZE * N_BLOCKS)); here the last chunk is partial and runs in a
-one-workgroup launch of its own after the others, from the state its
-predecessor recorded.  Checked against the FP64 cascade with the
+"""Cascades (PH, STCR, DTCR) and the block-state form at lengths that are
+no multiple of the chunk.  The reference requires a multiple
+(test_accuracy.cu:37-38: grid = N / (BLOCK_SIZE * N_BLOCKS)); here the last
+chunk is partial and runs in a one-workgroup launch of its own after the
+others, from the state its predecessor recorded.  Checked against the FP64 cascade with the
 reference's gate; guard samples around the buffers check that nothing past
 the signal is read into the result or written; the end state is checked
 against the FP64 direct-form histories of every section wherever the
@@ -152,3 +152,81 @@ def test_cascade_uneven_calls(device, kernel, geom, unrolled):
     zend = za.cpu().numpy()
     zref = _histories(sos, x)
     assert np.max(np.abs(zend - zref)) <= 1e-5 * max(1.0, float(np.max(np.abs(zref))))
+
+
+BS_CASES = [(64, 64), (16, 128), (32, 32)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("geom", BS_CASES)
+@pytest.mark.parametrize("order", [8, 16])
+def test_blockstate_any_length(device, geom, order):
+    """The block-state form at lengths that are no multiple of its G x L
+    chunk: the whole chunks on the matrix cores, the rest through the PH
+    last-chunk kernel from the direct-form histories the state-out map gives
+    after the whole chunks; signals shorter than one chunk go through PH
+    alone."""
+    ch = geom[0] * geom[1]
+    for n in (7 * ch + 1, 7 * ch + 2, 7 * ch + 777, 8 * ch - 1, 100, ch - 3):
+        print(f"case tail BS {geom} order={order} n={n}", flush=True)
+        sos, g, x = normal_workload(order, n, seed=59, wn=0.1)
+        y, _ = _run(device, "BS", geom, False, sos, x)
+        ok, err, mref, rel = O.gate(y, O.golden(sos, x))
+        assert ok, (n, rel)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("geom", BS_CASES)
+@pytest.mark.parametrize("r", [1, 2, 37])
+def test_blockstate_tail_state_out(device, geom, r):
+    ch = geom[0] * geom[1]
+    n = 5 * ch + r
+    print(f"case tail BS state {geom} n={n}", flush=True)
+    sos, g, x = normal_workload(16, n, seed=61, wn=0.1)
+    y, zf = _run(device, "BS", geom, False, sos, x, want_zf=True)
+    assert zf[0] == x[-2] and zf[1] == x[-1]
+    ref = _histories(sos, x)
+    u = x.astype(np.float64)
+    for s in range(sos.shape[0]):
+        u = signal.sosfilt(sos[s:s + 1].astype(np.float64), u)
+        scale = np.max(np.abs(u))
+        assert abs(zf[2 + 2 * s] - ref[2 + 2 * s]) <= 1e-5 * scale, s
+        assert abs(zf[3 + 2 * s] - ref[3 + 2 * s]) <= 1e-5 * scale, s
+
+
+@pytest.mark.gpu
+def test_blockstate_uneven_calls(device):
+    """Streaming calls of uneven lengths on the block-state form (one shorter
+    than a chunk, one updating the state in place) against the FP64 cascade
+    and its histories."""
+    import torch
+    geom = (64, 64)
+    ch = geom[0] * geom[1]
+    lens = [3 * ch + 77, ch // 3, 2 * ch + 1, 4 * ch - 13]
+    n = sum(lens)
+    print(f"case tail BS calls lens={lens}", flush=True)
+    sos, g, x = normal_workload(16, n, seed=67, wn=0.1)
+    ns = sos.shape[0]
+    xin = torch.from_numpy(x).to(device)
+    out = torch.empty_like(xin)
+    za = torch.zeros(2 + 2 * ns, dtype=torch.float32, device=device)
+    zb = torch.empty_like(za)
+    o = 0
+    for i, m in enumerate(lens):
+        xc = xin[o:o + m].clone()
+        yc = torch.empty_like(xc)
+        with iirx.Plan("BS", sos, m, block_size=geom[0], n_blocks=geom[1]) as p:
+            if i == 2:
+                p.execute_state(xc, yc, zi=za, zf=za)
+            else:
+                p.execute_state(xc, yc, zi=za, zf=zb)
+                za, zb = zb, za
+            torch.cuda.synchronize()
+            p.check()
+        out[o:o + m] = yc
+        o += m
+    y = out.cpu().numpy()
+    ok, err, mref, rel = O.gate(y, O.golden(sos, x))
+    assert ok, rel
+    zref = _histories(sos, x)
+    assert np.max(np.abs(za.cpu().numpy() - zref)) <= 1e-5 * max(1.0, float(np.max(np.abs(zref))))
