@@ -253,7 +253,7 @@ const char *iirx_status_string(int s)
     switch (s) {
     case IIRX_OK: return "ok";
     case IIRX_ERR_INVALID: return "invalid argument or unsupported configuration";
-    case IIRX_ERR_GEOMETRY: return "geometry: block-state length not a multiple of the chunk, or no kernel instance";
+    case IIRX_ERR_GEOMETRY: return "geometry: no kernel instance for this geometry, or too many chunks";
     case IIRX_ERR_CONDITIONING: return "CR factors ill-conditioned for this filter and N_BLOCKS";
     case IIRX_ERR_HIP: return "HIP runtime error";
     case IIRX_ERR_NO_DEVICE: return "no GPU device";

@@ -43,9 +43,8 @@ extern "C" {
 typedef enum {
     IIRX_OK = 0,
     IIRX_ERR_INVALID = 1,        /* bad argument or unsupported configuration */
-    IIRX_ERR_GEOMETRY = 2,       /* block-state length not a multiple of
-                                    the chunk, or no kernel instance for
-                                    this geometry */
+    IIRX_ERR_GEOMETRY = 2,       /* no kernel instance for this geometry,
+                                    or more than 2^31 - 1 chunks */
     IIRX_ERR_CONDITIONING = 3,   /* CR factors ill-conditioned for this
                                     (filter, N_BLOCKS): |e_n| too small */
     IIRX_ERR_HIP = 4,            /* a HIP runtime call failed */
@@ -172,11 +171,13 @@ int iirx_plan_set_plr_taps(iirx_plan *plan, const double *b, const double *a,
  * captured graph replays correctly.  Calls on one plan must
  * be ordered (one stream, or explicit dependencies); distinct plans are
  * independent.  d_in and d_out must be 16-byte aligned (INVALID otherwise),
- * except for PLR plans, which move one float at a time.  PH, STCR, DTCR and
- * PLR take any n_samples: the last chunk may be partial, its missing samples
- * read as zero and not written (the reference's cascades require a multiple
- * of the chunk, test_accuracy.cu:37-38); the block-state form needs a
- * multiple of its G x L chunk.
+ * except for PLR plans, which move one float at a time.  Every kernel takes
+ * any n_samples (the reference's cascades require a multiple of the chunk,
+ * test_accuracy.cu:37-38): the last chunk may be partial, its missing
+ * samples read as zero and not written.  PH, STCR and DTCR run it as a
+ * second, one-workgroup launch; the block-state form runs it through the PH
+ * last-chunk kernel (64 x 64) from the direct-form histories after its whole
+ * chunks, so such a call is four launches.
  */
 int iirx_execute(iirx_plan *plan, const float *d_in, float *d_out,
                  void *stream);
