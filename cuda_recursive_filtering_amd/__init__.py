@@ -168,8 +168,8 @@ def _cfg_dict(c):
 
 def tune_candidates(sos, n_samples, flags=0):
     """Host-only candidate list (the autotuner's search space) with each
-    candidate's status: 0 eligible, 2 length does not fit the chunk, 3 CR
-    factors ill-conditioned for this filter."""
+    candidate's status: 0 eligible, 2 no instance for the geometry (or too
+    many chunks), 3 CR factors ill-conditioned for this filter."""
     s = _sos_array(sos)
     n = ctypes.c_int32()
     check(lib().iirx_tune_candidates(s.ctypes.data, s.shape[0], int(n_samples), int(flags), None, 0,

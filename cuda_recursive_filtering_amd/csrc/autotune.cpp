@@ -3,8 +3,8 @@
 // Replaces the reference's build-time search: run_accuracy.sh and
 // run_performance.sh sweep KERNEL x (BLOCK_SIZE, N_BLOCKS) per order and
 // calibrate PLR_X_MAX from ptxas spill reports (run_accuracy.sh:95-151).  Here
-// every candidate configuration is classified on the host (length fits the
-// chunk, CR factors well conditioned: the same checks a plan makes), then the
+// every candidate configuration is classified on the host (an instance for
+// the geometry, CR factors well conditioned: the same checks a plan makes), then the
 // eligible ones are timed on the device and the fastest is returned.
 #include "../../include/iirx.h"
 

@@ -332,10 +332,10 @@ int iirx_plan_read_tables(const iirx_plan *p, float *out, int64_t cap,
  * runtime search.  Candidates: PH (N_BLOCKS 8..128), STCR and DTCR (loop and
  * hand-unrolled, N_BLOCKS 8..128), the block-state form (G x L) and PLR
  * (automatic geometry, spill-calibrated X; orders <= 4 unless
- * IIRX_TUNE_PLR_ALL_ORDERS).  A candidate is excluded when the length does
- * not fit its chunk (IIRX_ERR_GEOMETRY) or when its CR factors are
- * ill-conditioned for the filter (IIRX_ERR_CONDITIONING: STCR/DTCR fall back
- * to PH/BS).
+ * IIRX_TUNE_PLR_ALL_ORDERS).  A candidate is excluded when the plan would
+ * reject it: no instance or too many chunks (IIRX_ERR_GEOMETRY), or CR
+ * factors ill-conditioned for the filter (IIRX_ERR_CONDITIONING: STCR/DTCR
+ * fall back to PH/BS).  Every kernel takes any length.
  * ------------------------------------------------------------------------ */
 enum {
     IIRX_TUNE_NO_BLOCKSTATE = 1,   /* plain fp32 kernels only */
