@@ -492,6 +492,10 @@ __device__ __forceinline__ void stage_in(const float *__restrict__ src, float *l
     }
 }
 
+// Stores go through the L2 (not nontemporal): PH 64 x 32 at 2^25 samples,
+// alternating builds on one box, ran 682-693 GS/s at order 2 against 651-663
+// with nontemporal stores (574 vs 555-570 at order 4, within 1% at orders 8
+// and 16).  The loads stay nontemporal: plain loads ran 639-644 at order 2.
 template <int L, int N, int T = L>
 __device__ __forceinline__ void stage_out(const float *lds, float *__restrict__ dst)
 {
@@ -509,7 +513,7 @@ __device__ __forceinline__ void stage_out(const float *lds, float *__restrict__ 
         v.y = p[1];
         v.z = p[2];
         v.w = p[3];
-        __builtin_nontemporal_store(v, d4 + i * T + tid);
+        d4[i * T + tid] = v;
     }
 }
 
