@@ -30,7 +30,10 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    # untimed launches before each timed run: the GPU's clocks take about
+    # 20 ms of back-to-back launches to settle (order 2: 689-691 GS/s after 20
+    # warmup launches, 711 after 400, same box)
+    ap.add_argument("--warmup", type=int, default=400)
     ap.add_argument("--kernel", default="PH")
     ap.add_argument("--geom", default="64x32")
     ap.add_argument("--schedule", default="persistent")
