@@ -228,10 +228,14 @@ class Plan:
     kernel       'PH' | 'STCR' | 'DTCR' | 'PLR' | 'BS'      (KERNEL; BS has no counterpart)
     sos          (n_sections, 6) b0-normalised sections     (FILTER_SOS)
     n_samples    signal length                              (2^N_SAMPLES_LOG2)
-    block_size   lanes per chunk, 32 or 64; PLR: threads per
+    block_size   32 or 64: a chunk is block_size * n_blocks
+                 samples, as in the reference; one wave64
+                 holds it (64 lanes x n_blocks at 64, 64 lanes
+                 x n_blocks / 2 at 32); PLR: threads per
                  block (0/None = per-order default); BS:
                  tiles per chunk, 16, 32 or 64              (BLOCK_SIZE / PLR_BLOCK_SIZE)
-    n_blocks     samples per lane; BS: tile length, 32/64   (N_BLOCKS)
+    n_blocks     samples per reference lane (8..128, 16..128
+                 at block_size 32); BS: tile length         (N_BLOCKS)
     unrolled     literal-index back substitution            (HANDUNROLLED)
     plr_x, plr_x_max   PLR values per thread                (PLR_X / PLR_X_MAX)
     schedule     'persistent' (default) or 'ticket'         (no counterpart)

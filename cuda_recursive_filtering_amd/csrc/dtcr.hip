@@ -47,9 +47,10 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
     __syncthreads();  // previous chunk's LDS consumed
     if constexpr (LAST) stage_in_part<L, N, 128>(a.in + base, inn, a.last_valid);
     else stage_in<L, N, 128>(a.in + base, inn);
-    __syncthreads();
 
-    // raw input history before the chunk (sec 0, tx 0)
+    // raw input history before the chunk (sec 0, tx 0), read by both waves
+    // before the barrier: the LAST instance may write the end state to a zf
+    // that is the same buffer as zi, and does so only after it
     float g1 = 0.0f, g2 = 0.0f;
     if (chunk > 0) {
         g1 = a.in[base - 1];
@@ -58,6 +59,7 @@ __global__ __launch_bounds__(128) void dtcr_kernel(LaunchArgs a)
         g1 = a.zi[1];
         g2 = a.zi[0];
     }
+    __syncthreads();
     // the end state of the signal: samples last_valid - 2 and - 1 of the input
     // and of every section's output (sample -1: the value just ahead of the chunk)
     const int v = a.last_valid;

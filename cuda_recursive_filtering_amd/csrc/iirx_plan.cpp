@@ -104,10 +104,12 @@ int validate(const iirx_config *cfg, int64_t n)
     case IIRX_KERNEL_DTCR: {
         const int B = cfg->block_size, N = cfg->n_blocks;
         if (!(B == 32 || B == 64)) return IIRX_ERR_GEOMETRY;
-        if (!is_pow2(N) || N < 8 || N > 128) return IIRX_ERR_GEOMETRY;
+        // BLOCK_SIZE = 32 keeps the reference's chunk of 32 N samples on one
+        // wave64 as 64 lanes x N/2 (instances exist for 8..128 per lane)
+        if (!is_pow2(N) || N < (B == 32 ? 16 : 8) || N > 128) return IIRX_ERR_GEOMETRY;
         // any length: the last chunk may be partial (the reference requires a
         // multiple of the chunk, test_accuracy.cu:37-38)
-        const int64_t chunk = 64LL * N;
+        const int64_t chunk = (int64_t)B * N;
         if ((n + chunk - 1) / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
         return IIRX_OK;
     }
@@ -169,17 +171,22 @@ int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double
     const int ns = cfg->n_sections;
     sh.lanes = 64;
     sh.n_blocks = cfg->n_blocks;
+    // PH / STCR / DTCR: a chunk is BLOCK_SIZE x N_BLOCKS samples as in the
+    // reference (test_accuracy.cu:37-38); a wave64 holds it as 64 lanes of
+    // 32 N / 64 samples when BLOCK_SIZE is 32
+    if (cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR || cfg->kernel == IIRX_KERNEL_DTCR)
+        sh.n_blocks = cfg->n_blocks * cfg->block_size / 64;
     sh.threads = 64;
     sh.chunks_per_block = 1;
     sh.plr_block = sh.plr_x = 0;
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH:
-        sh.coef_stride = ph_tables(sos, ns, 64, cfg->n_blocks, tables);
+        sh.coef_stride = ph_tables(sos, ns, 64, sh.n_blocks, tables);
         break;
     case IIRX_KERNEL_STCR:
     case IIRX_KERNEL_DTCR:
         if (cfg->kernel == IIRX_KERNEL_DTCR) sh.threads = 128;  // even and odd sibling waves
-        sh.coef_stride = cr_tables(sos, ns, 64, cfg->n_blocks, tables);
+        sh.coef_stride = cr_tables(sos, ns, 64, sh.n_blocks, tables);
         if (sh.coef_stride < 0) return IIRX_ERR_CONDITIONING;
         break;
     case IIRX_KERNEL_BLOCKSTATE:

@@ -86,11 +86,14 @@ typedef enum {
  * (ph_kernels.cuh:6-14, stcr_kernels.cuh:7-22, dtcr_kernels.cuh:11-26,
  * plr_kernels.cuh:14-21) as runtime fields.
  *
- *   block_size  BLOCK_SIZE: lanes per chunk for PH/STCR/DTCR (32 or 64;
- *               both run as 64-lane chunks, two 32-lane chunks per wave).
+ *   block_size  BLOCK_SIZE for PH/STCR/DTCR: 32 or 64.  A chunk (the
+ *               lookback unit) is BLOCK_SIZE x N_BLOCKS samples, as in the
+ *               reference (test_accuracy.cu:37-38), held by one wave64:
+ *               64 lanes x N_BLOCKS samples at 64, 64 lanes x N_BLOCKS / 2
+ *               at 32 (so 32 x 128 is the same chunk and kernel as 64 x 64).
  *               PLR: threads per block, 0 = per-order default.
- *   n_blocks    N_BLOCKS: samples per lane, a power of two (PH: 8..128,
- *               STCR/DTCR: 8..128).  Ignored by PLR.
+ *   n_blocks    N_BLOCKS: a power of two, 8..128 (16..128 at BLOCK_SIZE
+ *               32).  Ignored by PLR.
  *   n_sections  N_SECTIONS = order / 2 (1..8).
  *   unrolled    STCR_HANDUNROLLED / DTCR_HANDUNROLLED: select the
  *               template-expanded literal-index back substitution

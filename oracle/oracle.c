@@ -41,6 +41,23 @@
 #include <stdlib.h>
 #include <string.h>
 
+/*
+ * ORC_DOUBLE_ROUND (the liboracle_dr.so build): every fused multiply-add of
+ * the restatements rounds the product and then the sum, as the reference's
+ * Python emulators evaluate fma(a, b, c) = F32(F32(a) * F32(b) + F32(c))
+ * (stcr_emulator.py:50, dtcr_emulator.py:26) and as plr_emulator.py writes
+ * its products.  With it the restatements are compared with the emulators'
+ * own outputs bit for bit (tests/test_emulator_fixtures.py).
+ */
+#ifdef ORC_DOUBLE_ROUND
+static inline float orc_dr_fma(float a, float b, float c)
+{
+    const float p = a * b;
+    return p + c;
+}
+#define fmaf orc_dr_fma
+#endif
+
 #define ORC_OK 0
 #define ORC_EINVAL -1
 #define ORC_ENOMEM -2

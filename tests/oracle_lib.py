@@ -12,7 +12,10 @@ import numpy as np
 _ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 _ORACLE_DIR = os.path.join(_ROOT, "oracle")
 _LIB_PATH = os.path.join(_ORACLE_DIR, "liboracle.so")
-_lib = None
+# the same restatements with double-rounded multiply-adds, the rounding of the
+# reference's Python emulators (for bit-exact comparison with their outputs)
+_LIB_DR_PATH = os.path.join(_ORACLE_DIR, "liboracle_dr.so")
+_libs = {}
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -22,26 +25,32 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _ORACLE_DIR])
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            build()
-        L = ctypes.CDLL(_LIB_PATH)
-        L.orc_sosfilt_f64.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_long, _f64p]
-        L.orc_golden_f64.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_long, _f64p]
-        L.orc_ph_f32.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
-        L.orc_stcr_f32.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                   _f32p, ctypes.c_long, _f32p]
-        L.orc_dtcr_f32.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
-        L.orc_plr_f32.argtypes = [_f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                  _f32p, ctypes.c_long, _f32p]
-        L.orc_direct_form_f32.argtypes = [_f64p, _f64p, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
-        L.orc_cascade_f32.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
-        for f in (L.orc_ph_f32, L.orc_stcr_f32, L.orc_dtcr_f32, L.orc_plr_f32):
-            f.restype = ctypes.c_int
-        _lib = L
-    return _lib
+def _load(path):
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    L.orc_sosfilt_f64.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_long, _f64p]
+    L.orc_golden_f64.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_long, _f64p]
+    L.orc_ph_f32.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
+    L.orc_stcr_f32.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               _f32p, ctypes.c_long, _f32p]
+    L.orc_dtcr_f32.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
+    L.orc_plr_f32.argtypes = [_f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              _f32p, ctypes.c_long, _f32p]
+    L.orc_direct_form_f32.argtypes = [_f64p, _f64p, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
+    L.orc_cascade_f32.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
+    for f in (L.orc_ph_f32, L.orc_stcr_f32, L.orc_dtcr_f32, L.orc_plr_f32):
+        f.restype = ctypes.c_int
+    return L
+
+
+def lib(double_round=False):
+    """The oracle library; double_round=True: the build whose multiply-adds
+    round the product and the sum separately, as the emulators do."""
+    key = bool(double_round)
+    if key not in _libs:
+        _libs[key] = _load(_LIB_DR_PATH if key else _LIB_PATH)
+    return _libs[key]
 
 
 def _f32(a):
@@ -74,27 +83,27 @@ def _check(rc, what):
         raise ValueError(f"oracle {what}: invalid arguments (rc={rc})")
 
 
-def ph(sos, x, B, N):
+def ph(sos, x, B, N, double_round=False):
     sos = _f32(sos); x = _f32(x); y = np.empty_like(x)
-    _check(lib().orc_ph_f32(sos.reshape(-1), sos.shape[0], B, N, x, x.size, y), "ph")
+    _check(lib(double_round).orc_ph_f32(sos.reshape(-1), sos.shape[0], B, N, x, x.size, y), "ph")
     return y
 
 
-def stcr(sos, x, B, N, unrolled=False):
+def stcr(sos, x, B, N, unrolled=False, double_round=False):
     sos = _f32(sos); x = _f32(x); y = np.empty_like(x)
-    _check(lib().orc_stcr_f32(sos.reshape(-1), sos.shape[0], B, N, int(unrolled), x, x.size, y), "stcr")
+    _check(lib(double_round).orc_stcr_f32(sos.reshape(-1), sos.shape[0], B, N, int(unrolled), x, x.size, y), "stcr")
     return y
 
 
-def dtcr(sos, x, B, N):
+def dtcr(sos, x, B, N, double_round=False):
     sos = _f32(sos); x = _f32(x); y = np.empty_like(x)
-    _check(lib().orc_dtcr_f32(sos.reshape(-1), sos.shape[0], B, N, x, x.size, y), "dtcr")
+    _check(lib(double_round).orc_dtcr_f32(sos.reshape(-1), sos.shape[0], B, N, x, x.size, y), "dtcr")
     return y
 
 
-def plr(bt, at, x, block, X):
+def plr(bt, at, x, block, X, double_round=False):
     bt = _f64(bt); at = _f64(at); x = _f32(x); y = np.empty_like(x)
-    _check(lib().orc_plr_f32(bt, at, bt.size, block, X, x, x.size, y), "plr")
+    _check(lib(double_round).orc_plr_f32(bt, at, bt.size, block, X, x, x.size, y), "plr")
     return y
 
 
@@ -104,9 +113,9 @@ def direct_form_f32(bt, at, x):
     return y
 
 
-def cascade_f32(sos, x):
+def cascade_f32(sos, x, double_round=False):
     sos = _f32(sos); x = _f32(x); y = np.empty_like(x)
-    lib().orc_cascade_f32(sos.reshape(-1), sos.shape[0], x, x.size, y)
+    lib(double_round).orc_cascade_f32(sos.reshape(-1), sos.shape[0], x, x.size, y)
     return y
 
 

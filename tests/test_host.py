@@ -115,7 +115,8 @@ def test_lookback_depth(wn, N, lo, hi):
 
 @pytest.mark.parametrize("kernel,B,N,n,ok", [
     ("PH", 64, 64, 1 << 20, True),
-    ("PH", 32, 32, 1 << 20, True),
+    ("PH", 32, 32, 1 << 20, True),       # BLOCK_SIZE 32: the reference's 1024-sample chunk
+    ("PH", 32, 8, 1 << 20, False),        # 32 x 8 would need 4 samples per lane of a wave64
     ("PH", 48, 64, 1 << 20, False),       # BLOCK_SIZE not 32/64
     ("PH", 64, 48, 1 << 20, False),       # N_BLOCKS not a power of two
     ("PH", 64, 64, (1 << 20) + 64, True),   # a partial last chunk (the reference needs a multiple)

@@ -45,7 +45,7 @@ def _run(device, kernel, geom, unrolled, sos, x, schedule="persistent", zi=None,
 @pytest.mark.parametrize("r", [1, 2, 3, 517, -1])
 def test_cascade_any_length(device, kernel, geom, unrolled, r):
     """n = 37 chunks + r samples (r = -1: one sample short of 38 chunks)."""
-    ch = 64 * geom[1]
+    ch = geom[0] * geom[1]
     n = 37 * ch + (r if r > 0 else ch - 1)
     print(f"case tail {kernel} {geom} unrolled={unrolled} n={n}", flush=True)
     sos, g, x = normal_workload(8, n, seed=41 + r % 7, wn=0.1)
@@ -60,7 +60,7 @@ def test_cascade_any_length(device, kernel, geom, unrolled, r):
 def test_cascade_shorter_than_one_chunk(device, kernel, geom, unrolled, schedule):
     """The whole signal inside one partial chunk (no main launch), and a
     signal of one chunk plus one sample."""
-    ch = 64 * geom[1]
+    ch = geom[0] * geom[1]
     for n in (ch - 5, ch + 1, 3):
         print(f"case tail short {kernel} {geom} {schedule} n={n}", flush=True)
         sos, g, x = normal_workload(4, n, seed=43, wn=0.1)
@@ -95,7 +95,7 @@ def _histories(sos, x, zi=None):
 def test_cascade_tail_state_out(device, kernel, geom, unrolled, r):
     """zf of a signal that ends r samples into its last chunk: r = 1 takes
     y_s[n-2] from the state before the chunk."""
-    ch = 64 * geom[1]
+    ch = geom[0] * geom[1]
     n = 9 * ch + r
     print(f"case tail state {kernel} {geom} n={n}", flush=True)
     sos, g, x = normal_workload(8, n, seed=47, wn=0.1)
@@ -117,7 +117,7 @@ def test_cascade_uneven_calls(device, kernel, geom, unrolled):
     multiple of it), each call's zf the next call's zi, one call with zf ==
     zi: equal to the whole signal in one call and to the FP64 cascade."""
     import torch
-    ch = 64 * geom[1]
+    ch = geom[0] * geom[1]
     lens = [5 * ch + 77, ch // 3, 2 * ch + 1, 7 * ch - 13]
     n = sum(lens)
     print(f"case tail calls {kernel} {geom} lens={lens}", flush=True)
@@ -152,6 +152,47 @@ def test_cascade_uneven_calls(device, kernel, geom, unrolled):
     zend = za.cpu().numpy()
     zref = _histories(sos, x)
     assert np.max(np.abs(zend - zref)) <= 1e-5 * max(1.0, float(np.max(np.abs(zref))))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel,geom,unrolled", CASES)
+def test_cascade_short_call_state_in_place(device, kernel, geom, unrolled):
+    """A call shorter than one chunk (only the last-chunk launch runs) with a
+    non-zero state updated in place (zf is zi): every workgroup must read the
+    input history x[-2], x[-1] from zi before any writes zf.  Checked against
+    scipy's lfilter from lfiltic's state, section by section, and the state
+    out against the FP64 histories."""
+    import torch
+    ch = geom[0] * geom[1]
+    sos, g, x0 = normal_workload(8, 3 * ch, seed=59, wn=0.1)
+    ns = sos.shape[0]
+    # a state: the FP64 histories after a first stretch of signal
+    zi = _histories(sos, x0[:ch])
+    for n in (ch - 7, 3):
+        x = x0[ch:ch + n].copy()
+        print(f"case tail in-place state {kernel} {geom} unrolled={unrolled} n={n}", flush=True)
+        xin = torch.from_numpy(x).to(device)
+        yt = torch.empty_like(xin)
+        z = torch.from_numpy(zi.astype(np.float32)).to(device)
+        with iirx.Plan(kernel, sos, n, block_size=geom[0], n_blocks=geom[1], unrolled=unrolled) as p:
+            p.execute_state(xin, yt, zi=z, zf=z)
+            torch.cuda.synchronize()
+            p.check()
+        y = yt.cpu().numpy()
+        zi32 = zi.astype(np.float32).astype(np.float64)
+        u = x.astype(np.float64)
+        prev = zi32[0:2]
+        for s in range(ns):
+            b, a = sos[s, :3].astype(np.float64), sos[s, 3:].astype(np.float64)
+            yh = zi32[2 + 2 * s: 4 + 2 * s]
+            st = signal.lfiltic(b, a, y=[yh[1], yh[0]], x=[prev[1], prev[0]])
+            u = signal.lfilter(b, a, u, zi=st)[0]
+            prev = yh
+        ok, err, mref, rel = O.gate(y, u)
+        assert ok, (n, rel)
+        zend = z.cpu().numpy()
+        zref = _histories(sos, x, zi=zi32)
+        assert np.max(np.abs(zend - zref)) <= 1e-5 * max(1.0, float(np.max(np.abs(zref)))), (n, zend, zref)
 
 
 BS_CASES = [(64, 64), (16, 128), (32, 32)]
