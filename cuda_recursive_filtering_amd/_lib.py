@@ -17,6 +17,7 @@ IIRX_ERR_HIP = 4
 IIRX_ERR_NO_DEVICE = 5
 IIRX_ERR_OOM = 6
 IIRX_ERR_TIMEOUT = 7
+IIRX_ERR_TOO_LARGE = 8
 
 SCHEDULES = {"persistent": 0, "ticket": 1}
 
@@ -153,6 +154,7 @@ _SIGS = {
     "iirx_batch_info_get": (ctypes.c_int, [_P, ctypes.POINTER(BatchInfo)]),
     "iirx_batch_destroy": (ctypes.c_int, [_P]),
     "iirx_execute_state": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "iirx_execute_state_noalloc": (ctypes.c_int, [ctypes.POINTER(Config), _P, ctypes.c_int64, _P, _P, _P, _P, _P]),
     "iirx_normalize_sos": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_double)]),
     "iirx_batch_read_tables": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     "iirx_plan_read_tables": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -266,6 +267,8 @@ const char *iirx_status_string(int s)
     case IIRX_ERR_NO_DEVICE: return "no GPU device";
     case IIRX_ERR_OOM: return "out of memory";
     case IIRX_ERR_TIMEOUT: return "lookback watchdog: a wait exceeded the spin limit (workgroups not co-resident)";
+    case IIRX_ERR_TOO_LARGE:
+        return "the call needs more device scratch than iirx_execute_state_noalloc holds (IIRX_NOALLOC_SCRATCH_BYTES)";
     default: return "unknown status";
     }
 }
@@ -674,3 +677,145 @@ int iirx_measure(iirx_plan *p, const float *const *d_inputs, int n_inputs, float
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// iirx_execute_state_noalloc: a plan's call with the plan's device memory
+// (coefficient tables, lookback records, control block) in one file-scope
+// device array instead of hipMalloc'd buffers, for callers that cannot
+// allocate.  Layout of the array: [control block | tables | records].
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoallocCtl = (kControlBytes + 255) & ~(size_t)255;
+constexpr size_t kNoallocTables = 64u << 10;
+static_assert(kNoallocCtl + kNoallocTables < IIRX_NOALLOC_SCRATCH_BYTES, "scratch layout");
+
+static __device__ __align__(256) unsigned char scratch[IIRX_NOALLOC_SCRATCH_BYTES];
+
+// the array's address on each device (each GPU holds its own copy); the
+// array's name in host code is a host-side stand-in, never a device address
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+unsigned char *g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
+unsigned char *scratch_on(int dev)
+{
+    std::call_once(g_scratch_once[dev], [dev]() {
+        void *ptr = nullptr;
+        g_scratch_err[dev] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? static_cast<unsigned char *>(ptr) : nullptr;
+    });
+    return g_scratch_ptr[dev];
+}
+
+// the tables reach the device as kernel arguments (copied at launch, so the
+// host buffer may go away at once and no staging copy is involved)
+constexpr int kPieceFloats = 1000;
+struct TablePiece {
+    float v[kPieceFloats];
+};
+
+__global__ __launch_bounds__(256) void noalloc_put_tables(float *dst, TablePiece piece, int count)
+{
+    for (int i = threadIdx.x; i < count; i += blockDim.x) dst[i] = piece.v[i];
+}
+
+// zero the control block and the records this call uses (a fill kernel, not a
+// memset node: the call may be captured into a graph)
+__global__ __launch_bounds__(256) void noalloc_clear(uint64_t *p, size_t words)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+}  // namespace
+
+extern "C" int iirx_execute_state_noalloc(const iirx_config *cfg, const float *sos, int64_t n_samples,
+                                          const float *d_in, float *d_out, const float *d_zi, float *d_zf,
+                                          void *stream)
+{
+    if (!cfg || !sos || !d_in || !d_out) return IIRX_ERR_INVALID;
+    if (!(cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR || cfg->kernel == IIRX_KERNEL_DTCR))
+        return IIRX_ERR_INVALID;
+    int st = validate(cfg, n_samples);
+    if (st != IIRX_OK) return st;
+    if (!sos_ok(sos, cfg->n_sections)) return IIRX_ERR_INVALID;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return IIRX_ERR_NO_DEVICE;
+    if (dev < 0 || dev >= kMaxDevices) return IIRX_ERR_INVALID;
+    unsigned char *base = scratch_on(dev);
+    if (!base) return IIRX_ERR_HIP;
+
+    // the plan of this call, on the host stack, its device memory in the array
+    iirx_plan p;
+    p.cfg = *cfg;
+    p.n_samples = n_samples;
+    p.device = dev;
+    p.sos.assign(sos, sos + 6 * cfg->n_sections);
+    Shape sh;
+    st = make_shape(cfg, p.sos.data(), n_samples, nullptr, nullptr, sh, p.host_coef);
+    if (st != IIRX_OK) return st;
+    const size_t table_bytes = p.host_coef.size() * sizeof(float);
+    const size_t lb_bytes = sh.lb_count * sizeof(uint64_t);
+    if (table_bytes > kNoallocTables || kNoallocCtl + kNoallocTables + lb_bytes > IIRX_NOALLOC_SCRATCH_BYTES)
+        return IIRX_ERR_TOO_LARGE;
+    p.shape = sh;
+    p.lanes = sh.lanes;
+    p.n_blocks = sh.n_blocks;
+    p.threads = sh.threads;
+    p.chunk = sh.chunk;
+    p.num_chunks = sh.num_chunks;
+    p.coef_stride = sh.coef_stride;
+    p.d_ticket = reinterpret_cast<unsigned long long *>(base);
+    p.d_epoch = reinterpret_cast<uint32_t *>(base + 8);
+    p.d_err = p.d_epoch + 1;
+    p.d_arrive = reinterpret_cast<uint32_t *>(base + 128);
+    p.d_coef = reinterpret_cast<float *>(base + kNoallocCtl);
+    p.d_lb = reinterpret_cast<uint64_t *>(base + kNoallocCtl + kNoallocTables);
+    p.d_bsmap = nullptr;
+    p.d_aux = nullptr;
+    p.d_mid = nullptr;
+    p.aux_stride = 0;
+    p.spin_limit = 1u << 22;
+    p.launches = 0;
+    p.split_last = n_samples % sh.chunk != 0;
+    p.main_chunks = p.num_chunks - (p.split_last ? 1u : 0u);
+    p.fn = kernel_fn(cfg, sh);
+    if (!p.fn) return IIRX_ERR_GEOMETRY;
+    p.grid = p.main_chunks;
+    if (cfg->schedule == IIRX_SCHEDULE_PERSISTENT) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p.fn, p.threads, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return IIRX_ERR_HIP;
+        p.grid = (uint32_t)std::min<int64_t>((int64_t)std::max(per_cu, 1) * std::max(cus, 1), p.main_chunks);
+    }
+
+    // the same argument checks as iirx_execute_state
+    if (!io_aligned(&p, d_in, d_out)) return IIRX_ERR_INVALID;
+    const uintptr_t ib = (uintptr_t)d_in, ie = ib + (uintptr_t)n_samples * 4;
+    const uintptr_t ob = (uintptr_t)d_out, oe = ob + (uintptr_t)n_samples * 4;
+    if (ib < oe && ob < ie) return IIRX_ERR_INVALID;
+    if (d_zf) {
+        const uintptr_t zb = (uintptr_t)d_zf, ze = zb + (uintptr_t)(2 + 2 * cfg->n_sections) * 4;
+        if ((zb < ie && ib < ze) || (zb < oe && ob < ze)) return IIRX_ERR_INVALID;
+    }
+
+    // the array keeps the last call's contents: control block and records
+    // start from zero (epoch 0, no flag set), then the tables, then the call
+    hipStream_t s = (hipStream_t)stream;
+    const size_t words = (kNoallocCtl + kNoallocTables + lb_bytes) / sizeof(uint64_t);
+    const unsigned clear_blocks = (unsigned)std::min<size_t>(1024, (words + 255) / 256);
+    hipLaunchKernelGGL(noalloc_clear, dim3(clear_blocks), dim3(256), 0, s, reinterpret_cast<uint64_t *>(base), words);
+    if (hipGetLastError() != hipSuccess) return IIRX_ERR_HIP;
+    const int total = (int)p.host_coef.size();
+    for (int off = 0; off < total; off += kPieceFloats) {
+        TablePiece piece;
+        const int cnt = std::min(kPieceFloats, total - off);
+        std::memcpy(piece.v, p.host_coef.data() + off, sizeof(float) * (size_t)cnt);
+        hipLaunchKernelGGL(noalloc_put_tables, dim3(1), dim3(256), 0, s, p.d_coef + off, piece, cnt);
+        if (hipGetLastError() != hipSuccess) return IIRX_ERR_HIP;
+    }
+    return launch(&p, d_in, d_out, s, d_zi, d_zf);
+}

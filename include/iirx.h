@@ -50,9 +50,12 @@ typedef enum {
     IIRX_ERR_HIP = 4,            /* a HIP runtime call failed */
     IIRX_ERR_NO_DEVICE = 5,      /* no GPU visible */
     IIRX_ERR_OOM = 6,
-    IIRX_ERR_TIMEOUT = 7         /* a lookback wait exceeded the plan's spin
+    IIRX_ERR_TIMEOUT = 7,        /* a lookback wait exceeded the plan's spin
                                     limit (workgroups not co-resident); the
                                     output of that call is invalid */
+    IIRX_ERR_TOO_LARGE = 8       /* iirx_execute_state_noalloc: the call needs
+                                    more than IIRX_NOALLOC_SCRATCH_BYTES of
+                                    device scratch */
 } iirx_status;
 
 /* Workgroup scheduling of a plan (no counterpart in the reference, which
@@ -204,6 +207,31 @@ int iirx_execute(iirx_plan *plan, const float *d_in, float *d_out,
  * kernel records at its start. */
 int iirx_execute_state(iirx_plan *plan, const float *d_in, float *d_out,
                        const float *d_zi, float *d_zf, void *stream);
+
+/*
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  iirx_plan_create followed by iirx_execute_state, with the
+ * plan's device memory (coefficient tables, lookback records, control block)
+ * in one static device array of IIRX_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends, instead of allocations.  Calls no hipMalloc,
+ * hipFree or allocating library routine.  PH, STCR and DTCR only (INVALID
+ * otherwise); same arguments, checks and results as iirx_plan_create +
+ * iirx_execute_state with this cfg and sos.  Each call recomputes the tables
+ * on the host, clears the control block and the records it uses with a fill
+ * kernel and uploads the tables as kernel arguments before the filter
+ * launch, all on `stream`.  IIRX_ERR_TOO_LARGE if the tables and records do
+ * not fit (32 MiB: 2^25 samples at any order and geometry).  Every call on
+ * a GPU uses that GPU's one array: calls that overlap on one GPU (different
+ * streams, or host threads) race and give wrong results; calls on one
+ * stream are ordered and safe.  A lookback watchdog trip (persistent
+ * schedule without co-residency) cannot be reported here; use
+ * IIRX_SCHEDULE_TICKET where co-residency is not assured.
+ */
+#define IIRX_NOALLOC_SCRATCH_BYTES (32u << 20)
+int iirx_execute_state_noalloc(const iirx_config *cfg, const float *sos, int64_t n_samples,
+                               const float *d_in, float *d_out, const float *d_zi,
+                               float *d_zf, void *stream);
 
 int iirx_plan_info_get(const iirx_plan *plan, iirx_plan_info *info);
 

@@ -1,0 +1,133 @@
+"""iirx_execute_state_noalloc (no counterpart in the reference; for callers
+that cannot allocate device memory) against iirx_plan_create +
+iirx_execute_state with the same configuration, through the C ABI: the same
+kernels and tables, so the outputs agree within 8 float32 ulps of max|ref|
+(the lookback may fold a predecessor's full record or its partials,
+depending on timing, which changes the last bits) and both pass the FP64
+gate, on two calls in a row on one stream, the second with a smaller input,
+so that the second call starts from what the first left in the shared
+device array.  Also the size limit and the kernels it does not take."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import cuda_recursive_filtering_amd as iirx
+from cuda_recursive_filtering_amd import _lib
+import oracle_lib as O
+from workloads import normal_workload, ulp_scaled
+
+ULPS = 8.0
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(kernel, ns, B, N, unrolled=False, schedule=0):
+    c = _lib.Config()
+    c.kernel = _lib.KERNELS[kernel]
+    c.block_size, c.n_blocks, c.n_sections = B, N, ns
+    c.unrolled, c.plr_x, c.plr_x_max, c.schedule = int(unrolled), 0, 0, schedule
+    return c
+
+
+def _noalloc(cfg, sos, x_t, y_t, zi=None, zf=None):
+    import torch
+    L = _lib.lib()
+    s = np.ascontiguousarray(sos, np.float32)
+    st = L.iirx_execute_state_noalloc(ctypes.byref(cfg), s.ctypes.data, x_t.numel(), x_t.data_ptr(), y_t.data_ptr(),
+                                      zi.data_ptr() if zi is not None else None,
+                                      zf.data_ptr() if zf is not None else None,
+                                      torch.cuda.current_stream().cuda_stream)
+    return st
+
+
+def _with_plan(kernel, sos, x_t, B, N, unrolled, schedule, zi=None, zf=None):
+    import torch
+    y = torch.empty_like(x_t)
+    with iirx.Plan(kernel, sos, x_t.numel(), block_size=B, n_blocks=N, unrolled=unrolled, schedule=schedule) as p:
+        p.execute_state(x_t, y, zi=zi, zf=zf)
+        torch.cuda.synchronize()
+        p.check()
+    return y.cpu().numpy()
+
+
+CASES = [("PH", 64, 32, False, "persistent"), ("PH", 32, 64, False, "ticket"), ("STCR", 64, 64, True, "persistent"),
+         ("STCR", 32, 32, False, "persistent"), ("DTCR", 64, 64, False, "persistent"), ("DTCR", 64, 16, True, "ticket")]
+
+
+@pytest.mark.parametrize("kernel,B,N,unrolled,schedule", CASES)
+def test_noalloc_equals_plan_call(device, kernel, B, N, unrolled, schedule):
+    import torch
+    order = 8
+    ns = order // 2
+    sched = _lib.SCHEDULES[schedule]
+    cfg = _cfg(kernel, ns, B, N, unrolled, sched)
+    # two calls in a row on one stream, the second smaller (and not a multiple
+    # of the chunk), without a synchronisation between them
+    n1, n2 = (1 << 20) + 3 * B * N, 5 * B * N + 77
+    sos, g, x1 = normal_workload(order, n1, seed=71, wn=0.1)
+    _, _, x2 = normal_workload(order, n2, seed=72, wn=0.1)
+    print(f"case noalloc {kernel} {B}x{N} unrolled={unrolled} {schedule} n={n1},{n2}", flush=True)
+    t1, t2 = torch.from_numpy(x1).to(device), torch.from_numpy(x2).to(device)
+    y1, y2 = torch.full_like(t1, np.nan), torch.full_like(t2, np.nan)
+    assert _noalloc(cfg, sos, t1, y1) == 0
+    assert _noalloc(cfg, sos, t2, y2) == 0
+    torch.cuda.synchronize()
+    got1, got2 = y1.cpu().numpy(), y2.cpu().numpy()
+    want1 = _with_plan(kernel, sos, t1, B, N, unrolled, schedule)
+    want2 = _with_plan(kernel, sos, t2, B, N, unrolled, schedule)
+    r1, r2 = O.golden(sos, x1), O.golden(sos, x2)
+    assert ulp_scaled(got1, want1, r1) <= ULPS
+    assert ulp_scaled(got2, want2, r2) <= ULPS
+    assert O.gate(got1, r1)[0] and O.gate(got2, r2)[0]
+
+
+def test_noalloc_streaming_state(device):
+    """A state in and out: two calls, the first call's zf the second's zi,
+    equal to the plan's calls within the same bound, the state too."""
+    import torch
+    order, B, N = 16, 64, 32
+    ns = order // 2
+    cfg = _cfg("PH", ns, B, N)
+    sos, g, x = normal_workload(order, 9 * B * N + 5, seed=73, wn=0.1)
+    n1 = 4 * B * N + 19
+    xs = [torch.from_numpy(x[:n1].copy()).to(device), torch.from_numpy(x[n1:].copy()).to(device)]
+    za = torch.zeros(2 + 2 * ns, device=device)
+    zb = torch.empty_like(za)
+    outs = [torch.empty_like(t) for t in xs]
+    assert _noalloc(cfg, sos, xs[0], outs[0], zi=za, zf=zb) == 0
+    assert _noalloc(cfg, sos, xs[1], outs[1], zi=zb, zf=za) == 0
+    torch.cuda.synchronize()
+    pa = torch.zeros(2 + 2 * ns, device=device)
+    pb = torch.empty_like(pa)
+    w0 = _with_plan("PH", sos, xs[0], B, N, False, "persistent", zi=pa, zf=pb)
+    w1 = _with_plan("PH", sos, xs[1], B, N, False, "persistent", zi=pb, zf=pa)
+    ref = O.golden(sos, x)
+    got = np.concatenate([outs[0].cpu().numpy(), outs[1].cpu().numpy()])
+    assert ulp_scaled(got, np.concatenate([w0, w1]), ref) <= ULPS
+    assert O.gate(got, ref)[0]
+    za_h, pa_h = za.cpu().numpy(), pa.cpu().numpy()
+    assert np.all(za_h[:2] == pa_h[:2])  # the input history: copied samples
+    assert np.max(np.abs(za_h - pa_h)) <= ULPS * np.finfo(np.float32).eps * max(1.0, float(np.max(np.abs(pa_h))))
+
+
+def test_noalloc_limits(device):
+    """More records than the array holds: IIRX_ERR_TOO_LARGE before any
+    launch; kernels without a no-allocation path: INVALID."""
+    import torch
+    sos, g = iirx.butterworth_sos(16, 0.1)
+    x = torch.zeros(1024, device=device)
+    y = torch.empty_like(x)
+    L = _lib.lib()
+    s = np.ascontiguousarray(sos, np.float32)
+    big = _cfg("PH", 8, 64, 8)
+    n_big = 1 << 27  # 262144 chunks x 8 sections x 32 bytes = 64 MiB of records
+    st = L.iirx_execute_state_noalloc(ctypes.byref(big), s.ctypes.data, n_big, x.data_ptr(), y.data_ptr(),
+                                      None, None, None)
+    assert st == _lib.IIRX_ERR_TOO_LARGE
+    assert b"scratch" in L.iirx_status_string(st)
+    for k in ("PLR", "BS"):
+        c = _cfg(k, 8, 64, 64)
+        st = L.iirx_execute_state_noalloc(ctypes.byref(c), s.ctypes.data, 1024, x.data_ptr(), y.data_ptr(),
+                                          None, None, None)
+        assert st == _lib.IIRX_ERR_INVALID
