@@ -44,7 +44,13 @@ def parse():
     ap.add_argument("--geom-steps", type=int, default=None,
                     help="timed launches per configuration of the reference-geometry grid "
                          "(default: --steps; 0 skips the grid)")
-    ap.add_argument("--check", action="store_true", help="gate the last output against the FP64 cascade (scipy sosfilt)")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip the FP64 gate of every reported configuration (scipy sosfilt float64 golden)")
+    ap.add_argument("--check", action="store_true", help="ragged/sweep workloads: gate sampled channels")
+    ap.add_argument("--latency-steps", type=int, default=20,
+                    help="single launches, each after a synchronisation, timed for the one-call latency")
+    ap.add_argument("--no-ticket", action="store_true", help="skip the ticket-schedule measurement")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the all-cores CPU baseline (0: all)")
     ap.add_argument("--bs-geom", default="64x64",
                     help="block-state form geometry GxL for the MFMA vs plain fp32 comparison (orders 8 and 16)")
     ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA vs plain fp32 comparison")
@@ -57,9 +63,10 @@ def parse():
                          "(0: the library's choice, 8; 9: none)")
     ap.add_argument("--batch-bs-tiles", type=int, default=32,
                     help="ragged batch: block-state tiles of 64 samples per chunk (16, 32 or 64)")
-    ap.add_argument("--copy-steps", type=int, default=50,
-                    help="device-to-device copies of one 2^25-sample buffer, timed as the measured HBM ceiling "
-                         "(also the known-size kernel that scales rocprof FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--copy-steps", type=int, default=400,
+                    help="launches of the library's 16-byte streaming copy of one 2^25-sample buffer, timed as "
+                         "the measured HBM ceiling before the filters (also the known-size kernel that scales "
+                         "rocprof FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -91,14 +98,29 @@ def time_order(iirx, torch, order, args, bufs, out, stream, kernel=None, geom=No
     return res
 
 
-REFERENCE_GEOMETRIES = ("32x32", "32x64", "32x128", "64x64")  # BLOCK_SIZE x N_BLOCKS
+# the reference's BLOCK_SIZE x N_BLOCKS grid (32x32, 32x64, 32x128, 64x64); a
+# chunk is BLOCK_SIZE x N_BLOCKS samples on one wave64, so 32x128 is the 64x64
+# chunk and kernel and is reported once, as 64x64
+REFERENCE_GEOMETRIES = ("32x32", "32x64", "64x64")
 
 
-def reference_geometries(iirx, torch, order, args, bufs, out, stream):
+def gated(gs, out, ref, args):
+    """A measured configuration with its FP64 gate verdict; a configuration
+    that fails the gate reports 0 GS/s, as the reference's tables do, with
+    the measured figure beside it."""
+    if args.no_check or ref is None:
+        return {"gsamples_per_s": gs}
+    ok, rel = gate(out.cpu().numpy(), ref)
+    return {"gsamples_per_s": gs if ok else 0.0, "measured_gsamples_per_s": gs, "gate_pass": bool(ok),
+            "gate_rel": rel}
+
+
+def reference_geometries(iirx, torch, order, args, bufs, out, stream, ref):
     """The reference's own grid for one order: PH, STCR (loop and hand-unrolled)
-    and DTCR at BLOCK_SIZE x N_BLOCKS = 32x32, 32x64, 32x128, 64x64, and PLR at
-    its automatic geometry; GSamples/s over args.geom_steps timed launches, or
-    the status name when the configuration is rejected (CR conditioning)."""
+    and DTCR at BLOCK_SIZE x N_BLOCKS = 32x32, 32x64, 64x64 (= 32x128), and PLR
+    at its automatic geometry; GSamples/s over args.geom_steps timed launches
+    with each configuration's gate verdict, or the status name when the
+    configuration is rejected (CR conditioning)."""
     sos, g = iirx.butterworth_sos(order, args.wn)
     res = {}
     cases = [(k, u, geom) for k, u in (("PH", False), ("STCR", False), ("STCR", True), ("DTCR", False))
@@ -126,27 +148,75 @@ def reference_geometries(iirx, torch, order, args, bufs, out, stream):
         torch.cuda.synchronize()
         plan.check()
         plan.close()
-        res[name] = round(N_SAMPLES / (e0.elapsed_time(e1) / args.geom_steps * 1e-3) / 1e9, 1)
+        res[name] = gated(round(N_SAMPLES / (e0.elapsed_time(e1) / args.geom_steps * 1e-3) / 1e9, 1), out, ref, args)
     return res
 
 
-def copy_ceiling(torch, args, dev, stream):
-    """Measured copy bandwidth: read 2^25 floats + write 2^25 floats per copy."""
+def time_plan(torch, plan, bufs, out, stream, steps, warmup=5):
+    """ms per launch over `steps` back-to-back launches (HIP events on the
+    launch stream), after `warmup` launches."""
+    for i in range(warmup):
+        plan.execute(bufs[i % len(bufs)], out, stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for i in range(steps):
+        plan.execute(bufs[i % len(bufs)], out, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    plan.check()
+    return e0.elapsed_time(e1) / steps
+
+
+def latency(torch, plan, bufs, out, stream, steps):
+    """One call's latency: a single launch after a synchronisation, HIP events
+    around it, median over `steps` calls (the back-to-back figure hides the
+    launch overhead and the ramp of a lone launch)."""
+    ts = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for i in range(steps):
+        torch.cuda.synchronize()
+        e0.record(stream)
+        plan.execute(bufs[i % len(bufs)], out, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    plan.check()
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def copy_ceiling(iirx, torch, args, dev, stream):
+    """Measured HBM ceiling: the library's 16-byte streaming copy kernel over
+    one 2^25-float buffer (read 4 B + write 4 B per sample, as the filters),
+    inputs rotating over 4 buffers; nontemporal stores (mode 0) and stores
+    through the L2 (mode 1, as the cascades store); torch's copy_ beside it.
+    Run before the filters, its back-to-back launches also bring the clocks
+    up."""
     a = torch.empty(N_SAMPLES, dtype=torch.float32, device=dev).normal_()
     bufs = [a.clone() for _ in range(4)]
     dst = torch.empty_like(a)
-    for i in range(5):
-        dst.copy_(bufs[i % 4])
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for i in range(args.copy_steps):
-        dst.copy_(bufs[i % 4])
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / max(args.copy_steps, 1)
-    return {"ms": round(ms, 5), "gbs": round(BYTES_PER_SAMPLE * N_SAMPLES / (ms * 1e-3) / 1e9, 1)}
+    res = {}
+    for name, fn in (("stream_copy_nt_store", lambda i: iirx.copy_ceiling(bufs[i % 4], dst, N_SAMPLES, 0, stream)),
+                     ("stream_copy_l2_store", lambda i: iirx.copy_ceiling(bufs[i % 4], dst, N_SAMPLES, 1, stream)),
+                     ("torch_copy", lambda i: dst.copy_(bufs[i % 4]))):
+        for i in range(5):
+            fn(i)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for i in range(args.copy_steps):
+            fn(i)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / max(args.copy_steps, 1)
+        res[name] = {"ms": round(ms, 5), "gbs": round(BYTES_PER_SAMPLE * N_SAMPLES / (ms * 1e-3) / 1e9, 1)}
+    assert torch.equal(dst, bufs[(args.copy_steps - 1) % 4])
+    best = max(res["stream_copy_nt_store"]["gbs"], res["stream_copy_l2_store"]["gbs"])
+    res["gbs"] = best
+    res["frac_of_peak"] = round(best / HBM_PEAK_GBS, 4)
+    return res
 
 
 def measured_traffic(args):
@@ -172,67 +242,81 @@ def fp64_reference(sos, x):
 def gate(y, ref, rel_tol=1e-4):
     """The reference's accuracy gate (test_accuracy.cu:178-190): max|y - ref| <= rel_tol * max|ref|."""
     import numpy as np
-    err = float(np.max(np.abs(np.asarray(y, np.float64) - ref)))
+    y = np.asarray(y, np.float64)
+    if not np.all(np.isfinite(y)):
+        return False, None  # non-finite output (JSON has no inf / nan)
+    err = float(np.max(np.abs(y - ref)))
     mref = float(np.max(np.abs(ref)))
     return err <= rel_tol * mref, err / mref
+
+
+def _timed(fn, budget):
+    """Passes of fn() for at least `budget` seconds (one pass at least): (passes, seconds)."""
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget:
+            return reps, el
 
 
 def cpu_baseline(iirx, args):
     """The oracle's float32 PH restatement (a port of the reference algorithm at
     its published 32x64 geometry), single thread, over the 2^25-sample
-    order-2 workload, repeated for about --cpu-seconds."""
+    order-2 workload, repeated for about --cpu-seconds; and the same
+    restatement on all host cores (OpenMP over chunks, bit-identical output)."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
     import numpy as np
     import oracle_lib as O
     sos, g = iirx.butterworth_sos(2, args.wn)
     x = (g * iirx.fill_normal(1, N_SAMPLES).astype(np.float64)).astype(np.float32)
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        O.ph(sos, x, 32, 64)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= args.cpu_seconds:
-            break
-    return {"value": round(reps * N_SAMPLES / el / 1e9, 4), "unit": "GSamples/s", "cores": 1, "kind": "port",
+    reps, el = _timed(lambda: O.ph(sos, x, 32, 64), args.cpu_seconds)
+    one = {"value": round(reps * N_SAMPLES / el / 1e9, 4), "unit": "GSamples/s", "cores": 1, "kind": "port",
+           "sample": f"oracle PH float32 restatement (32x64, order 2, Wn={args.wn}) over 2^25 N(0,1) samples, "
+                     f"{reps} passes in {el:.1f} s, single thread"}
+    threads = args.cpu_threads or (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
+    reps, el = _timed(lambda: O.ph_mt(sos, x, 32, 64, threads), args.cpu_seconds / 2)
+    allc = {"value": round(reps * N_SAMPLES / el / 1e9, 4), "unit": "GSamples/s", "cores": threads, "kind": "port",
             "sample": f"oracle PH float32 restatement (32x64, order 2, Wn={args.wn}) over 2^25 N(0,1) samples, "
-                      f"{reps} passes in {el:.1f} s, single thread"}
+                      f"{reps} passes in {el:.1f} s, OpenMP over chunks ({threads} threads, output identical "
+                      f"to the single-thread restatement)"}
+    return one, allc
 
 
 def cpu_baselines_more(iirx, args):
-    """The other CPU references the workload names, each timed single-threaded
-    on the host over a bounded sample of the order-2 (and order-16) input:
-    scipy.signal.sosfilt in float64 (the golden) and the oracle's float32
-    restatements of the reference's STCR, DTCR and PLR emulators
-    (tools/stcr_emulator.py, dtcr_emulator.py, plr_emulator.py) at their
-    published geometries."""
+    """The other CPU references the workload names, each over the same
+    2^25-sample input at orders 2 and 16 (one pass at least): scipy.signal.
+    sosfilt in float64 (the golden; single thread), the oracle's PH
+    restatement on all host cores, and the oracle's float32 restatements of
+    the reference's STCR, DTCR and PLR emulators (tools/stcr_emulator.py,
+    dtcr_emulator.py, plr_emulator.py) at their published geometries, single
+    thread."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
     import numpy as np
     from scipy import signal
     import oracle_lib as O
-    n = 1 << 22
-    budget = max(1.0, args.cpu_seconds / 4)
+    n = N_SAMPLES
+    threads = args.cpu_threads or (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
     out = []
     for order in (2, 16):
         sos, g = iirx.butterworth_sos(order, args.wn)
         x = (g * iirx.fill_normal(1, n).astype(np.float64)).astype(np.float32)
         bt, at = (np.asarray(v, np.float64) for v in iirx.direct_form_taps(sos))
-        cases = [("scipy.signal.sosfilt float64", "library",
+        cases = [("scipy.signal.sosfilt float64", "library", 1,
                   lambda: signal.sosfilt(sos.astype(np.float64), x.astype(np.float64))),
-                 ("oracle STCR float32 restatement 32x64", "port", lambda: O.stcr(sos, x, 32, 64)),
-                 ("oracle DTCR float32 restatement 32x64", "port", lambda: O.dtcr(sos, x, 32, 64))]
+                 ("oracle STCR float32 restatement 32x64", "port", 1, lambda: O.stcr(sos, x, 32, 64)),
+                 ("oracle DTCR float32 restatement 32x64", "port", 1, lambda: O.dtcr(sos, x, 32, 64))]
         if order == 2:
-            cases.append(("oracle PLR float32 restatement 256x16", "port", lambda: O.plr(bt, at, x, 256, 16)))
-        for name, kind, fn in cases:
-            reps, t0 = 0, time.perf_counter()
-            while True:
-                fn()
-                reps += 1
-                el = time.perf_counter() - t0
-                if el >= budget / len(cases):
-                    break
-            out.append({"value": round(reps * n / el / 1e9, 4), "unit": "GSamples/s", "cores": 1, "kind": kind,
-                        "sample": f"{name}, order {order}, Wn={args.wn}, 2^22 N(0,1) samples, {reps} passes in "
-                                  f"{el:.1f} s, single thread"})
+            cases.append(("oracle PLR float32 restatement 256x16", "port", 1, lambda: O.plr(bt, at, x, 256, 16)))
+        else:
+            cases.append(("oracle PH float32 restatement 32x64", "port", threads, lambda: O.ph_mt(sos, x, 32, 64, threads)))
+        for name, kind, cores, fn in cases:
+            reps, el = _timed(fn, 0.5)
+            how = "single thread" if cores == 1 else f"OpenMP over chunks, {cores} threads"
+            out.append({"value": round(reps * n / el / 1e9, 4), "unit": "GSamples/s", "cores": cores, "kind": kind,
+                        "sample": f"{name}, order {order}, Wn={args.wn}, 2^25 N(0,1) samples, {reps} passes in "
+                                  f"{el:.1f} s, {how}"})
     return out
 
 
@@ -385,9 +469,17 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
 
+    # the measured ceiling first: its launches also bring the clocks up
+    ceiling = copy_ceiling(iirx, torch, args, dev, stream) if args.copy_steps > 0 else None
+    if ceiling:
+        print(f"# copy ceiling: {ceiling}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+
     nbuf = max(4, math.ceil(2 * MALL_BYTES / (4 * N_SAMPLES)) + 1)
     base = iirx.fill_normal(1, N_SAMPLES)
     orders = [int(o) for o in args.orders.split(",")]
+    if args.geom_steps is None:
+        args.geom_steps = args.steps
     per, mfma = {}, {}
     for order in orders:
         sos, g = iirx.butterworth_sos(order, args.wn)
@@ -404,41 +496,55 @@ def main():
                  "algorithmic_gflops": round(gs * 4 * order, 1), "vgprs": r["info"]["vgprs"],
                  "scratch": r["info"]["scratch_bytes"], "chunks": r["info"]["num_chunks"],
                  "grid": r["info"]["grid"]}
-        if args.check:
-            ok, rel = gate(out.cpu().numpy(), fp64_reference(sos, host))
+        if ceiling:
+            entry["ceiling_frac"] = round(gbs / ceiling["gbs"], 4)
+        # the FP64 golden of this order's input (every buffer holds the same samples)
+        ref = None if args.no_check else fp64_reference(sos, host)
+        if ref is not None:
+            ok, rel = gate(out.cpu().numpy(), ref)
             entry["gate_rel"] = rel
             entry["gate_pass"] = bool(ok)
+        # one call on its own, after a synchronisation
+        if args.latency_steps > 0:
+            entry["latency_us"] = round(1e3 * latency(torch, r["plan"], bufs, out, stream, args.latency_steps), 2)
+        # the forward-progress-safe schedule (one workgroup per chunk, atomic ticket)
+        if not args.no_ticket and args.kernel.upper() != "PLR":
+            B, N = (int(v) for v in args.geom.lower().split("x"))
+            with iirx.Plan(args.kernel, sos, N_SAMPLES, block_size=B, n_blocks=N, schedule="ticket") as pt:
+                ms_t = time_plan(torch, pt, bufs, out, stream, args.geom_steps or args.steps)
+            entry["ticket"] = gated(round(N_SAMPLES / (ms_t * 1e-3) / 1e9, 3), out, ref, args)
+            entry["ticket"]["ms"] = round(ms_t, 5)
         if not args.no_mfma and order in (8, 16):
             # MFMA block-state form beside the plain fp32 kernel, same input, error vs FP64
-            ref = fp64_reference(sos, host)
-            rms = float(np.sqrt(np.mean(ref ** 2)))
-            last = bufs[r["last_input"]]
+            refm = ref if ref is not None else fp64_reference(sos, host)
+            rms = float(np.sqrt(np.mean(refm ** 2)))
+            last = bufs[0]
             def errs(plan):
                 plan.execute(last, out, stream)
                 torch.cuda.synchronize()
-                d = out.cpu().numpy().astype(np.float64) - ref
+                d = out.cpu().numpy().astype(np.float64) - refm
                 return float(np.max(np.abs(d))) / rms, float(np.sqrt(np.mean(d ** 2))) / rms
             e_fp32, r_fp32 = errs(r["plan"])
             rb = time_order(iirx, torch, order, args, bufs, out, stream, kernel="BS", geom=args.bs_geom)
             e_bs, r_bs = errs(rb["plan"])
+            ok_bs, rel_bs = gate(out.cpu().numpy(), refm)
             rb["plan"].close()
             mfma[str(order)] = {
                 "bs_gsamples_per_s": round(N_SAMPLES / (rb["ms"] * 1e-3) / 1e9, 3), "bs_geometry": args.bs_geom,
+                "bs_split": "fp16 hi/lo",
                 "fp32_gsamples_per_s": entry["gsamples_per_s"], "fp32_kernel": f"{args.kernel} {args.geom}",
                 "bs_max_err_over_rms": e_bs, "fp32_max_err_over_rms": e_fp32,
-                "bs_rms_err_over_rms": r_bs, "fp32_rms_err_over_rms": r_fp32}
+                "bs_rms_err_over_rms": r_bs, "fp32_rms_err_over_rms": r_fp32,
+                "bs_gate_pass": bool(ok_bs), "bs_gate_rel": rel_bs}
             print(f"# mfma order {order}: {mfma[str(order)]}", file=sys.stderr, flush=True)
-        if args.geom_steps is None:
-            args.geom_steps = args.steps
         if args.geom_steps > 0:
-            entry["reference_geometries"] = reference_geometries(iirx, torch, order, args, bufs, out, stream)
+            entry["reference_geometries"] = reference_geometries(iirx, torch, order, args, bufs, out, stream, ref)
         per[str(order)] = entry
         r["plan"].close()
         del bufs, src, out
         torch.cuda.empty_cache()
         print(f"# order {order}: {entry}", file=sys.stderr, flush=True)
 
-    ceiling = copy_ceiling(torch, args, dev, stream) if args.copy_steps > 0 else None
     o2 = per[str(orders[0])]
     line = {
         "metric": "GSamples/s & % HBM roof at 2^25, order 2-16; MFMA vs plain fp32 @ equal err",
@@ -458,6 +564,8 @@ def main():
                    "kernel": args.kernel, "geometry": args.geom, "schedule": args.schedule,
                    "n_samples": N_SAMPLES,
                    "input_buffers": nbuf},
+        "gate_pass": o2.get("gate_pass"),
+        "gate_rel": o2.get("gate_rel"),
         "orders": per,
         "roofline": {"bound": "hbm", "achieved": o2["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": o2["roof_frac"], "traffic": measured_traffic(args),
@@ -467,7 +575,7 @@ def main():
     }
     line["host_cores"] = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(iirx, args)
+        line["cpu_baseline"], line["cpu_baseline_all_cores"] = cpu_baseline(iirx, args)
         line["cpu_baselines_more"] = cpu_baselines_more(iirx, args)
     print(json.dumps(line), flush=True)
 

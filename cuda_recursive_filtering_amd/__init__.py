@@ -144,6 +144,13 @@ def fill_normal(seed, n):
     return out
 
 
+def copy_ceiling(d_src, d_dst, n, mode=0, stream=None):
+    """One launch of the library's 16-byte streaming copy of n floats (the
+    measured HBM ceiling; mode 0: nontemporal loads and stores, mode 1:
+    stores through the L2)."""
+    check(lib().iirx_copy_ceiling(_ptr(d_src), _ptr(d_dst), int(n), int(mode), _stream_ptr(stream)), "copy_ceiling")
+
+
 def plan_tables(kernel, sos, n_samples, **kw):
     """Host copy of the FP64-precomputed tables a plan would upload: (tables, stride)."""
     s = _sos_array(sos)

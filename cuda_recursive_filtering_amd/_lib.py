@@ -154,6 +154,7 @@ _SIGS = {
     "iirx_batch_info_get": (ctypes.c_int, [_P, ctypes.POINTER(BatchInfo)]),
     "iirx_batch_destroy": (ctypes.c_int, [_P]),
     "iirx_execute_state": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "iirx_copy_ceiling": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.c_int32, _P]),
     "iirx_execute_state_noalloc": (ctypes.c_int, [ctypes.POINTER(Config), _P, ctypes.c_int64, _P, _P, _P, _P, _P]),
     "iirx_normalize_sos": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_double)]),
     "iirx_batch_read_tables": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),

@@ -264,6 +264,17 @@ int iirx_measure(iirx_plan *plan, const float *const *d_inputs, int n_inputs,
                  float *d_out, int warmup, int batches, int batch,
                  void *stream, float *batch_ms);
 
+/*
+ * No counterpart in the reference (measure.cuh times only the filter
+ * kernels; the reference quotes the DRAM bandwidth from the data sheet,
+ * README.md:138-141): the measured HBM ceiling the filters are compared
+ * with.  Copies n floats (n a multiple of 4, both buffers 16-byte aligned)
+ * with a streaming 16-byte copy kernel on `stream`, the filters' 8 B per
+ * sample without arithmetic.  mode 0: nontemporal loads and stores;
+ * mode 1: nontemporal loads, stores through the L2 (as the cascades store).
+ */
+int iirx_copy_ceiling(const float *d_src, float *d_dst, int64_t n, int32_t mode, void *stream);
+
 /* Library version string. */
 const char *iirx_version(void);
 

@@ -11,6 +11,8 @@
  *                        of tools/ref_generate.py:89.
  *   orc_ph_f32           float32 restatement of PH_32/PH_64
  *                        (include/ph_kernels.cuh:55-200 host, 208-820 device).
+ *   orc_ph_f32_mt        the same on several host threads (OpenMP over
+ *                        chunks), bit-identical; the all-cores CPU baseline.
  *   orc_stcr_f32         float32 restatement of STCR_{32,64}_{LOOP,UNROLL_*}
  *                        (include/stcr_kernels.cuh:78-272 host, 279-3608
  *                        device; operation order as tools/stcr_emulator.py).
@@ -258,6 +260,120 @@ int orc_ph_f32(const float *sos, int ns, int B, int N, const float *x, long n,
     }
     free(h1); free(h2); free(Cx); free(reg); free(inn); free(fullc);
     free(yi1); free(yi2); free(xi1); free(xi2);
+    return ORC_OK;
+}
+
+/*
+ * orc_ph_f32_mt: orc_ph_f32 on `threads` host threads (OpenMP over chunks),
+ * the same float operations in the same order, so the output is identical.
+ * Per section: (A) every chunk's zero-state pass and lane scan in parallel;
+ * (B) the chunk carries serially (each chunk's full carry from its
+ * predecessor's, ph:394-469); (C) every chunk's finalisation and
+ * back-correction in parallel.  The input history of a chunk's first lane
+ * (the previous chunk's last two section inputs) is saved before (A)
+ * overwrites it.  Used as the all-cores CPU baseline.
+ */
+int orc_ph_f32_mt(const float *sos, int ns, int B, int N, const float *x, long n,
+                  float *y, int threads)
+{
+    if (!(B == 32 || B == 64) || !is_pow2(N) || N < 4 || ns < 1 || ns > 64 || threads < 1) return ORC_EINVAL;
+    const long CH = (long)B * N;
+    if (n % CH != 0) return ORC_EINVAL;
+    const long nch = n / CH;
+    float *h1 = malloc(sizeof(float) * (size_t)ns * N);
+    float *h2 = malloc(sizeof(float) * (size_t)ns * N);
+    float *Cx = malloc(sizeof(float) * (size_t)ns * B * 4);
+    float *yi = malloc(sizeof(float) * (size_t)nch * B * 2);   /* per chunk: yi2[B], yi1[B] */
+    float *bnd = malloc(sizeof(float) * (size_t)nch * 2);      /* lane 0 input history */
+    float *xin = malloc(sizeof(float) * (size_t)nch * 2);      /* chunk carry in */
+    if (!h1 || !h2 || !Cx || !yi || !bnd || !xin) return ORC_ENOMEM;
+    float b1[64], b2[64], a1[64], a2[64];
+    for (int s = 0; s < ns; s++) {
+        b1[s] = sos[6 * s + 1]; b2[s] = sos[6 * s + 2];
+        a1[s] = -sos[6 * s + 4]; a2[s] = -sos[6 * s + 5];
+        float C[4];
+        ph_tables(a1[s], a2[s], N, h1 + (size_t)s * N, h2 + (size_t)s * N, C);
+        for (int k = 0; k < B; k++) mat2_pow(C, k + 1, Cx + ((size_t)s * B + k) * 4);
+    }
+    if (y != x) memcpy(y, x, sizeof(float) * (size_t)n);
+    for (int s = 0; s < ns; s++) {
+        const float *Cs = Cx + (size_t)s * B * 4;
+        /* the first lane's input history: raw input (section 0) or the
+         * previous section's output before the chunk, 0 for chunk 0 */
+        for (long c = 0; c < nch; c++) {
+            bnd[2 * c] = (c == 0) ? 0.0f : y[c * CH - 2];
+            bnd[2 * c + 1] = (c == 0) ? 0.0f : y[c * CH - 1];
+        }
+        /* (A) zero-state pass and lane scan */
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (long c = 0; c < nch; c++) {
+            float *reg = y + c * CH;
+            float *yi2 = yi + (size_t)c * B * 2, *yi1 = yi2 + B;
+            for (int t = B - 1; t >= 0; t--) {
+                float _x2, _x1;
+                if (t == 0) { _x2 = bnd[2 * c]; _x1 = bnd[2 * c + 1]; }
+                else { _x2 = reg[(size_t)(t - 1) * N + N - 2]; _x1 = reg[(size_t)(t - 1) * N + N - 1]; }
+                float _y2 = 0.0f, _y1 = 0.0f;
+                float *r = reg + (size_t)t * N;
+                for (int k = 0; k < N; k++) {
+                    const float xc = r[k];
+                    float yc = fmaf(b1[s], _x1, fmaf(b2[s], _x2, xc));
+                    if (k == 1) yc = fmaf(a1[s], _y1, yc);
+                    else if (k > 1) yc = fmaf(a1[s], _y1, fmaf(a2[s], _y2, yc));
+                    if (k < N - 2) r[k] = yc;
+                    _x2 = _x1; _x1 = xc; _y2 = _y1; _y1 = yc;
+                }
+                yi2[t] = _y2; yi1[t] = _y1;
+            }
+            for (int h = 1; h < B; h <<= 1) {
+                for (int t = 0; t < B; t++) {
+                    if (!(t & h)) continue;
+                    const int src = (t & ~(2 * h - 1)) + h - 1;
+                    const float *Cm = Cs + (size_t)(t % h) * 4;
+                    const float s2 = yi2[src], s1 = yi1[src];
+                    yi2[t] = fmaf(Cm[1], s1, fmaf(Cm[0], s2, yi2[t]));
+                    yi1[t] = fmaf(Cm[3], s1, fmaf(Cm[2], s2, yi1[t]));
+                }
+            }
+        }
+        /* (B) chunk carries: the full carry of chunk c is its last lane's
+         * finalised end state */
+        {
+            const float *Cm = Cs + (size_t)(B - 1) * 4;
+            float X0 = 0.0f, X1 = 0.0f;
+            for (long c = 0; c < nch; c++) {
+                xin[2 * c] = X0;
+                xin[2 * c + 1] = X1;
+                const float *yi2 = yi + (size_t)c * B * 2, *yi1 = yi2 + B;
+                const float f0 = fmaf(Cm[1], X1, fmaf(Cm[0], X0, yi2[B - 1]));
+                const float f1 = fmaf(Cm[3], X1, fmaf(Cm[2], X0, yi1[B - 1]));
+                X0 = f0;
+                X1 = f1;
+            }
+        }
+        /* (C) finalisation and back-correction */
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (long c = 0; c < nch; c++) {
+            float *reg = y + c * CH;
+            const float *yi2 = yi + (size_t)c * B * 2, *yi1 = yi2 + B;
+            const float X0 = xin[2 * c], X1 = xin[2 * c + 1];
+            for (int t = 0; t < B; t++) {
+                const float *Cm = Cs + (size_t)t * 4;
+                float *r = reg + (size_t)t * N;
+                r[N - 2] = fmaf(Cm[1], X1, fmaf(Cm[0], X0, yi2[t]));
+                r[N - 1] = fmaf(Cm[3], X1, fmaf(Cm[2], X0, yi1[t]));
+            }
+            for (int t = B - 1; t >= 0; t--) {
+                float c2, c1;
+                if (t == 0) { c2 = X0; c1 = X1; }
+                else { c2 = reg[(size_t)(t - 1) * N + N - 2]; c1 = reg[(size_t)(t - 1) * N + N - 1]; }
+                float *r = reg + (size_t)t * N;
+                const float *hh1 = h1 + (size_t)s * N, *hh2 = h2 + (size_t)s * N;
+                for (int k = 0; k < N - 2; k++) r[k] = fmaf(hh1[k], c1, fmaf(hh2[k], c2, r[k]));
+            }
+        }
+    }
+    free(h1); free(h2); free(Cx); free(yi); free(bnd); free(xin);
     return ORC_OK;
 }
 

@@ -32,6 +32,8 @@ def _load(path):
     L.orc_sosfilt_f64.argtypes = [_f64p, ctypes.c_int, _f64p, ctypes.c_long, _f64p]
     L.orc_golden_f64.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_long, _f64p]
     L.orc_ph_f32.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
+    L.orc_ph_f32_mt.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_long, _f32p,
+                                ctypes.c_int]
     L.orc_stcr_f32.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                _f32p, ctypes.c_long, _f32p]
     L.orc_dtcr_f32.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
@@ -39,7 +41,7 @@ def _load(path):
                               _f32p, ctypes.c_long, _f32p]
     L.orc_direct_form_f32.argtypes = [_f64p, _f64p, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
     L.orc_cascade_f32.argtypes = [_f32p, ctypes.c_int, _f32p, ctypes.c_long, _f32p]
-    for f in (L.orc_ph_f32, L.orc_stcr_f32, L.orc_dtcr_f32, L.orc_plr_f32):
+    for f in (L.orc_ph_f32, L.orc_ph_f32_mt, L.orc_stcr_f32, L.orc_dtcr_f32, L.orc_plr_f32):
         f.restype = ctypes.c_int
     return L
 
@@ -86,6 +88,13 @@ def _check(rc, what):
 def ph(sos, x, B, N, double_round=False):
     sos = _f32(sos); x = _f32(x); y = np.empty_like(x)
     _check(lib(double_round).orc_ph_f32(sos.reshape(-1), sos.shape[0], B, N, x, x.size, y), "ph")
+    return y
+
+
+def ph_mt(sos, x, B, N, threads):
+    """orc_ph_f32 on `threads` host threads (bit-identical output)."""
+    sos = _f32(sos); x = _f32(x); y = np.empty_like(x)
+    _check(lib().orc_ph_f32_mt(sos.reshape(-1), sos.shape[0], B, N, x, x.size, y, int(threads)), "ph_mt")
     return y
 
 

@@ -206,3 +206,16 @@ def test_plr_restatement_exact_on_well_conditioned_direct_form(order):
     for B, X in ((64, 1), (64, 3), (128, 5), (512, 2)):
         y = O.plr(bt, at, x, B, X)
         assert np.max(np.abs(y - ref)) < 1e-5 * np.max(np.abs(ref)), (B, X)
+
+
+@pytest.mark.parametrize("order", [2, 8, 16])
+@pytest.mark.parametrize("B,N", [(32, 64), (64, 32)])
+def test_ph_multithreaded_restatement_is_identical(order, B, N):
+    """orc_ph_f32_mt (the all-cores CPU baseline) runs the same float
+    operations as orc_ph_f32 with the chunks split over threads: equal bit
+    for bit, for any thread count."""
+    sos, g = iirx.butterworth_sos(order, 0.1)
+    x = (g * iirx.fill_normal(3, 37 * B * N).astype(np.float64)).astype(np.float32)
+    want = O.ph(sos, x, B, N)
+    for threads in (1, 3, 8):
+        assert np.array_equal(O.ph_mt(sos, x, B, N, threads), want)
