@@ -74,6 +74,18 @@ def butterworth_sos(order, wn):
     return sos, g.value
 
 
+def chebyshev_sos(kind, order, db, wn):
+    """Chebyshev lowpass as b0-normalised float32 sections and the float64 gain:
+    kind 1 = scipy.signal.cheby1(order, db, wn, output='sos') (passband ripple
+    db), kind 2 = cheby2 (stopband attenuation db), then normalize_sos."""
+    ns = order // 2
+    sos = np.zeros((ns, 6), np.float32)
+    g = ctypes.c_double()
+    check(lib().iirx_chebyshev_sos(int(kind), int(order), float(db), float(wn), sos.ctypes.data, ctypes.byref(g)),
+          "chebyshev")
+    return sos, g.value
+
+
 def normalize_sos(sos):
     """General sections (rows b0 b1 b2 a0 a1 a2, float64, any scaling; scipy's
     cheby1/ellip/... output) into the filter contract: (float32 rows with

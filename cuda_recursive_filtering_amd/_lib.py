@@ -134,6 +134,7 @@ _SIGS = {
     "iirx_measure": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P]),
     "iirx_version": (ctypes.c_char_p, []),
     "iirx_butterworth_sos": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, _P, _P]),
+    "iirx_chebyshev_sos": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P, _P]),
     "iirx_direct_form_taps": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
     "iirx_fill_normal": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, _P]),
     "iirx_plan_tables": (ctypes.c_int, [ctypes.POINTER(Config), _P, ctypes.c_int64, _P, ctypes.c_int64,

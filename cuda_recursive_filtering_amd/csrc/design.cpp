@@ -9,6 +9,10 @@
 // does (poles farthest from the unit circle first), then the b0 == 1
 // normalisation of ref_generate.py:63-67 with the removed gain returned.
 //
+// iirx_chebyshev_sos restates scipy.signal.cheby1 / cheby2 in the same way
+// (no reference counterpart; iir_utils.hpp:27-141 holds the reference's
+// unused tf2sos route to sections).
+//
 // iirx_fill_normal is the documented N(0,1) generator of the synthetic
 // workloads: SplitMix64 (Steele, Lea & Flood 2014) drives the polar-free
 // Box-Muller transform in float64; sample 2i is r cos(theta), sample 2i+1 is
@@ -60,6 +64,112 @@ double butterworth_sections(int order, double wn, std::vector<double> &a12)
         g *= (1.0 + a1 + a2) / 4.0;  // unit DC gain per section
     }
     return g;
+}
+
+// Chebyshev lowpass of even `order` (type 1: passband ripple `db`; type 2:
+// stopband attenuation `db`) at `wn`, as scipy.signal.cheby1 / cheby2(order,
+// db, wn, output='sos') compute it: the analog prototype (cheb1ap: poles
+// -sinh(mu + i theta_m), mu = asinh(1 / eps) / order, eps = sqrt(10^(db/10) -
+// 1), gain prod(-p) / sqrt(1 + eps^2); cheb2ap: zeros i / sin(theta_m) and
+// poles 1 / (sinh(mu) Re q + i cosh(mu) Im q) of the Butterworth q, mu =
+// asinh(sqrt(10^(db/10) - 1)) / order, gain prod(-p) / prod(-z)), scaled by
+// the pre-warped cutoff 4 tan(pi wn / 2), the bilinear map z = (4 + s) /
+// (4 - s) with the gain times prod(4 - z_s) / prod(4 - p_s), then zpk2sos's
+// 'nearest' pairing (the pole closest to the unit circle goes last, each with
+// the zero nearest to it) and the gain in the first section.  Rows are FP64
+// {b0, b1, b2, 1, a1, a2}; the caller normalises them.
+void chebyshev_rows(int type, int order, double db, double wn, std::vector<double> &rows)
+{
+    typedef std::complex<double> cd;
+    const double pi = 3.14159265358979323846264338327950288;
+    const double wo = 4.0 * std::tan(pi * wn / 2.0);
+    std::vector<cd> za, pa;
+    double k;
+    if (type == 1) {
+        const double eps = std::sqrt(std::pow(10.0, 0.1 * db) - 1.0);
+        const double mu = 1.0 / order * std::asinh(1.0 / eps);
+        for (int m = -order + 1; m < order; m += 2) pa.push_back(-std::sinh(cd(mu, pi * m / (2.0 * order))));
+        cd prod = 1.0;
+        for (const cd &q : pa) prod *= -q;
+        k = prod.real() / std::sqrt(1.0 + eps * eps);
+    } else {
+        const double de = 1.0 / std::sqrt(std::pow(10.0, 0.1 * db) - 1.0);
+        const double mu = std::asinh(1.0 / de) / order;
+        for (int m = -order + 1; m < order; m += 2) za.push_back(-std::conj(cd(0.0, 1.0) / std::sin(m * pi / (2.0 * order))));
+        for (int m = -order + 1; m < order; m += 2) {
+            const cd q = -std::exp(cd(0.0, pi * m / (2 * order)));
+            pa.push_back(1.0 / cd(std::sinh(mu) * q.real(), std::cosh(mu) * q.imag()));
+        }
+        cd pp = 1.0, pz = 1.0;
+        for (const cd &q : pa) pp *= -q;
+        for (const cd &q : za) pz *= -q;
+        k = (pp / pz).real();
+    }
+    // lp2lp_zpk and bilinear_zpk (fs = 2)
+    for (cd &q : za) q *= wo;
+    for (cd &q : pa) q *= wo;
+    k *= std::pow(wo, (double)(pa.size() - za.size()));
+    cd nz = 1.0, np_ = 1.0;
+    for (const cd &q : za) nz *= 4.0 - q;
+    for (const cd &q : pa) np_ *= 4.0 - q;
+    k *= (nz / np_).real();
+    std::vector<cd> zd, pd;
+    for (const cd &q : za) zd.push_back((4.0 + q) / (4.0 - q));
+    for (size_t i = za.size(); i < pa.size(); ++i) zd.push_back(-1.0);
+    for (const cd &q : pa) pd.push_back((4.0 + q) / (4.0 - q));
+    // one of each conjugate pair (positive imaginary part); real zeros kept
+    std::vector<cd> zc, zr, pc;
+    for (const cd &q : zd) {
+        if (q.imag() > 0.0) zc.push_back(q);
+        else if (q.imag() == 0.0) zr.push_back(q);
+    }
+    for (const cd &q : pd)
+        if (q.imag() > 0.0) pc.push_back(q);
+    std::vector<cd> z = zc;
+    z.insert(z.end(), zr.begin(), zr.end());
+    const int ns = order / 2;
+    rows.assign(6 * (size_t)ns, 0.0);
+    for (int si = ns - 1; si >= 0; --si) {
+        // the pole closest to the unit circle (first of equals)
+        size_t pi1 = 0;
+        for (size_t i = 1; i < pc.size(); ++i)
+            if (std::fabs(1.0 - std::abs(pc[i])) < std::fabs(1.0 - std::abs(pc[pi1]))) pi1 = i;
+        const cd p1 = pc[pi1];
+        pc.erase(pc.begin() + (long)pi1);
+        // the nearest zero (first of equals); a real zero takes the next real one
+        size_t zi1 = 0;
+        for (size_t i = 1; i < z.size(); ++i)
+            if (std::abs(z[i] - p1) < std::abs(z[zi1] - p1)) zi1 = i;
+        const cd z1 = z[zi1];
+        z.erase(z.begin() + (long)zi1);
+        double b[3];
+        if (z1.imag() != 0.0) {
+            b[0] = 1.0;
+            b[1] = -(z1 + std::conj(z1)).real();
+            b[2] = (z1 * std::conj(z1)).real();
+        } else {
+            size_t zi2 = 0;
+            double best = INFINITY;
+            for (size_t i = 0; i < z.size(); ++i)
+                if (z[i].imag() == 0.0 && std::abs(z[i] - p1) < best) {
+                    best = std::abs(z[i] - p1);
+                    zi2 = i;
+                }
+            const double z2 = z[zi2].real();
+            z.erase(z.begin() + (long)zi2);
+            b[0] = 1.0;
+            b[1] = -(z1.real() + z2);
+            b[2] = z1.real() * z2;
+        }
+        double *r = rows.data() + 6 * si;
+        r[0] = b[0];
+        r[1] = b[1];
+        r[2] = b[2];
+        r[3] = 1.0;
+        r[4] = -(p1 + std::conj(p1)).real();
+        r[5] = (p1 * std::conj(p1)).real();
+    }
+    for (int j = 0; j < 3; ++j) rows[(size_t)j] *= k;
 }
 
 // Direct form of b0 = a0 = 1 FP64 rows: the polynomial products of the
@@ -144,6 +254,19 @@ int iirx_butterworth_sos(int order, double wn, float *sos, double *gain)
     }
     if (gain) *gain = g;
     return IIRX_OK;
+}
+
+int iirx_chebyshev_sos(int type, int order, double db, double wn, float *sos, double *gain)
+{
+    if ((type != 1 && type != 2) || order < 2 || order > 16 || (order & 1) || !(db > 0.0 && db < 400.0) ||
+        !(wn > 0.0 && wn < 1.0) || !sos)
+        return IIRX_ERR_INVALID;
+    std::vector<double> rows;
+    chebyshev_rows(type, order, db, wn, rows);
+    double g = 0.0;
+    const int st = iirx_normalize_sos(rows.data(), order / 2, sos, &g);
+    if (st == IIRX_OK && gain) *gain = g;
+    return st;
 }
 
 int iirx_direct_form_taps(const float *sos, int ns, double *b, double *a)

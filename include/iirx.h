@@ -289,6 +289,17 @@ const char *iirx_version(void);
  */
 int iirx_butterworth_sos(int order, double wn, float *sos, double *gain);
 
+/*
+ * Chebyshev lowpass of even `order` (2..16) at cutoff `wn` (x Nyquist), as
+ * scipy.signal.cheby1(order, db, wn, output='sos') (type 1, passband ripple
+ * db) or cheby2(order, db, wn, output='sos') (type 2, stopband attenuation
+ * db) followed by iirx_normalize_sos: b0-normalised float32 rows and the
+ * removed gain.  No counterpart in the reference, which designs only
+ * Butterworth filters (ref_generate.py:62); its iir_utils.hpp:27-141 holds an
+ * unused route from direct-form taps to sections.
+ */
+int iirx_chebyshev_sos(int type, int order, double db, double wn, float *sos, double *gain);
+
 /* General sections into the filter contract: rows {b0, b1, b2, a0, a1, a2}
  * (float64, any scaling, b2 = a2 = 0 for a first-order section) become
  * float32 rows {1, b1/b0, b2/b0, 1, a1/a0, a2/a0} and *gain = prod(b0/a0),

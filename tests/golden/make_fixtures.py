@@ -8,6 +8,8 @@ Imports scipy only (not the reference).  Contents:
   x_ref_o{order}          reference workload input, 3 chunks of 32x32:
                           f32(gain * cos(2 pi 120 i / 1000))      (ref_generate.py:84-86)
   y_ref_o{order}          sosfilt(f64(sos_f32), f64(x_ref))      (ref_generate.py:89)
+  cheby{k}_o{order}_d{db}_w{wn}  cheby1 (k = 1, ripple db) / cheby2 (k = 2,
+                          attenuation db)(order, db, wn, 'sos') float64
 Run: python tests/golden/make_fixtures.py
 """
 import os
@@ -45,6 +47,11 @@ def main():
         x = (g * np.cos(2.0 * np.pi * 120.0 * i / 1000.0)).astype(np.float32)
         d[f"x_ref_o{o}"] = x
         d[f"y_ref_o{o}"] = signal.sosfilt(sos32.astype(np.float64), x.astype(np.float64))
+    for k, design, dbs in ((1, signal.cheby1, (0.5, 3.0)), (2, signal.cheby2, (40.0, 80.0))):
+        for o in ORDERS:
+            for db in dbs:
+                for wn in (0.02, 0.1, 0.3):
+                    d[f"cheby{k}_o{o}_d{db}_w{wn}"] = design(o, db, wn, output="sos")
     np.savez_compressed(OUT, **d)
     print("wrote", OUT)
 

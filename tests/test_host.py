@@ -51,6 +51,33 @@ def test_butterworth_matches_scipy_bit_exact_float32(order, wn):
     assert abs(g - g_ref) <= 1e-12 * g_ref
 
 
+@pytest.mark.parametrize("kind,db", [(1, 0.5), (1, 3.0), (2, 40.0), (2, 80.0)])
+@pytest.mark.parametrize("order", [2, 4, 8, 16])
+@pytest.mark.parametrize("wn", [0.02, 0.1, 0.3])
+def test_chebyshev_matches_scipy_bit_exact_float32(kind, db, order, wn):
+    """iirx_chebyshev_sos against frozen scipy 1.15.3 cheby1 / cheby2 designs
+    (tests/golden/scipy_fixtures.npz): the float32 b0-normalised rows bit for
+    bit, in scipy's section order, the gain to FP64 rounding; and the
+    designed filter has the nominal response (type 1: ripple db in the
+    passband, unit peak; type 2: at least db attenuation past wn)."""
+    ref = FIX[f"cheby{kind}_o{order}_d{db}_w{wn}"]
+    want, g_want = iirx.normalize_sos(ref)
+    sos, g = iirx.chebyshev_sos(kind, order, db, wn)
+    assert np.array_equal(sos, want)
+    assert abs(g - g_want) <= 1e-12 * abs(g_want)
+    full = sos.astype(np.float64)
+    full[0, :3] *= g
+    w, h = signal.sosfreqz(full, worN=4096)
+    mag = np.abs(h)
+    if kind == 1:
+        pb = mag[w <= 0.98 * np.pi * wn]
+        assert np.max(pb) == pytest.approx(1.0, abs=1e-3)
+        assert np.min(pb) >= 10 ** (-db / 20) - 1e-3
+    else:
+        sb = mag[w >= 1.02 * np.pi * wn]
+        assert np.max(sb) <= 10 ** (-db / 20) * 1.01
+
+
 @pytest.mark.parametrize("order", [2, 4, 8, 16])
 def test_direct_form_taps_match_scipy_ba(order):
     b, a = FIX[f"ba_b_o{order}_w0.2"], FIX[f"ba_a_o{order}_w0.2"]
