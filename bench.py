@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the all-cores CPU baseline (0: all)")
     ap.add_argument("--bs-geom", default="64x64",
                     help="block-state form geometry GxL for the MFMA vs plain fp32 comparison (orders 8 and 16)")
+    ap.add_argument("--bs-forms", default="BS:64x64,BS:16x128,BS_BF16:32x64,BS_BF16:16x128",
+                    help="block-state forms (kernel:GxL) timed beside the plain fp32 kernel at orders 8 and 16: "
+                         "BS = FP16 hi/lo split, BS_BF16 = BF16 hi/mid/lo split")
     ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA vs plain fp32 comparison")
     ap.add_argument("--workload", default="single", choices=["single", "ragged", "sweep"],
                     help="single: one 2^25-sample channel per order; ragged: the 4096-channel batch; "
@@ -525,17 +528,24 @@ def main():
                 d = out.cpu().numpy().astype(np.float64) - refm
                 return float(np.max(np.abs(d))) / rms, float(np.sqrt(np.mean(d ** 2))) / rms
             e_fp32, r_fp32 = errs(r["plan"])
-            rb = time_order(iirx, torch, order, args, bufs, out, stream, kernel="BS", geom=args.bs_geom)
-            e_bs, r_bs = errs(rb["plan"])
-            ok_bs, rel_bs = gate(out.cpu().numpy(), refm)
-            rb["plan"].close()
+            forms = []
+            for spec in [f"BS:{args.bs_geom}"] + [f for f in args.bs_forms.split(",") if f and f != f"BS:{args.bs_geom}"]:
+                kname, geom = spec.split(":")
+                rb = time_order(iirx, torch, order, args, bufs, out, stream, kernel=kname, geom=geom)
+                e_bs, r_bs = errs(rb["plan"])
+                ok_bs, rel_bs = gate(out.cpu().numpy(), refm)
+                rb["plan"].close()
+                forms.append({"kernel": kname, "split": "bf16 hi/mid/lo" if kname == "BS_BF16" else "fp16 hi/lo",
+                              "geometry": geom, "gsamples_per_s": round(N_SAMPLES / (rb["ms"] * 1e-3) / 1e9, 3),
+                              "max_err_over_rms": e_bs, "rms_err_over_rms": r_bs, "gate_pass": bool(ok_bs),
+                              "gate_rel": rel_bs})
+            f0 = forms[0]
             mfma[str(order)] = {
-                "bs_gsamples_per_s": round(N_SAMPLES / (rb["ms"] * 1e-3) / 1e9, 3), "bs_geometry": args.bs_geom,
-                "bs_split": "fp16 hi/lo",
+                "bs_gsamples_per_s": f0["gsamples_per_s"], "bs_geometry": f0["geometry"], "bs_split": f0["split"],
                 "fp32_gsamples_per_s": entry["gsamples_per_s"], "fp32_kernel": f"{args.kernel} {args.geom}",
-                "bs_max_err_over_rms": e_bs, "fp32_max_err_over_rms": e_fp32,
-                "bs_rms_err_over_rms": r_bs, "fp32_rms_err_over_rms": r_fp32,
-                "bs_gate_pass": bool(ok_bs), "bs_gate_rel": rel_bs}
+                "bs_max_err_over_rms": f0["max_err_over_rms"], "fp32_max_err_over_rms": e_fp32,
+                "bs_rms_err_over_rms": f0["rms_err_over_rms"], "fp32_rms_err_over_rms": r_fp32,
+                "bs_gate_pass": f0["gate_pass"], "bs_gate_rel": f0["gate_rel"], "forms": forms}
             print(f"# mfma order {order}: {mfma[str(order)]}", file=sys.stderr, flush=True)
         if args.geom_steps > 0:
             entry["reference_geometries"] = reference_geometries(iirx, torch, order, args, bufs, out, stream, ref)

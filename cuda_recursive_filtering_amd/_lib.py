@@ -21,7 +21,7 @@ IIRX_ERR_TOO_LARGE = 8
 
 SCHEDULES = {"persistent": 0, "ticket": 1}
 
-KERNELS = {"PH": 0, "STCR": 1, "DTCR": 2, "PLR": 3, "BS": 4}
+KERNELS = {"PH": 0, "STCR": 1, "DTCR": 2, "PLR": 3, "BS": 4, "BS_BF16": 5}
 
 
 class Config(ctypes.Structure):

@@ -20,9 +20,11 @@ namespace iirx {
 constexpr int kBsSlots = 16;  // lookback partials folded at most (LDS: 16 x 16 floats)
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ f4 mfma_h(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma_b(bf8 a, bf8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ f4 mfma_f(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 // FP64 matrix core: lane l, register r of the result holds row l/16 + 4 r
 // (the FP32 shapes hold row 4 (l/16) + r)
@@ -64,6 +66,43 @@ __device__ __forceinline__ void split8(const f4 &a, const f4 &b, int s, h8 &hi, 
         hi[j] = h;
         lo[j] = (_Float16)(x - (float)h);
     }
+}
+
+// BF16 hi/mid/lo split of (a, b) * 2^s, the 8 K rows of one x32 B operand:
+// each component the remainder so far rounded to nearest
+__device__ __forceinline__ void split8_bf(const f4 &a, const f4 &b, int s, bf8 &hi, bf8 &mid, bf8 &lo)
+{
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float x = ldexpf(j < 4 ? a[j & 3] : b[j & 3], s);
+        const __bf16 h = (__bf16)x;
+        const float r1 = x - (float)h;
+        const __bf16 m = (__bf16)r1;
+        const float r2 = r1 - (float)m;
+        hi[j] = h;
+        mid[j] = m;
+        lo[j] = (__bf16)r2;
+    }
+}
+
+// acc + A X over the split: FP16 hi/lo (lo hi + hi lo + hi hi) or BF16
+// hi/mid/lo (the six products above 2^-24 of hi hi, smallest first), A's
+// planes from `a(hl)`, X the split of the raw samples
+template <int SPLIT, class AOp>
+__device__ __forceinline__ f4 split_product(const AOp &a, const f4 &xa, const f4 &xb, int eX, f4 acc)
+{
+    if constexpr (SPLIT == kBsSplitBF16) {
+        bf8 xh, xm, xl;
+        split8_bf(xa, xb, eX, xh, xm, xl);
+        const bf8 ah = __builtin_bit_cast(bf8, a(0)), am = __builtin_bit_cast(bf8, a(1)), al = __builtin_bit_cast(bf8, a(2));
+        acc = mfma_b(al, xh, acc);
+        acc = mfma_b(ah, xl, acc);
+        acc = mfma_b(am, xm, acc);
+        acc = mfma_b(am, xh, acc);
+        acc = mfma_b(ah, xm, acc);
+        acc = mfma_b(ah, xh, acc);
+    }
+    return acc;
 }
 
 // acc + M S over the first NSL K slabs; M is this lane's float4 of an FP32
@@ -156,7 +195,7 @@ struct BsRaw {
     f4 a[NB][KQ], b[NB][KQ];
 };
 
-template <int L, int G, int NSL, class Ops, class Store>
+template <int L, int G, int NSL, int SPLIT = kBsSplitF16, class Ops, class Store>
 __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32_t chunk, uint32_t part_flag,
                                          uint32_t full_flag, uint64_t *lb, const float *Mc, int depth, float *lp,
                                          uint32_t spin_limit, uint32_t *err, uint32_t first, int sT, int sG,
@@ -168,7 +207,9 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
     constexpr int NB = G / 16;  // column blocks of 16 tiles
     const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
 
-    // 1. per-chunk scale and FP16 hi/lo split of X
+    // 1. per-chunk power-of-two scale and (FP16) hi/lo split of X; the BF16
+    //    three-way split is formed at each use from the raw samples instead
+    //    (its 12 registers per operand would not all fit)
     h8 xh[NB][KQ], xl[NB][KQ];
     int eX;
     {
@@ -180,10 +221,12 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
 #pragma unroll
                 for (int j = 0; j < 4; ++j) mx = fmaxf(mx, fmaxf(fabsf(xr.a[nb][q][j]), fabsf(xr.b[nb][q][j])));
         eX = scale_exp(wave_max(mx));
+        if constexpr (SPLIT == kBsSplitF16) {
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
+            for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-            for (int q = 0; q < KQ; ++q) split8(xr.a[nb][q], xr.b[nb][q], eX, xh[nb][q], xl[nb][q]);
+                for (int q = 0; q < KQ; ++q) split8(xr.a[nb][q], xr.b[nb][q], eX, xh[nb][q], xl[nb][q]);
+        }
     }
 
     // 2. V = Gam X (zero chunk-incoming state); Gam's rows are ordered so that
@@ -195,10 +238,14 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
         f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int q = 0; q < KQ; ++q) {
-            const h8 gh = ops.gam(0, q), gl = ops.gam(1, q);
-            acc = mfma_h(gl, xh[nb][q], acc);
-            acc = mfma_h(gh, xl[nb][q], acc);
-            acc = mfma_h(gh, xh[nb][q], acc);
+            if constexpr (SPLIT == kBsSplitF16) {
+                const h8 gh = ops.gam(0, q), gl = ops.gam(1, q);
+                acc = mfma_h(gl, xh[nb][q], acc);
+                acc = mfma_h(gh, xl[nb][q], acc);
+                acc = mfma_h(gh, xh[nb][q], acc);
+            } else {
+                acc = split_product<SPLIT>([&](int hl) { return ops.gam(hl, q); }, xr.a[nb][q], xr.b[nb][q], eX, acc);
+            }
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) Z[nb][j] = ldexp((double)acc[j], -(sG + eX));
@@ -225,23 +272,59 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
     // the predecessor's record is in flight while Y0 = T X runs on the matrix cores
     const uint64_t poll0 = (chunk > first) ? lookback_first_poll<16>(lb, 32, chunk) : 0ull;
     f4 Y0[MB][NB];
+    if constexpr (SPLIT == kBsSplitF16) {
 #pragma unroll
-    for (int mb = 0; mb < MB; ++mb) {
-        const int qmax = bs_tq(mb);  // T(16 mb + r, 32 q + c) = 0 beyond
+        for (int mb = 0; mb < MB; ++mb) {
+            const int qmax = bs_tq(mb);  // T(16 mb + r, 32 q + c) = 0 beyond
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (int nb = 0; nb < NB; ++nb) {
+                f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int q = 0; q < KQ; ++q) {
+                    if (q > qmax) break;
+                    const h8 th = ops.tmat(0, mb, q), tl = ops.tmat(1, mb, q);
+                    acc = mfma_h(tl, xh[nb][q], acc);
+                    acc = mfma_h(th, xl[nb][q], acc);
+                    acc = mfma_h(th, xh[nb][q], acc);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) Y0[mb][nb][j] = ldexpf(acc[j], -(sT + eX));
+            }
+        }
+    } else {
+        // slab by slab, so that each slab's split is formed once for all row blocks
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) Y0[mb][nb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
             for (int q = 0; q < KQ; ++q) {
-                if (q > qmax) break;
-                const h8 th = ops.tmat(0, mb, q), tl = ops.tmat(1, mb, q);
-                acc = mfma_h(tl, xh[nb][q], acc);
-                acc = mfma_h(th, xl[nb][q], acc);
-                acc = mfma_h(th, xh[nb][q], acc);
+                bf8 xh3, xm3, xl3;
+                split8_bf(xr.a[nb][q], xr.b[nb][q], eX, xh3, xm3, xl3);
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb) {
+                    if (q > bs_tq(mb)) continue;
+                    const bf8 th = __builtin_bit_cast(bf8, ops.tmat(0, mb, q));
+                    const bf8 tm = __builtin_bit_cast(bf8, ops.tmat(1, mb, q));
+                    const bf8 tl = __builtin_bit_cast(bf8, ops.tmat(2, mb, q));
+                    f4 acc = Y0[mb][nb];
+                    acc = mfma_b(tl, xh3, acc);
+                    acc = mfma_b(th, xl3, acc);
+                    acc = mfma_b(tm, xm3, acc);
+                    acc = mfma_b(tm, xh3, acc);
+                    acc = mfma_b(th, xm3, acc);
+                    acc = mfma_b(th, xh3, acc);
+                    Y0[mb][nb] = acc;
+                }
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) Y0[mb][nb][j] = ldexpf(acc[j], -(sT + eX));
-        }
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) Y0[mb][nb][j] = ldexpf(Y0[mb][nb][j], -(sT + eX));
     }
 
     const float Xj = lookbackK<16, kBsSlots>(lb, 32, chunk, part_flag, full_flag, Mc, depth, lp, spin_limit, err,

@@ -56,11 +56,11 @@ __device__ double sqrt_rn(double w)
 
 __device__ uint16_t half_bits_d(_Float16 h) { return __builtin_bit_cast(uint16_t, h); }
 
-template <int L>
+template <int L, int SPLIT>
 __global__ __launch_bounds__(kSetupThreads) void bs_setup_kernel(const BsSetupJob *jobs, float *out_base, int G)
 {
 #pragma clang fp contract(off)
-    constexpr BsLayout Y = bs_layout(L);
+    constexpr BsLayout Y = bs_layout(L, SPLIT);
     const BsSetupJob &job = jobs[blockIdx.x];
     float *out = out_base + job.out_off;
     const int ns = job.ns, p = 2 * ns, t = threadIdx.x;
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kSetupThreads) void bs_setup_kernel(const BsSetupJo
     // put_block32: lane l, half j -> row r0 + l % 16, column 32 q + (j < 4 ?
     // 4 g + j : 16 + 4 g + j - 4), g = l / 16
     {
-        constexpr int nT = 2 * Y.MB * Y.KQ, nBlk = nT + 2 * Y.KQ;
+        constexpr int nT = SPLIT * Y.MB * Y.KQ, nBlk = nT + SPLIT * Y.KQ;
         for (int idx = t; idx < nBlk * 512; idx += kSetupThreads) {
             const int blk = idx >> 9, e = idx & 511, l = e >> 3, j = e & 7, g = l >> 4;
             int hl, r, q, s;
@@ -258,9 +258,13 @@ __global__ __launch_bounds__(kSetupThreads) void bs_setup_kernel(const BsSetupJo
             }
             // hi = v rounded once to FP16, lo = the exact remainder rounded once
             v = ldexp(v, s);
-            const _Float16 hi = (_Float16)v;
-            const _Float16 lo = (_Float16)(v - (double)hi);
-            dst[l * 8 + j] = half_bits_d(hl == 0 ? hi : lo);
+            if constexpr (SPLIT == kBsSplitBF16) {
+                dst[l * 8 + j] = bf16x3_bits(v, hl);
+            } else {
+                const _Float16 hi = (_Float16)v;
+                const _Float16 lo = (_Float16)(v - (double)hi);
+                dst[l * 8 + j] = half_bits_d(hl == 0 ? hi : lo);
+            }
         }
     }
     // FP32 lane-order blocks: word 4 l + k = M[r0 + l % 16][l / 16 + 4 k]
@@ -299,16 +303,24 @@ __global__ __launch_bounds__(kSetupThreads) void bs_setup_kernel(const BsSetupJo
 
 }  // namespace
 
-hipError_t launch_bs_setup(int L, int G, const BsSetupJob *jobs, int n_jobs, float *out, hipStream_t s)
+template <int SPLIT>
+static hipError_t launch_setup(int L, int G, const BsSetupJob *jobs, int n_jobs, float *out, hipStream_t s)
 {
-    if (n_jobs <= 0) return hipSuccess;
     switch (L) {
-    case 32: hipLaunchKernelGGL(bs_setup_kernel<32>, dim3(n_jobs), dim3(kSetupThreads), 0, s, jobs, out, G); break;
-    case 64: hipLaunchKernelGGL(bs_setup_kernel<64>, dim3(n_jobs), dim3(kSetupThreads), 0, s, jobs, out, G); break;
-    case 128: hipLaunchKernelGGL(bs_setup_kernel<128>, dim3(n_jobs), dim3(kSetupThreads), 0, s, jobs, out, G); break;
+    case 32: hipLaunchKernelGGL((bs_setup_kernel<32, SPLIT>), dim3(n_jobs), dim3(kSetupThreads), 0, s, jobs, out, G); break;
+    case 64: hipLaunchKernelGGL((bs_setup_kernel<64, SPLIT>), dim3(n_jobs), dim3(kSetupThreads), 0, s, jobs, out, G); break;
+    case 128: hipLaunchKernelGGL((bs_setup_kernel<128, SPLIT>), dim3(n_jobs), dim3(kSetupThreads), 0, s, jobs, out, G); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_bs_setup(int L, int G, const BsSetupJob *jobs, int n_jobs, float *out, hipStream_t s, int split)
+{
+    if (n_jobs <= 0) return hipSuccess;
+    if (split == kBsSplitBF16) return launch_setup<kBsSplitBF16>(L, G, jobs, n_jobs, out, s);
+    if (split == kBsSplitF16) return launch_setup<kBsSplitF16>(L, G, jobs, n_jobs, out, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace iirx

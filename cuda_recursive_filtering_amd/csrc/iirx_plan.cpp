@@ -81,6 +81,10 @@ namespace {
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
+// the block-state form and its operand split
+bool is_bs(int kernel) { return kernel == IIRX_KERNEL_BLOCKSTATE || kernel == IIRX_KERNEL_BLOCKSTATE_BF16; }
+int bs_split_of(int kernel) { return kernel == IIRX_KERNEL_BLOCKSTATE_BF16 ? kBsSplitBF16 : kBsSplitF16; }
+
 // rows of the filter contract: finite, b0 == 1 and a0 == 1 (iirx_normalize_sos makes them)
 bool sos_ok(const float *sos, int ns)
 {
@@ -114,10 +118,12 @@ int validate(const iirx_config *cfg, int64_t n)
         if ((n + chunk - 1) / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
         return IIRX_OK;
     }
-    case IIRX_KERNEL_BLOCKSTATE: {
+    case IIRX_KERNEL_BLOCKSTATE:
+    case IIRX_KERNEL_BLOCKSTATE_BF16: {
         const int L = cfg->n_blocks, G = cfg->block_size;
         if (!(L == 32 || L == 64 || L == 128) || !(G == 16 || G == 32 || G == 64) || (L == 128 && G == 64))
             return IIRX_ERR_GEOMETRY;
+        if (cfg->kernel == IIRX_KERNEL_BLOCKSTATE_BF16 && (L == 32 || G == 64)) return IIRX_ERR_GEOMETRY;
         // any length: the samples past the last whole chunk go through PH (below)
         const int64_t chunk = (int64_t)G * L;
         if ((n + chunk - 1) / chunk > 0x7fffffffLL) return IIRX_ERR_GEOMETRY;
@@ -191,8 +197,9 @@ int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double
         if (sh.coef_stride < 0) return IIRX_ERR_CONDITIONING;
         break;
     case IIRX_KERNEL_BLOCKSTATE:
-        sh.coef_stride = bs_tables(sos, ns, cfg->n_blocks, cfg->block_size, tables);
-        sh.chunks_per_block = bs_wg_waves(cfg->n_blocks, cfg->block_size);
+    case IIRX_KERNEL_BLOCKSTATE_BF16:
+        sh.coef_stride = bs_tables(sos, ns, cfg->n_blocks, cfg->block_size, tables, bs_split_of(cfg->kernel));
+        sh.chunks_per_block = bs_wg_waves(cfg->n_blocks, cfg->block_size, bs_split_of(cfg->kernel));
         sh.threads = 64 * sh.chunks_per_block;
         sh.chunk = (int64_t)cfg->block_size * cfg->n_blocks;
         sh.num_chunks = (uint32_t)((n + sh.chunk - 1) / sh.chunk);
@@ -245,7 +252,9 @@ const void *kernel_fn(const iirx_config *cfg, const Shape &sh)
     case IIRX_KERNEL_STCR: return kernel_stcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_DTCR: return kernel_dtcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_PLR: return kernel_plr(2 * cfg->n_sections, sh.plr_block, sh.plr_x);
-    case IIRX_KERNEL_BLOCKSTATE: return kernel_bs(cfg->n_blocks, cfg->block_size, cfg->n_sections);
+    case IIRX_KERNEL_BLOCKSTATE:
+    case IIRX_KERNEL_BLOCKSTATE_BF16:
+        return kernel_bs(cfg->n_blocks, cfg->block_size, cfg->n_sections, bs_split_of(cfg->kernel));
     default: return nullptr;
     }
 }
@@ -317,7 +326,7 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
 
     p->d_bsmap = nullptr;
     p->split_last = (cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR ||
-                     cfg->kernel == IIRX_KERNEL_DTCR || cfg->kernel == IIRX_KERNEL_BLOCKSTATE) &&
+                     cfg->kernel == IIRX_KERNEL_DTCR || is_bs(cfg->kernel)) &&
                     n_samples % sh.chunk != 0;
     p->main_chunks = p->num_chunks - (p->split_last ? 1u : 0u);
     p->d_aux = nullptr;
@@ -339,7 +348,7 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     p->d_epoch = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(p->d_ticket) + 8);
     p->d_err = p->d_epoch + 1;
     p->d_arrive = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(p->d_ticket) + 128);
-    if (cfg->kernel == IIRX_KERNEL_BLOCKSTATE) {
+    if (is_bs(cfg->kernel)) {
         // the block-state tables are computed in FP64 on the GPU (bs_setup.hip)
         BsSetupJob job{};
         std::copy(p->sos.begin(), p->sos.end(), job.sos);
@@ -347,7 +356,8 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
         BsSetupJob *d_job = nullptr;
         if (hipMalloc((void **)&d_job, sizeof job) != hipSuccess) return fail(IIRX_ERR_OOM);
         const bool ok = hipMemcpy(d_job, &job, sizeof job, hipMemcpyHostToDevice) == hipSuccess &&
-                        launch_bs_setup(cfg->n_blocks, cfg->block_size, d_job, 1, p->d_coef, nullptr) == hipSuccess &&
+                        launch_bs_setup(cfg->n_blocks, cfg->block_size, d_job, 1, p->d_coef, nullptr,
+                                        bs_split_of(cfg->kernel)) == hipSuccess &&
                         hipDeviceSynchronize() == hipSuccess;
         hipFree(d_job);
         if (!ok) return fail(IIRX_ERR_HIP);
@@ -476,7 +486,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
     a.last_valid = (int32_t)(p->n_samples - (int64_t)p->main_chunks * p->chunk);
     hipError_t e = hipSuccess;
     float *bs_z0 = nullptr, *bs_tail = nullptr;
-    if (p->cfg.kernel == IIRX_KERNEL_BLOCKSTATE && p->d_bsmap) {
+    if (is_bs(p->cfg.kernel) && p->d_bsmap) {
         // the block-state kernel takes the balanced state; it writes the state
         // entering its last tile for the histories out
         bs_z0 = reinterpret_cast<float *>(p->d_bsmap + 1);
@@ -509,8 +519,11 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
         break;
     }
     case IIRX_KERNEL_PLR: e = launch_plr(2 * p->cfg.n_sections, p->shape.plr_block, p->shape.plr_x, a, s); break;
-    case IIRX_KERNEL_BLOCKSTATE: {
-        e = p->main_chunks > 0 ? launch_bs(p->n_blocks, p->cfg.block_size, p->cfg.n_sections, a, s) : hipSuccess;
+    case IIRX_KERNEL_BLOCKSTATE:
+    case IIRX_KERNEL_BLOCKSTATE_BF16: {
+        e = p->main_chunks > 0 ? launch_bs(p->n_blocks, p->cfg.block_size, p->cfg.n_sections, a, s,
+                                           bs_split_of(p->cfg.kernel))
+                               : hipSuccess;
         if (e == hipSuccess && p->split_last) {
             // the partial last chunk: direct-form histories after the whole
             // chunks (the state-out map), then the PH last-chunk kernel from them
@@ -537,7 +550,7 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
     }
     if (e == hipSuccess && d_zf) {
         if (p->cfg.kernel == IIRX_KERNEL_PLR) e = launch_plr_end_state(a, d_zf, s);
-        else if (p->cfg.kernel == IIRX_KERNEL_BLOCKSTATE && !p->split_last)
+        else if (is_bs(p->cfg.kernel) && !p->split_last)
             e = launch_bs_state_out(d_in, p->n_samples, p->cfg.n_blocks, bs_tail, p->d_bsmap, d_zf, s);
         else if (!p->split_last) e = launch_ph_end_state(a, (int)p->chunk, d_zf, s);
     }

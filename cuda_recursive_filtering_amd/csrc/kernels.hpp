@@ -192,20 +192,27 @@ namespace iirx {
 //   [kO]  O row blocks          [kScale] exponents of T and Gam
 //   [kMc] chunk transition Phi^G (fp32 row major)    [kDepth] lookback depth
 // ---------------------------------------------------------------------------
+// The operand split of T and Gam: kBsSplitF16 = FP16 hi/lo (2 planes,
+// v_mfma_f32_16x16x32_f16), kBsSplitBF16 = BF16 hi/mid/lo (3 planes,
+// v_mfma_f32_16x16x32_bf16); plane hl of a block is component hl.
+constexpr int kBsSplitF16 = 2;
+constexpr int kBsSplitBF16 = 3;
+
 struct BsLayout {
-    int MB, KQ, kT, kG, kP, kQ, kPhi, kO, kScale, kMc, kDepth, stride;
+    int MB, KQ, split, kT, kG, kP, kQ, kPhi, kO, kScale, kMc, kDepth, stride;
 };
 
 constexpr int kBsBlockWords = 256;
 
-constexpr BsLayout bs_layout(int L)
+constexpr BsLayout bs_layout(int L, int split = kBsSplitF16)
 {
     BsLayout p{};
     p.MB = L / 16;
     p.KQ = L / 32;
+    p.split = split;
     p.kT = 0;
-    p.kG = p.kT + 2 * p.MB * p.KQ * kBsBlockWords;
-    p.kP = p.kG + 2 * p.KQ * kBsBlockWords;
+    p.kG = p.kT + split * p.MB * p.KQ * kBsBlockWords;
+    p.kP = p.kG + split * p.KQ * kBsBlockWords;
     p.kQ = p.kP + 6 * kBsBlockWords;
     p.kPhi = p.kQ + 3 * kBsBlockWords;
     p.kO = p.kPhi + kBsBlockWords;
@@ -240,17 +247,57 @@ constexpr int bs_tidx(int mb)
 // waves per SIMD the register budget is set for, and waves per workgroup
 // (the workgroup shares T, Gam and Phi^(2^j) in LDS; each wave runs its own
 // chunks with its own input buffer of G L floats and 16 lookback slots)
-constexpr int bs_waves(int L, int G)
+constexpr int bs_waves(int L, int G, int split = kBsSplitF16)
 {
+    // the BF16 split keeps the raw samples and forms the split at each use:
+    // one wave less at L = 64, so that it does not spill
+    if (split == kBsSplitBF16 && L == 64) return G == 32 ? 2 : 3;
     if (L == 128) return G == 16 ? 3 : 2;
     return G == 64 ? (L == 32 ? 3 : 2) : G == 32 ? (L == 32 ? 4 : 3) : (L == 32 ? 5 : 4);
 }
 // L = 128: T alone takes 40 KB of LDS, so the per-wave input buffers (8 KB
 // per 16 tiles) cap the workgroup at 11 waves (G = 16) or 6 (G = 32)
-constexpr int bs_wg_waves(int L, int G)
+constexpr int bs_wg_waves_f16(int L, int G)
 {
     if (L == 128) return G == 16 ? 11 : 6;
     return G == 64 ? (L == 32 ? 12 : 8) : G == 32 ? (L == 32 ? 8 : 12) : (L == 32 ? 10 : 16);
+}
+// LDS of a workgroup of w waves: T's nonzero blocks and Gam in `split`
+// planes, 6 Phi^(2^j) blocks, and per wave its input buffer (G L floats) and
+// 16 lookback slots of 16 floats
+constexpr int bs_lds_bytes(int L, int G, int split, int w)
+{
+    return 1024 * (split * (bs_tidx(L / 16) + L / 32) + 6) + w * (4 * G * L + 1024);
+}
+// the FP16 count, at most the waves per SIMD times 4, reduced until the
+// workgroup fits the 160 KB of LDS
+constexpr int bs_wg_waves(int L, int G, int split = kBsSplitF16)
+{
+    int w = bs_wg_waves_f16(L, G);
+    if (w > 4 * bs_waves(L, G, split)) w = 4 * bs_waves(L, G, split);
+    while (w > 1 && bs_lds_bytes(L, G, split, w) > 160 * 1024) --w;
+    return w;
+}
+
+// BF16 bits of a float, rounded to nearest even (finite values), and the
+// value of BF16 bits; the same integer steps on the host and the device
+__host__ __device__ inline uint16_t bf16_bits_rne(float f)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline double bf16_value(uint16_t b) { return (double)__builtin_bit_cast(float, (uint32_t)b << 16); }
+// component hl (0 hi, 1 mid, 2 lo) of the BF16 three-way split of v: each
+// the remainder so far (exact in FP64) rounded to float32, then to BF16
+__host__ __device__ inline uint16_t bf16x3_bits(double v, int hl)
+{
+    uint16_t b = 0;
+    for (int k = 0; k <= hl; ++k) {
+        b = bf16_bits_rne((float)v);
+        v -= bf16_value(b);
+    }
+    return b;
 }
 // scale slots: 0 T, 1 Gam
 
@@ -301,7 +348,7 @@ struct BatchArgs {
 hipError_t launch_batch(int algo, int n, const BatchArgs &a, hipStream_t s);
 const void *kernel_batch(int algo, int n);
 
-hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s);
+hipError_t launch_bs(int L, int G, int ns, const LaunchArgs &a, hipStream_t s, int split = kBsSplitF16);
 // end state of a PH, STCR or DTCR launch whose length is a multiple of the
 // chunk, in the layout of LaunchArgs::zi, from the last chunk's full records
 // and the last two input samples (a partial last chunk records it itself)
@@ -332,7 +379,8 @@ struct BsSetupJob {
     int32_t pad;
     int64_t out_off;  // floats from the output base
 };
-hipError_t launch_bs_setup(int L, int G, const BsSetupJob *jobs, int n_jobs, float *out, hipStream_t s);
-const void *kernel_bs(int L, int G, int ns);
+hipError_t launch_bs_setup(int L, int G, const BsSetupJob *jobs, int n_jobs, float *out, hipStream_t s,
+                           int split = kBsSplitF16);
+const void *kernel_bs(int L, int G, int ns, int split = kBsSplitF16);
 
 }  // namespace iirx

@@ -273,15 +273,20 @@ uint16_t half_bits(_Float16 h)
 // scaled by 2^s, hi (hl = 0) or lo (1) part.  With perm, operand row i holds
 // matrix row r0 + bs_state_of_row(i): the product's row 4 g + j then lands on
 // lane group g, register j as state g + 4 j, the FP64 MFMA's order.
-void put_block32(float *dst, const double *m, int rows, int cols, int r0, int q, int s, int hl, bool perm = false)
+void put_block32(float *dst, const double *m, int rows, int cols, int r0, int q, int s, int hl, bool perm = false,
+                 int split = kBsSplitF16)
 {
     uint16_t *h = reinterpret_cast<uint16_t *>(dst);
     for (int l = 0; l < 64; ++l)
         for (int j = 0; j < 8; ++j) {
             const int g = l / 16, r = r0 + (perm ? bs_state_of_row(l % 16) : l % 16);
             const int c = 32 * q + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
-            // hi = v rounded once to FP16, lo = the exact remainder rounded once
             const double v = (r < rows && c < cols) ? std::ldexp(m[(size_t)r * cols + c], s) : 0.0;
+            if (split == kBsSplitBF16) {
+                h[l * 8 + j] = bf16x3_bits(v, hl);
+                continue;
+            }
+            // hi = v rounded once to FP16, lo = the exact remainder rounded once
             const _Float16 hi = (_Float16)v;
             const _Float16 lo = (_Float16)(v - (double)hi);
             h[l * 8 + j] = half_bits(hl == 0 ? hi : lo);
@@ -367,9 +372,9 @@ void bs_realization(const float *sos, int ns, std::vector<double> &A, std::vecto
     }
 }
 
-int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out)
+int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out, int split)
 {
-    const BsLayout Y = bs_layout(L);
+    const BsLayout Y = bs_layout(L, split);
     out.assign((size_t)Y.stride, 0.0f);
     const int p = 2 * ns;
     Mat A;
@@ -423,10 +428,12 @@ int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out)
     const int sT = pow2_scale(Tm.data(), Tm.size()), sG = pow2_scale(Gm.data(), Gm.size());
     sc[0] = (float)sT;
     sc[1] = (float)sG;
-    for (int hl = 0; hl < 2; ++hl) {
+    for (int hl = 0; hl < split; ++hl) {
         for (int mb = 0; mb < Y.MB; ++mb)
-            for (int q = 0; q < Y.KQ; ++q) put_block32(out.data() + bs_T(Y, hl, mb, q), Tm.data(), L, L, 16 * mb, q, sT, hl);
-        for (int q = 0; q < Y.KQ; ++q) put_block32(out.data() + bs_G(Y, hl, q), Gm.data(), 16, L, 0, q, sG, hl, true);
+            for (int q = 0; q < Y.KQ; ++q)
+                put_block32(out.data() + bs_T(Y, hl, mb, q), Tm.data(), L, L, 16 * mb, q, sT, hl, false, split);
+        for (int q = 0; q < Y.KQ; ++q)
+            put_block32(out.data() + bs_G(Y, hl, q), Gm.data(), 16, L, 0, q, sG, hl, true, split);
     }
     for (int j = 0; j < 6; ++j) put_f32(out.data() + bs_P(Y, j), P[j].data(), 0);
     for (int nb = 1; nb < 4; ++nb) put_f32(out.data() + bs_Q(Y, nb), Q[nb].data(), 0);

@@ -47,7 +47,7 @@ int cr_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &ou
 int plr_tables(const double *b, const double *a, int K, int CH, std::vector<float> &out);
 
 // Block-state tables (layout kernels.hpp:bs_layout) for tiles of L samples.
-int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out);
+int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out, int split = 2);  // 2: FP16 hi/lo, 3: BF16x3
 
 // The block-state realization in FP64: A (16 x 16), B, C of the cascade of
 // direct-form-II-transposed sections (state (w1, w2) of section s at 2s,

@@ -73,7 +73,7 @@ typedef enum {
     IIRX_KERNEL_STCR = 1,  /* single-stage tridiagonal CR, stcr_kernels.cuh */
     IIRX_KERNEL_DTCR = 2,  /* decoupled two-sided CR, dtcr_kernels.cuh */
     IIRX_KERNEL_PLR = 3,   /* direct-form hierarchical merge, plr_kernels.cuh */
-    IIRX_KERNEL_BLOCKSTATE = 4  /* no counterpart in the reference: the whole
+    IIRX_KERNEL_BLOCKSTATE = 4, /* no counterpart in the reference: the whole
                                    cascade as a state-space block filter on the
                                    matrix cores (FP16 hi/lo operands for the
                                    Toeplitz and input products, FP32 operands
@@ -82,6 +82,11 @@ typedef enum {
                                    (32, 64 or 128), block_size = tiles per
                                    chunk G (16, 32 or 64; 16 or 32 at L =
                                    128), chunk = G * L samples */
+    IIRX_KERNEL_BLOCKSTATE_BF16 = 5  /* the block-state form with the Toeplitz
+                                        and input products split three ways
+                                        into BF16 (hi, mid, lo; six MFMA
+                                        products per K slab) instead of FP16
+                                        hi/lo; L = 64 or 128, G = 16 or 32 */
 } iirx_kernel;
 
 /*

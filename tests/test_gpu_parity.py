@@ -357,6 +357,72 @@ def test_blockstate_parity(device, order, G, L, wn):
     assert e_bs <= max(8.0 * e_ph, 1e-6), (e_bs, e_ph)
 
 
+BF16_GEOMS = [(16, 64), (32, 64), (16, 128), (32, 128)]
+
+
+@pytest.mark.parametrize("order", ORDERS + [6, 12])
+@pytest.mark.parametrize("G,L", BF16_GEOMS)
+@pytest.mark.parametrize("wn", [0.1, 0.02])
+def test_blockstate_bf16_parity(device, order, G, L, wn):
+    """The block-state form with the BF16 hi/mid/lo split (six MFMA
+    products per K slab): the FP64 gate, and an error (max|err| / RMS) no
+    more than 8x the PH kernel's, as for the FP16 hi/lo split; no spills."""
+    print(f"case BS_BF16 order={order} G={G} L={L} wn={wn}", flush=True)
+    sos, g, x = normal_workload(order, 1 << 20, seed=3, wn=wn)
+    y, info = run_gpu(device, "BS_BF16", sos, x, G, L)
+    ref = O.golden(sos, x)
+    ok, err, mref, rel = O.gate(y, ref)
+    yp, _ = run_gpu(device, "PH", sos, x, 64, 32)
+    e_bs, e_ph = err_rms(y, ref), err_rms(yp, ref)
+    print(f"  rel={rel:.2e} err/rms BF16={e_bs:.2e} PH={e_ph:.2e} vgprs={info['vgprs']} scratch={info['scratch_bytes']}")
+    assert ok, rel
+    assert e_bs <= max(8.0 * e_ph, 1e-6), (e_bs, e_ph)
+    assert info["scratch_bytes"] == 0
+
+
+@pytest.mark.parametrize("G,L", [(16, 64), (32, 128)])
+def test_blockstate_bf16_any_length_and_streaming(device, G, L):
+    """A length that is no multiple of the chunk (the tail through the PH
+    last-chunk kernel) and two streamed calls equal to one, as for the FP16
+    split."""
+    import torch
+    ch = G * L
+    n = 9 * ch + 777
+    sos, g, x = normal_workload(16, n, seed=31, wn=0.1)
+    y, _ = run_gpu(device, "BS_BF16", sos, x, G, L)
+    ref = O.golden(sos, x)
+    assert O.gate(y, ref)[0]
+    n1 = 4 * ch + 5
+    xs = [torch.from_numpy(x[:n1].copy()).to(device), torch.from_numpy(x[n1:].copy()).to(device)]
+    z = torch.zeros(2 + 2 * sos.shape[0], device=device)
+    outs = []
+    for t in xs:
+        o = torch.empty_like(t)
+        with iirx.Plan("BS_BF16", sos, t.numel(), block_size=G, n_blocks=L) as p:
+            p.execute_state(t, o, zi=z, zf=z)
+            torch.cuda.synchronize()
+            p.check()
+        outs.append(o.cpu().numpy())
+    y2 = np.concatenate(outs)
+    assert O.gate(y2, ref)[0]
+    assert np.max(np.abs(y2 - y)) <= 1e-5 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("order", [2, 8, 16])
+@pytest.mark.parametrize("G,L", [(16, 64), (32, 128)])
+def test_blockstate_bf16_device_tables_equal_host(device, order, G, L):
+    """The BF16 planes built in FP64 on the GPU equal the host's bit for bit
+    (the split is the same integer rounding on both sides)."""
+    sos, g = iirx.butterworth_sos(order, 0.1)
+    n = G * L * 4
+    host, _ = iirx.plan_tables("BS_BF16", sos, n, block_size=G, n_blocks=L)
+    with iirx.Plan("BS_BF16", sos, n, block_size=G, n_blocks=L) as p:
+        dev = p.device_tables()
+    assert dev.size == host.size
+    diff = np.flatnonzero(dev.view(np.uint32) != host.view(np.uint32))
+    assert diff.size == 0, (diff[:8], dev[diff[:8]], host[diff[:8]])
+
+
 @pytest.mark.parametrize("order", [2, 4, 8, 12, 16])
 @pytest.mark.parametrize("G,L", [(64, 64), (16, 32), (32, 128)])
 @pytest.mark.parametrize("wn", [0.1, 0.02, 0.4])

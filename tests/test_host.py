@@ -159,24 +159,29 @@ def test_config_check(kernel, B, N, n, ok):
     assert (st == 0) == ok, st
 
 
-def _bs_tables(sos, L, G=64):
-    """Dequantised block-state tables (layout of kernels.hpp:bs_layout)."""
-    tab, stride = iirx.plan_tables("BS", sos, G * L * 4, block_size=G, n_blocks=L)
+def _bs_tables(sos, L, G=64, kernel="BS"):
+    """Dequantised block-state tables (layout of kernels.hpp:bs_layout):
+    FP16 hi/lo planes (BS) or BF16 hi/mid/lo planes (BS_BF16)."""
+    tab, stride = iirx.plan_tables(kernel, sos, G * L * 4, block_size=G, n_blocks=L)
     MB, KQ = L // 16, L // 32
+    split = 3 if kernel == "BS_BF16" else 2
     kT = 0
-    kG = kT + 2 * MB * KQ * 256
-    kP = kG + 2 * KQ * 256
+    kG = kT + split * MB * KQ * 256
+    kP = kG + split * KQ * 256
     kQ = kP + 6 * 256
     kPhi = kQ + 3 * 256
     kO = kPhi + 256
     kS = kO + MB * 256
     kMc = kS + 4
-    halves = tab.view(np.uint32).view(np.float16)  # 2 halves per word
+    if split == 2:
+        halves = tab.view(np.uint32).view(np.float16).astype(np.float64)  # 2 halves per word
+    else:  # BF16: the upper 16 bits of a float32
+        halves = (tab.view(np.uint32).view(np.uint16).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
     def block32(word_off):
         """16 x 32 x32-MFMA A operand: lane l, half j = A[l%16][K row 8(l/16)+j],
         K row -> sample 4g + j (j < 4) or 16 + 4g + j - 4 of the slab."""
-        h = halves[2 * word_off: 2 * word_off + 512].astype(np.float64).reshape(64, 8)
+        h = halves[2 * word_off: 2 * word_off + 512].reshape(64, 8)
         m = np.zeros((16, 32))
         for l in range(64):
             g = l // 16
@@ -198,14 +203,14 @@ def _bs_tables(sos, L, G=64):
     for mb in range(MB):
         for q in range(KQ):
             off = kT + (mb * KQ + q) * 256
-            T[16 * mb:16 * mb + 16, 32 * q:32 * q + 32] = block32(off) + block32(off + MB * KQ * 256)
+            T[16 * mb:16 * mb + 16, 32 * q:32 * q + 32] = sum(block32(off + hl * MB * KQ * 256) for hl in range(split))
     T *= 2.0 ** (-tab[kS])
     # Gam's operand row i carries state (i >> 2) + 4 (i & 3) (kernels.hpp: bs_state_of_row)
     Gm = np.zeros((16, L))
     rows = [(i >> 2) + 4 * (i & 3) for i in range(16)]
     for q in range(KQ):
         off = kG + q * 256
-        Gm[rows, 32 * q:32 * q + 32] = block32(off) + block32(off + KQ * 256)
+        Gm[rows, 32 * q:32 * q + 32] = sum(block32(off + hl * KQ * 256) for hl in range(split))
     Gm *= 2.0 ** (-tab[kS + 1])
     Om = np.vstack([blockf(kO + mb * 256) for mb in range(MB)])
     P0 = blockf(kP)
@@ -215,15 +220,17 @@ def _bs_tables(sos, L, G=64):
 
 
 @pytest.mark.parametrize("order", [2, 8, 16])
-@pytest.mark.parametrize("L,G", [(32, 16), (32, 64), (64, 16), (64, 64), (128, 16), (128, 32)])
-def test_blockstate_tables_reproduce_the_cascade(order, L, G):
-    """The block-state tables (FP64 state space of the cascade, FP16 hi/lo
-    operands) define the same filter: running y_t = T x_t + O s_t,
-    s_{t+1} = Phi s_t + Gam x_t in float64 from the dequantised tables
-    matches the FP64 cascade to the operand precision (2^-22 relative)."""
+@pytest.mark.parametrize("kernel,L,G", [("BS", 32, 16), ("BS", 32, 64), ("BS", 64, 16), ("BS", 64, 64),
+                                        ("BS", 128, 16), ("BS", 128, 32), ("BS_BF16", 64, 16),
+                                        ("BS_BF16", 64, 32), ("BS_BF16", 128, 16), ("BS_BF16", 128, 32)])
+def test_blockstate_tables_reproduce_the_cascade(order, kernel, L, G):
+    """The block-state tables (FP64 state space of the cascade, FP16 hi/lo or
+    BF16 hi/mid/lo operands) define the same filter: running y_t = T x_t +
+    O s_t, s_{t+1} = Phi s_t + Gam x_t in float64 from the dequantised
+    tables matches the FP64 cascade to the operand precision."""
     from oracle_lib import golden
     sos, g = iirx.butterworth_sos(order, 0.1)
-    T, Om, Gm, P0, Phi, Mc, depth = _bs_tables(sos, L, G)
+    T, Om, Gm, P0, Phi, Mc, depth = _bs_tables(sos, L, G, kernel)
     assert np.array_equal(P0, Phi)
     # states 0..order-1 in order: the rows past them are empty
     dead = list(range(order, 16))
@@ -241,3 +248,31 @@ def test_blockstate_tables_reproduce_the_cascade(order, L, G):
         s = Phi @ s + Gm @ xt
     assert np.max(np.abs(y - ref)) <= 1e-5 * np.max(np.abs(ref))
     assert depth >= 1
+
+
+@pytest.mark.parametrize("L,G", [(64, 16), (128, 32)])
+@pytest.mark.parametrize("order", [8, 16])
+def test_blockstate_bf16_split_holds_24_bits(L, G, order):
+    """The BF16 three-way split of the Toeplitz block holds every entry
+    h[i - j] (the FP64 impulse response of the cascade) to 24 significant
+    bits, |hi + mid + lo - h| <= 2^-23 |h|, across the whole dynamic range:
+    BF16 keeps FP32's exponent range.  The FP16 pair, with one power-of-two
+    scale for the whole block, loses the entries far below the largest (at
+    order 16 the impulse response spans more than 10 decades)."""
+    from scipy import signal
+    sos, g = iirx.butterworth_sos(order, 0.1)
+    with np.errstate(invalid="ignore"):
+        T3, _, _, _, _, _, _ = _bs_tables(sos, L, G, "BS_BF16")
+        T2, _, _, _, _, _, _ = _bs_tables(sos, L, G, "BS")
+    imp = np.zeros(L)
+    imp[0] = 1.0
+    h = signal.sosfilt(sos.astype(np.float64), imp)
+    T = np.zeros((L, L))
+    for i in range(L):
+        T[i, :i + 1] = h[i::-1]
+    nz = T != 0
+    e3 = np.abs(T3 - T)[nz] / np.abs(T)[nz]
+    e2 = np.abs(T2 - T)[nz] / np.abs(T)[nz]
+    assert np.max(e3) <= 2.0 ** -23, np.max(e3)
+    if order == 16:
+        assert np.max(e2) > 1e3 * np.max(e3)
