@@ -215,6 +215,23 @@ __device__ __forceinline__ Ticket take_ticket(unsigned long long *ctr, uint32_t 
     return t;
 }
 
+// LDS written by this wave is visible to all of its lanes: the wave's LDS
+// operations complete in order, so waiting for its own outstanding ones
+// suffices (the kernels with several chunks per workgroup use it where a
+// one-wave workgroup uses a barrier)
+__device__ __forceinline__ void wave_lds_sync()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int WV>
+__device__ __forceinline__ void chunk_sync()
+{
+    if constexpr (WV == 1) __syncthreads();
+    else wave_lds_sync();
+}
+
 // ------------------------------------------------------------------------
 // lookback records
 // ------------------------------------------------------------------------
@@ -444,12 +461,27 @@ __device__ __forceinline__ float lookbackK(uint64_t *lb, size_t rec_stride, uint
 // chunk driver: runs body(chunk, part_flag, full_flag) for every chunk this
 // workgroup owns under the plan's schedule (see the protocol notes above)
 // ------------------------------------------------------------------------
-template <typename Body>
+template <int WV = 1, int WPC = 1, typename Body>
 __device__ __forceinline__ void for_each_chunk(unsigned long long *ticket, uint32_t *epoch_word, uint32_t *arrive,
                                                uint32_t num_chunks, int persistent, uint32_t spin_limit,
                                                uint32_t *err, Body &&body)
 {
-    if (persistent) {
+    if constexpr (WV > 1) {
+        // The ticket schedule with WV chunks (WPC waves each) per workgroup,
+        // one workgroup per group of WV consecutive chunks: one atomic ticket
+        // per group (G = num_chunks / WV per launch; ticket t is group t mod
+        // G of epoch t / G).  A chunk waits only on chunks of lower tickets,
+        // held by workgroups already running, or of its own workgroup, whose
+        // waves never meet at a barrier after this one: no co-residency needed.
+        __shared__ unsigned long long s_tk;
+        const uint32_t groups = num_chunks / WV;
+        if (threadIdx.x == 0) s_tk = atomicAdd(ticket, 1ull);
+        __syncthreads();
+        const unsigned long long tk = s_tk;
+        const unsigned long long epoch = tk / groups;
+        const uint32_t pf = (uint32_t)(2ull * epoch + 1ull);
+        body((uint32_t)(tk - epoch * groups) * WV + (threadIdx.x >> 6) / WPC, pf, pf + 1u);
+    } else if (persistent) {
         const uint32_t epoch = epoch_load(epoch_word);
         epoch_arrive(arrive, epoch);
         const uint32_t pf = 2u * epoch + 1u, ff = pf + 1u;
@@ -469,13 +501,14 @@ __device__ __forceinline__ void for_each_chunk(unsigned long long *ticket, uint3
 // Chunk of L lanes x N samples, lane t owns samples [t*N, t*N + N); LDS rows
 // are padded to N+1 floats so that the per-lane row reads hit distinct banks.
 // ------------------------------------------------------------------------
+// tid: this thread among the T that move the chunk (threadIdx.x unless the
+// workgroup holds several chunks)
 template <int L, int N, int T = L>
-__device__ __forceinline__ void stage_in(const float *__restrict__ src, float *lds)
+__device__ __forceinline__ void stage_in(const float *__restrict__ src, float *lds, int tid = threadIdx.x)
 {
     constexpr int V = L * N / 4;           // float4s in the chunk
     constexpr int PER = V / T;             // float4s per thread
     static_assert(V % T == 0, "chunk must split evenly over the threads");
-    const int tid = threadIdx.x;
     const f4 *s4 = reinterpret_cast<const f4 *>(src);
     f4 v[PER];
 #pragma unroll
@@ -497,11 +530,10 @@ __device__ __forceinline__ void stage_in(const float *__restrict__ src, float *l
 // with nontemporal stores (574 vs 555-570 at order 4, within 1% at orders 8
 // and 16).  The loads stay nontemporal: plain loads ran 639-644 at order 2.
 template <int L, int N, int T = L>
-__device__ __forceinline__ void stage_out(const float *lds, float *__restrict__ dst)
+__device__ __forceinline__ void stage_out(const float *lds, float *__restrict__ dst, int tid = threadIdx.x)
 {
     constexpr int V = L * N / 4;
     constexpr int PER = V / T;
-    const int tid = threadIdx.x;
     f4 *d4 = reinterpret_cast<f4 *>(dst);
 #pragma unroll
     for (int i = 0; i < PER; ++i) {

@@ -400,6 +400,15 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     // a block-state workgroup runs chunks_per_block waves, each its own chunks
     const int64_t blocks_needed = ((int64_t)p->main_chunks + sh.chunks_per_block - 1) / sh.chunks_per_block;
     p->grid = (uint32_t)blocks_needed;
+    // PH and STCR under the ticket schedule: one workgroup per group of
+    // ph_ticket_waves chunks, one ticket per group (the kernels' launch rule).
+    // DTCR keeps one chunk per workgroup: its two waves meet at barriers
+    // inside the section loop, where a chunk of the same workgroup could be
+    // waiting for its record.
+    if ((cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR) &&
+        cfg->schedule == IIRX_SCHEDULE_TICKET && p->main_chunks > 0 &&
+        p->main_chunks % (uint32_t)ph_ticket_waves(sh.n_blocks) == 0)
+        p->grid = p->main_chunks / (uint32_t)ph_ticket_waves(sh.n_blocks);
     if (cfg->schedule == IIRX_SCHEDULE_PERSISTENT) {
         int per_cu = 0, cus = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->fn, p->threads, 0) != hipSuccess)

@@ -85,6 +85,14 @@ constexpr PhLayout ph_layout(int L, int N)
 // last = true: the one-workgroup launch of the partial last chunk (chunk
 // a.num_chunks, a.last_valid samples), after the launch of the others
 hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last = false);
+// PH and STCR under the ticket schedule: chunks (waves) per workgroup, one
+// atomic ticket per workgroup, the workgroup's LDS tiles within 64 KB; used
+// when it divides the launch's chunk count (else one chunk per workgroup).
+// After the ticket's barrier the waves of such a workgroup synchronise only
+// within themselves (their LDS tiles are their own), so a chunk that waits
+// for the record of an earlier chunk of its workgroup never holds that chunk
+// back.
+constexpr int ph_ticket_waves(int N) { return N <= 32 ? 8 : (N == 64 ? 4 : 2); }
 const void *kernel_ph(int lanes, int n_blocks);
 
 }  // namespace iirx

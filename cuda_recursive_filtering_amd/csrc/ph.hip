@@ -17,6 +17,8 @@
 #include "device_common.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace iirx {
 
 // LAST = false: chunks 0 .. num_chunks - 1 under the plan's schedule.
@@ -25,21 +27,24 @@ namespace iirx {
 // on the same stream: its incoming state is the full record its predecessor
 // left (no lookback), samples past the end read as zero and are not written,
 // and with a.tail set it records the end state there.
-template <int N, bool LAST>
-__global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
+// WV > 1: the ticket schedule with WV chunks per workgroup, one wave and one
+// LDS tile each (one atomic ticket per workgroup).
+template <int N, bool LAST, int WV = 1>
+__global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
 {
     constexpr int L = 64;
     constexpr PhLayout Y = ph_layout(L, N);
-    __shared__ float lds[L * (N + 1)];
+    __shared__ float lds_w[WV][L * (N + 1)];
+    float *lds = lds_w[WV == 1 ? 0 : threadIdx.x >> 6];
 
-    const int lane = threadIdx.x;
+    const int lane = WV == 1 ? (int)threadIdx.x : (int)(threadIdx.x & 63);
     const auto body = [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
     const long base = (long)chunk * (L * N);
 
-    __syncthreads();  // LDS of the previous chunk fully consumed
+    chunk_sync<WV>();  // LDS of the previous chunk fully consumed
     if constexpr (LAST) stage_in_part<L, N>(a.in + base, lds, a.last_valid);
-    else stage_in<L, N>(a.in + base, lds);
-    __syncthreads();
+    else stage_in<L, N>(a.in + base, lds, lane);
+    chunk_sync<WV>();
 
     float r[N];
 #pragma unroll
@@ -134,15 +139,15 @@ __global__ __launch_bounds__(64) void ph_kernel(LaunchArgs a)
             if (a.tail) tail_pair<N>(r, lds, a.last_valid, X.y1, a.tail + 2 + 2 * sec);
     }
 
-    __syncthreads();
+    chunk_sync<WV>();
 #pragma unroll
     for (int n = 0; n < N; ++n) lds[lane * (N + 1) + n] = r[n];
-    __syncthreads();
+    chunk_sync<WV>();
     if constexpr (LAST) stage_out_part<L, N>(lds, a.out + base, a.last_valid);
-    else stage_out<L, N>(lds, a.out + base);
+    else stage_out<L, N>(lds, a.out + base, lane);
     };
     if constexpr (LAST) body(a.num_chunks, 0u, 0u);
-    else for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err, body);
+    else for_each_chunk<WV>(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err, body);
 }
 
 // zf = x[n-2], x[n-1], then (y2, y1) of every section from the last chunk's
@@ -167,7 +172,10 @@ hipError_t launch_ph_end_state(const LaunchArgs &a, int chunk_samples, float *zf
 template <int N>
 static hipError_t launch_n(const LaunchArgs &a, bool last, hipStream_t s)
 {
+    constexpr int WV = ph_ticket_waves(N);
     if (last) hipLaunchKernelGGL((ph_kernel<N, true>), dim3(1), dim3(64), 0, s, a);
+    else if (!a.persistent && a.num_chunks % WV == 0)
+        hipLaunchKernelGGL((ph_kernel<N, false, WV>), dim3(a.num_chunks / WV), dim3(64 * WV), 0, s, a);
     else hipLaunchKernelGGL((ph_kernel<N, false>), dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }

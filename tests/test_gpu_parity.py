@@ -107,8 +107,8 @@ def test_launch_versioning_stale_flags(device, kernel, schedule):
         p.check()
         info = p.info()
         assert info["launches"] == 9
-        if schedule == "ticket":
-            assert info["grid"] == info["num_chunks"]
+        if schedule == "ticket":  # PH/STCR: groups of chunks per workgroup, DTCR: one chunk each
+            assert 1 <= info["grid"] <= info["num_chunks"]
         else:
             assert info["grid"] <= info["num_chunks"]
 
