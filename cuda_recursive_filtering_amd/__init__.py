@@ -211,7 +211,7 @@ def autotune(sos, n_samples, flags=0, reps=10):
     ms = ctypes.c_float()
     check(lib().iirx_autotune(s.ctypes.data, s.shape[0], int(n_samples), int(flags), int(reps), ctypes.byref(best),
                               ctypes.byref(ms), arr, n.value, ctypes.byref(n)), "autotune")
-    table = [dict(_cfg_dict(e.cfg), status=e.status, ms=e.ms) for e in arr]
+    table = [dict(_cfg_dict(e.cfg), status=e.status, ms=e.ms, rel_err=e.rel_err) for e in arr]
     return _cfg_dict(best), ms.value, table
 
 
@@ -415,6 +415,35 @@ class Batch:
         check(lib().iirx_batch_create_ex(ctypes.byref(self._cfg), self.n_channels, off.ctypes.data, ns.ctypes.data,
                                          sos.ctypes.data, codes.ctypes.data, ctypes.byref(h)), "batch_create")
         self._h = h
+
+    @classmethod
+    def autotuned(cls, offsets, sections, n_sections=None, schedule="persistent", reps=10):
+        """A batch ('auto' channels) with the n_blocks, bs_tiles and
+        bs_min_sections the batch autotuner measures fastest on this device;
+        the measured ms per launch is left in `.tuned_ms`."""
+        off = np.ascontiguousarray(np.asarray(offsets, np.int64))
+        if n_sections is None:
+            secs = [_sos_array(s) for s in sections]
+            ns = np.array([s.shape[0] for s in secs], np.int32)
+            sos = np.ascontiguousarray(np.concatenate(secs, axis=0), np.float32)
+        else:
+            ns = np.ascontiguousarray(np.asarray(n_sections, np.int32))
+            sos = np.ascontiguousarray(np.asarray(sections, np.float32).reshape(-1, 6))
+        if off.size != ns.size + 1:
+            raise ValueError("offsets must have one entry more than there are channels")
+        cfg = _lib.BatchConfig()
+        cfg.n_blocks = 32
+        cfg.schedule = _lib.SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
+        cfg.bs_tiles = 32
+        best = _lib.BatchConfig()
+        ms = ctypes.c_float()
+        check(lib().iirx_batch_autotune(ctypes.byref(cfg), int(ns.size), off.ctypes.data, ns.ctypes.data,
+                                        sos.ctypes.data, int(reps), ctypes.byref(best), ctypes.byref(ms)),
+              "batch_autotune")
+        b = cls(off, sos, n_sections=ns, n_blocks=best.n_blocks, schedule=best.schedule,
+                bs_min_sections=best.bs_min_sections, bs_tiles=best.bs_tiles)
+        b.tuned_ms = ms.value
+        return b
 
     def _check_len(self, *ts):
         for t in ts:

@@ -18,6 +18,7 @@ IIRX_ERR_NO_DEVICE = 5
 IIRX_ERR_OOM = 6
 IIRX_ERR_TIMEOUT = 7
 IIRX_ERR_TOO_LARGE = 8
+IIRX_ERR_ACCURACY = 9
 
 SCHEDULES = {"persistent": 0, "ticket": 1}
 
@@ -68,6 +69,7 @@ class TuneEntry(ctypes.Structure):
         ("cfg", Config),
         ("status", ctypes.c_int32),
         ("ms", ctypes.c_float),
+        ("rel_err", ctypes.c_float),
     ]
 
 
@@ -152,6 +154,8 @@ _SIGS = {
     "iirx_batch_capture": (ctypes.c_int, [_P, _P, _P, _P]),
     "iirx_batch_replay": (ctypes.c_int, [_P, _P]),
     "iirx_batch_check": (ctypes.c_int, [_P]),
+    "iirx_batch_autotune": (ctypes.c_int, [ctypes.POINTER(BatchConfig), ctypes.c_int64, _P, _P, _P, ctypes.c_int32,
+                                           ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_float)]),
     "iirx_batch_info_get": (ctypes.c_int, [_P, ctypes.POINTER(BatchInfo)]),
     "iirx_batch_destroy": (ctypes.c_int, [_P]),
     "iirx_execute_state": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),

@@ -278,6 +278,7 @@ const char *iirx_status_string(int s)
     case IIRX_ERR_TIMEOUT: return "lookback watchdog: a wait exceeded the spin limit (workgroups not co-resident)";
     case IIRX_ERR_TOO_LARGE:
         return "the call needs more device scratch than iirx_execute_state_noalloc holds (IIRX_NOALLOC_SCRATCH_BYTES)";
+    case IIRX_ERR_ACCURACY: return "autotune: output fails the accuracy screen against the FP64 cascade";
     default: return "unknown status";
     }
 }

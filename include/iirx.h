@@ -53,9 +53,13 @@ typedef enum {
     IIRX_ERR_TIMEOUT = 7,        /* a lookback wait exceeded the plan's spin
                                     limit (workgroups not co-resident); the
                                     output of that call is invalid */
-    IIRX_ERR_TOO_LARGE = 8       /* iirx_execute_state_noalloc: the call needs
+    IIRX_ERR_TOO_LARGE = 8,      /* iirx_execute_state_noalloc: the call needs
                                     more than IIRX_NOALLOC_SCRATCH_BYTES of
                                     device scratch */
+    IIRX_ERR_ACCURACY = 9        /* autotuner: the candidate's output missed
+                                    the accuracy screen (max|err| above 1e-4
+                                    max|ref| and above 10x the best
+                                    candidate's), so it is not eligible */
 } iirx_status;
 
 /* Workgroup scheduling of a plan (no counterpart in the reference, which
@@ -407,6 +411,9 @@ typedef struct {
     iirx_config cfg;
     int32_t status;   /* iirx_status of the candidate's tables; OK = eligible */
     float ms;         /* mean ms per launch (iirx_autotune), -1 if not timed */
+    float rel_err;    /* iirx_autotune: max|out - ref| / max|ref| over the
+                         first min(n, 2^18) samples against the host FP64
+                         cascade, -1 if not timed */
 } iirx_tune_entry;
 
 /* Host only: the candidate list for (sos, n_samples) with each candidate's
@@ -416,8 +423,14 @@ int iirx_tune_candidates(const float *sos, int32_t n_sections, int64_t n_samples
                          int32_t *count);
 /* Time every eligible candidate (reps launches after 3 warm-up launches on
  * N(0,1) device buffers of n_samples floats, rotating over up to 8 inputs so
- * that long signals stream from HBM) and return the fastest in *best /
- * *best_ms; entries (optional) receive the full table. */
+ * that long signals stream from HBM), screen its output against the host
+ * FP64 cascade over the first min(n, 2^18) samples (the reference verifies
+ * accuracy after its timing runs, run_performance.sh), and return the
+ * fastest candidate that passes in *best / *best_ms.  A candidate passes if
+ * max|err| <= 1e-4 max|ref| (the reference's gate, test_accuracy.cu:178-190)
+ * or, for filters where no float32 form meets that, within 10x the most
+ * accurate candidate's error; the others get IIRX_ERR_ACCURACY.  entries
+ * (optional) receive the full table. */
 int iirx_autotune(const float *sos, int32_t n_sections, int64_t n_samples,
                   uint32_t flags, int32_t reps, iirx_config *best,
                   float *best_ms, iirx_tune_entry *entries, int32_t cap,
@@ -502,6 +515,15 @@ int iirx_batch_capture(iirx_batch *b, const float *d_in, float *d_out,
 int iirx_batch_replay(iirx_batch *b, void *stream);
 /* Synchronise and report the lookback watchdog, as iirx_plan_check. */
 int iirx_batch_check(iirx_batch *b);
+/* The geometry of a batch by measurement (no counterpart in the reference,
+ * whose tile geometry is fixed per binary): every combination of n_blocks
+ * {16, 32, 64}, bs_tiles {16, 32, 64} and bs_min_sections {4, 8, 9 (none)}
+ * with cfg->schedule, each timed over `reps` launches of the whole batch on
+ * N(0,1) inputs after 2 warm-up launches; the fastest configuration goes to
+ * *best and its mean ms per launch to *best_ms. */
+int iirx_batch_autotune(const iirx_batch_config *cfg, int64_t n_channels, const int64_t *offsets,
+                        const int32_t *n_sections, const float *sos, int32_t reps,
+                        iirx_batch_config *best, float *best_ms);
 int iirx_batch_info_get(const iirx_batch *b, iirx_batch_info *info);
 /* The batch's per-channel tables on the device, copied back (channel after
  * channel, each 16-byte aligned): min(cap, size) floats to out (out may be

@@ -82,3 +82,25 @@ def test_autotune_falls_back_from_cr(device):
     best, ms, table = iirx.autotune(QUADRATURE, 1 << 20, reps=3)
     assert best["kernel"] in ("PH", "BS")
     assert all(e["ms"] < 0 for e in table if e["kernel"] in ("STCR", "DTCR"))
+
+
+@pytest.mark.gpu
+def test_autotune_screens_accuracy(device):
+    """Every timed candidate carries its error against the FP64 cascade of the
+    tuning input; the float32 direct form at order 16 (offered only with
+    TUNE_PLR_ALL_ORDERS) fails the screen and is never chosen."""
+    print("case autotune accuracy screen order 16", flush=True)
+    sos, _ = iirx.butterworth_sos(16, 0.1)
+    best, ms, table = iirx.autotune(sos, 1 << 20, flags=iirx._lib.TUNE_PLR_ALL_ORDERS, reps=3)
+    timed = [e for e in table if e["ms"] > 0]
+    assert timed and all(e["rel_err"] >= 0 for e in timed)
+    plr = [e for e in table if e["kernel"] == "PLR"]
+    assert plr and all(e["status"] == iirx._lib.IIRX_ERR_ACCURACY for e in plr)
+    assert best["kernel"] != "PLR"
+    passing = [e for e in timed if e["status"] == 0]
+    assert passing and all(e["rel_err"] <= 1e-4 for e in passing)
+    assert ms <= min(e["ms"] for e in passing) + 1e-6
+
+
+def test_accuracy_status_has_a_message():
+    assert "accuracy" in iirx._lib.lib().iirx_status_string(iirx._lib.IIRX_ERR_ACCURACY).decode()

@@ -297,3 +297,36 @@ def test_batch_full_ragged_workload(device):
         ref = O.golden(sections[c], x_cpu[c])
         ok, err, mref, rel = O.gate(y_cpu[c], ref)
         assert ok, (c, rel)
+
+
+@pytest.mark.gpu
+def test_batch_autotuned(device):
+    """The batch autotuner picks one of its searched geometries and the batch
+    it builds filters every channel within the accuracy gate."""
+    import torch
+    print("case batch autotune 64 channels", flush=True)
+    offsets, sections, gains = ragged_channels(64, seed=6, hi_log2=15)
+    x = ragged_input(offsets, gains, seed=600)
+    xin = torch.from_numpy(x).to(device)
+    out = torch.empty_like(xin)
+    with iirx.Batch.autotuned(offsets, sections, reps=3) as b:
+        assert b.tuned_ms > 0
+        assert b._cfg.n_blocks in (16, 32, 64) and b._cfg.bs_tiles in (16, 32, 64)
+        assert b._cfg.bs_min_sections in (4, 8, 9)
+        b.execute(xin, out)
+        b.check()
+    _check_channels(out.cpu().numpy(), x, offsets, sections)
+
+
+def test_batch_autotune_rejects_bad_arguments():
+    import ctypes
+    from cuda_recursive_filtering_amd import _lib
+    cfg = _lib.BatchConfig()
+    best = _lib.BatchConfig()
+    ms = ctypes.c_float()
+    off = np.array([0, 10], np.int64)
+    ns = np.array([1], np.int32)
+    sos = np.array([[1, 0, 0, 1, -0.5, 0]], np.float32)
+    st = _lib.lib().iirx_batch_autotune(ctypes.byref(cfg), 1, off.ctypes.data, ns.ctypes.data, sos.ctypes.data, 0,
+                                        ctypes.byref(best), ctypes.byref(ms))
+    assert st == _lib.IIRX_ERR_INVALID
