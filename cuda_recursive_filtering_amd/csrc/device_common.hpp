@@ -13,7 +13,8 @@
 //     it by then: the last chunk's lookback observed its predecessor's
 //     record, which observed its own predecessor's, and so on).  A bounded
 //     spin turns a violated co-residency assumption into an error flag.
-//   * Ticket schedule: one 64-bit counter per plan, incremented once per workgroup.
+//   * Ticket schedule: one 64-bit counter per plan, incremented once per
+//     workgroup (PH and STCR: once per group of chunks, for_each_chunk).
 //     ticket = epoch * num_chunks + chunk.  Workgroups of one launch draw
 //     tickets in dispatch order, so a chunk only ever waits on chunks that
 //     already hold a ticket, i.e. are resident or finished (AMD hardware
@@ -467,20 +468,43 @@ __device__ __forceinline__ void for_each_chunk(unsigned long long *ticket, uint3
                                                uint32_t *err, Body &&body)
 {
     if constexpr (WV > 1) {
-        // The ticket schedule with WV chunks (WPC waves each) per workgroup,
-        // one workgroup per group of WV consecutive chunks: one atomic ticket
-        // per group (G = num_chunks / WV per launch; ticket t is group t mod
-        // G of epoch t / G).  A chunk waits only on chunks of lower tickets,
-        // held by workgroups already running, or of its own workgroup, whose
-        // waves never meet at a barrier after this one: no co-residency needed.
-        __shared__ unsigned long long s_tk;
+        // The ticket schedule with WV chunks (WPC waves each) per workgroup:
+        // workgroups loop over groups of WV consecutive chunks, one atomic
+        // ticket per group.  A launch hands out G = num_chunks / WV group
+        // tickets and then one "done" ticket to each of its gridDim.x
+        // workgroups, which stop at their first one: G + gridDim.x tickets
+        // per launch, ticket t is group t mod (G + gridDim.x) of epoch
+        // t / (G + gridDim.x), so that every launch starts on an epoch
+        // boundary.  The first wave claims the next group's ticket as soon
+        // as its chunk is done, while the group's later chunks, which wait on
+        // it through the lookback, finish.  PH 64 x 32 at 2^25 samples with
+        // groups of 8, alternating builds on one box, ran 617 / 532 / 395 /
+        // 246 GS/s at orders 2 / 4 / 8 / 16 against 565 / 504 / 401 / 253 with one
+        // group per workgroup, and 545 / 474 / 345 / 217 with the claim made
+        // by the last wave a group ahead (its loads then wait on the atomic).
+        // Forward progress without co-residency: a workgroup works on its
+        // tickets in increasing order, so the lowest unfinished ticket is the
+        // one its holder is working on, and every chunk it waits for belongs
+        // to a lower ticket (finished) or to its own group, whose waves meet
+        // at no barrier before the group ends.
+        __shared__ unsigned long long s_tk[2];
         const uint32_t groups = num_chunks / WV;
-        if (threadIdx.x == 0) s_tk = atomicAdd(ticket, 1ull);
+        const unsigned long long per = (unsigned long long)groups + gridDim.x;
+        const bool claimer = threadIdx.x == 0;
+        if (claimer) s_tk[0] = atomicAdd(ticket, 1ull);
         __syncthreads();
-        const unsigned long long tk = s_tk;
-        const unsigned long long epoch = tk / groups;
+        unsigned long long tk = s_tk[0];
+        const unsigned long long epoch = tk / per;
+        const unsigned long long first = epoch * per;
         const uint32_t pf = (uint32_t)(2ull * epoch + 1ull);
-        body((uint32_t)(tk - epoch * groups) * WV + (threadIdx.x >> 6) / WPC, pf, pf + 1u);
+        int k = 0;
+        while (tk - first < groups) {
+            body((uint32_t)(tk - first) * WV + (threadIdx.x >> 6) / WPC, pf, pf + 1u);
+            k ^= 1;
+            if (claimer) s_tk[k] = atomicAdd(ticket, 1ull);
+            __syncthreads();
+            tk = s_tk[k];
+        }
     } else if (persistent) {
         const uint32_t epoch = epoch_load(epoch_word);
         epoch_arrive(arrive, epoch);

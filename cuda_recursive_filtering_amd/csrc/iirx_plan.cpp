@@ -259,6 +259,41 @@ const void *kernel_fn(const iirx_config *cfg, const Shape &sh)
     }
 }
 
+// Workgroups of the main launch.  Persistent schedule: one per resident slot
+// of the kernel (occupancy API), at most one per workgroup's worth of chunks.
+// Ticket schedule: one per workgroup's worth of chunks, except PH and STCR
+// when ph_ticket_waves(N) divides the chunk count: their grouped instance
+// loops over groups of that many chunks, one workgroup per resident slot of
+// that instance.  DTCR keeps one chunk per workgroup: its two waves meet at
+// barriers inside the section loop, where a chunk of the same workgroup
+// could be waiting for its record.
+int plan_grid(const iirx_config *cfg, const Shape &sh, uint32_t main_chunks, const void *fn, int threads, int dev,
+              uint32_t &grid)
+{
+    const int64_t blocks_needed = ((int64_t)main_chunks + sh.chunks_per_block - 1) / sh.chunks_per_block;
+    grid = (uint32_t)blocks_needed;
+    const int wv = ph_ticket_waves(sh.n_blocks);
+    const bool grouped = (cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR) &&
+                         cfg->schedule == IIRX_SCHEDULE_TICKET && main_chunks > 0 && main_chunks % (uint32_t)wv == 0;
+    if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && !grouped) return IIRX_OK;
+    const void *occ_fn = fn;
+    int occ_threads = threads;
+    int64_t want = blocks_needed;
+    if (grouped) {
+        occ_fn = cfg->kernel == IIRX_KERNEL_PH ? kernel_ph(sh.lanes, sh.n_blocks, true)
+                                               : kernel_stcr(sh.lanes, sh.n_blocks, cfg->unrolled, true);
+        if (!occ_fn) return IIRX_ERR_GEOMETRY;
+        occ_threads = 64 * wv;
+        want = main_chunks / (uint32_t)wv;
+    }
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, occ_fn, occ_threads, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return IIRX_ERR_HIP;
+    grid = (uint32_t)std::min<int64_t>((int64_t)std::max(per_cu, 1) * std::max(cus, 1), want);
+    return IIRX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -399,26 +434,8 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     if (!p->fn) return fail(IIRX_ERR_GEOMETRY);
     // workgroups: one per chunk (ticket) or one per resident slot (persistent);
     // a block-state workgroup runs chunks_per_block waves, each its own chunks
-    const int64_t blocks_needed = ((int64_t)p->main_chunks + sh.chunks_per_block - 1) / sh.chunks_per_block;
-    p->grid = (uint32_t)blocks_needed;
-    // PH and STCR under the ticket schedule: one workgroup per group of
-    // ph_ticket_waves chunks, one ticket per group (the kernels' launch rule).
-    // DTCR keeps one chunk per workgroup: its two waves meet at barriers
-    // inside the section loop, where a chunk of the same workgroup could be
-    // waiting for its record.
-    if ((cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR) &&
-        cfg->schedule == IIRX_SCHEDULE_TICKET && p->main_chunks > 0 &&
-        p->main_chunks % (uint32_t)ph_ticket_waves(sh.n_blocks) == 0)
-        p->grid = p->main_chunks / (uint32_t)ph_ticket_waves(sh.n_blocks);
-    if (cfg->schedule == IIRX_SCHEDULE_PERSISTENT) {
-        int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->fn, p->threads, 0) != hipSuccess)
-            return fail(IIRX_ERR_HIP);
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess)
-            return fail(IIRX_ERR_HIP);
-        const int64_t slots = (int64_t)std::max(per_cu, 1) * std::max(cus, 1);
-        p->grid = (uint32_t)std::min<int64_t>(slots, blocks_needed);
-    }
+    const int st_grid = plan_grid(cfg, sh, p->main_chunks, p->fn, p->threads, p->device, p->grid);
+    if (st_grid != IIRX_OK) return fail(st_grid);
     if (hipDeviceSynchronize() != hipSuccess) return fail(IIRX_ERR_HIP);
     *out = p;
     return IIRX_OK;
@@ -806,14 +823,8 @@ extern "C" int iirx_execute_state_noalloc(const iirx_config *cfg, const float *s
     p.main_chunks = p.num_chunks - (p.split_last ? 1u : 0u);
     p.fn = kernel_fn(cfg, sh);
     if (!p.fn) return IIRX_ERR_GEOMETRY;
-    p.grid = p.main_chunks;
-    if (cfg->schedule == IIRX_SCHEDULE_PERSISTENT) {
-        int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p.fn, p.threads, 0) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            return IIRX_ERR_HIP;
-        p.grid = (uint32_t)std::min<int64_t>((int64_t)std::max(per_cu, 1) * std::max(cus, 1), p.main_chunks);
-    }
+    st = plan_grid(cfg, sh, p.main_chunks, p.fn, p.threads, dev, p.grid);
+    if (st != IIRX_OK) return st;
 
     // the same argument checks as iirx_execute_state
     if (!io_aligned(&p, d_in, d_out)) return IIRX_ERR_INVALID;

@@ -85,15 +85,21 @@ constexpr PhLayout ph_layout(int L, int N)
 // last = true: the one-workgroup launch of the partial last chunk (chunk
 // a.num_chunks, a.last_valid samples), after the launch of the others
 hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last = false);
-// PH and STCR under the ticket schedule: chunks (waves) per workgroup, one
-// atomic ticket per workgroup, the workgroup's LDS tiles within 64 KB; used
-// when it divides the launch's chunk count (else one chunk per workgroup).
-// After the ticket's barrier the waves of such a workgroup synchronise only
+// PH and STCR under the ticket schedule: chunks (waves) per group, one
+// atomic ticket per group; used when it divides the launch's chunk count
+// (else one chunk per workgroup).  16 waves at N <= 32 (135 KB of LDS at
+// N = 32, 4 waves/SIMD as with one wave per workgroup): PH 64 x 32 ran
+// 638 / 522 / 395 / 255 GS/s at orders 2 / 4 / 8 / 16 against 617 / 532 /
+// 395 / 246 with 8; STCR 64 x 64 ran 581 / 219 with 8 waves against 588 /
+// 219 with 4 at orders 2 / 16.
+// Between the ticket barriers the waves of such a workgroup synchronise only
 // within themselves (their LDS tiles are their own), so a chunk that waits
-// for the record of an earlier chunk of its workgroup never holds that chunk
+// for the record of an earlier chunk of its group never holds that chunk
 // back.
-constexpr int ph_ticket_waves(int N) { return N <= 32 ? 8 : (N == 64 ? 4 : 2); }
-const void *kernel_ph(int lanes, int n_blocks);
+constexpr int ph_ticket_waves(int N) { return N <= 32 ? 16 : (N == 64 ? 4 : 2); }
+// grouped: the ticket-schedule instance with ph_ticket_waves(n_blocks) chunks
+// per workgroup
+const void *kernel_ph(int lanes, int n_blocks, bool grouped = false);
 
 }  // namespace iirx
 
@@ -139,7 +145,7 @@ constexpr CrLayout cr_layout(int L)
 }
 
 hipError_t launch_stcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last = false);
-const void *kernel_stcr(int lanes, int n_blocks, int unrolled);
+const void *kernel_stcr(int lanes, int n_blocks, int unrolled, bool grouped = false);
 hipError_t launch_dtcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last = false);
 const void *kernel_dtcr(int lanes, int n_blocks, int unrolled);
 

@@ -175,7 +175,7 @@ static hipError_t launch_n(const LaunchArgs &a, bool last, hipStream_t s)
     constexpr int WV = ph_ticket_waves(N);
     if (last) hipLaunchKernelGGL((ph_kernel<N, true>), dim3(1), dim3(64), 0, s, a);
     else if (!a.persistent && a.num_chunks % WV == 0)
-        hipLaunchKernelGGL((ph_kernel<N, false, WV>), dim3(a.num_chunks / WV), dim3(64 * WV), 0, s, a);
+        hipLaunchKernelGGL((ph_kernel<N, false, WV>), dim3(a.grid), dim3(64 * WV), 0, s, a);
     else hipLaunchKernelGGL((ph_kernel<N, false>), dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
@@ -193,15 +193,22 @@ hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s
     }
 }
 
-const void *kernel_ph(int lanes, int n_blocks)
+template <int N>
+static const void *fn_n(bool grouped)
+{
+    if (grouped) return (const void *)ph_kernel<N, false, ph_ticket_waves(N)>;
+    return (const void *)ph_kernel<N, false>;
+}
+
+const void *kernel_ph(int lanes, int n_blocks, bool grouped)
 {
     if (lanes != 64) return nullptr;
     switch (n_blocks) {
-    case 8: return (const void *)ph_kernel<8, false>;
-    case 16: return (const void *)ph_kernel<16, false>;
-    case 32: return (const void *)ph_kernel<32, false>;
-    case 64: return (const void *)ph_kernel<64, false>;
-    case 128: return (const void *)ph_kernel<128, false>;
+    case 8: return fn_n<8>(grouped);
+    case 16: return fn_n<16>(grouped);
+    case 32: return fn_n<32>(grouped);
+    case 64: return fn_n<64>(grouped);
+    case 128: return fn_n<128>(grouped);
     default: return nullptr;
     }
 }

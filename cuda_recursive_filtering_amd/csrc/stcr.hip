@@ -308,8 +308,7 @@ static hipError_t launch_n(const LaunchArgs &a, bool last, hipStream_t s)
     constexpr int WV = ph_ticket_waves(N);
     if (last) hipLaunchKernelGGL((stcr_kernel<N, U, true>), dim3(1), dim3(64), 0, s, a);
     else if (!a.persistent && a.num_chunks % WV == 0)
-        hipLaunchKernelGGL((stcr_kernel<N, U, false, WV>), dim3(a.num_chunks / WV), dim3(64 * WV), 0,
-                           s, a);
+        hipLaunchKernelGGL((stcr_kernel<N, U, false, WV>), dim3(a.grid), dim3(64 * WV), 0, s, a);
     else hipLaunchKernelGGL((stcr_kernel<N, U, false>), dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
@@ -328,16 +327,23 @@ hipError_t launch_stcr(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t
     }
 }
 
-const void *kernel_stcr(int lanes, int n_blocks, int unrolled)
+template <int N, bool U>
+static const void *fn_n(bool grouped)
+{
+    if (grouped) return (const void *)stcr_kernel<N, U, false, ph_ticket_waves(N)>;
+    return (const void *)stcr_kernel<N, U, false>;
+}
+
+const void *kernel_stcr(int lanes, int n_blocks, int unrolled, bool grouped)
 {
     if (lanes != 64) return nullptr;
     const bool u = unrolled != 0;
     switch (n_blocks) {
-    case 8: return u ? (const void *)stcr_kernel<8, true, false> : (const void *)stcr_kernel<8, false, false>;
-    case 16: return u ? (const void *)stcr_kernel<16, true, false> : (const void *)stcr_kernel<16, false, false>;
-    case 32: return u ? (const void *)stcr_kernel<32, true, false> : (const void *)stcr_kernel<32, false, false>;
-    case 64: return u ? (const void *)stcr_kernel<64, true, false> : (const void *)stcr_kernel<64, false, false>;
-    case 128: return u ? (const void *)stcr_kernel<128, true, false> : (const void *)stcr_kernel<128, false, false>;
+    case 8: return u ? fn_n<8, true>(grouped) : fn_n<8, false>(grouped);
+    case 16: return u ? fn_n<16, true>(grouped) : fn_n<16, false>(grouped);
+    case 32: return u ? fn_n<32, true>(grouped) : fn_n<32, false>(grouped);
+    case 64: return u ? fn_n<64, true>(grouped) : fn_n<64, false>(grouped);
+    case 128: return u ? fn_n<128, true>(grouped) : fn_n<128, false>(grouped);
     default: return nullptr;
     }
 }

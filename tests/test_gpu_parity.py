@@ -113,6 +113,33 @@ def test_launch_versioning_stale_flags(device, kernel, schedule):
             assert info["grid"] <= info["num_chunks"]
 
 
+@pytest.mark.parametrize("kernel,B,N,waves", [("PH", 64, 32, 16), ("STCR", 64, 64, 4), ("PH", 64, 16, 16)])
+def test_ticket_groups_loop(device, kernel, B, N, waves):
+    """Ticket schedule with more groups of chunks than resident workgroups:
+    every workgroup claims several groups, one atomic ticket each, and a
+    launch hands out one done ticket per workgroup.  Alternating inputs over
+    five launches: a ticket count that drifted off the epoch boundary would
+    mix flags of two launches."""
+    import torch
+    print(f"case ticket groups loop {kernel} {B}x{N}", flush=True)
+    n = 1 << 24
+    sos, g, xa = normal_workload(4, n, seed=9, wn=0.1)
+    _, _, xb = normal_workload(4, n, seed=10, wn=0.1)
+    ra, rb = O.golden(sos, xa), O.golden(sos, xb)
+    ta, tb = torch.from_numpy(xa).to(device), torch.from_numpy(xb).to(device)
+    out = torch.empty_like(ta)
+    with iirx.Plan(kernel, sos, n, block_size=B, n_blocks=N, schedule="ticket") as p:
+        info = p.info()
+        assert info["num_chunks"] % waves == 0 and info["grid"] * waves < info["num_chunks"], info
+        for i in range(5):
+            src, ref = (ta, ra) if i % 2 == 0 else (tb, rb)
+            out.fill_(float("nan"))
+            p.execute(src, out)
+            p.check()
+            ok, err, mref, rel = O.gate(out.cpu().numpy(), ref)
+            assert ok, (i, rel)
+
+
 @pytest.mark.parametrize("schedule", ["persistent", "ticket"])
 @pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("n_chunks", [1, 2, 63, 64, 65, 200, 5000])
