@@ -22,7 +22,7 @@ IIRX_ERR_ACCURACY = 9
 
 SCHEDULES = {"persistent": 0, "ticket": 1}
 
-KERNELS = {"PH": 0, "STCR": 1, "DTCR": 2, "PLR": 3, "BS": 4, "BS_BF16": 5}
+KERNELS = {"PH": 0, "STCR": 1, "DTCR": 2, "PLR": 3, "BS": 4, "BS_BF16": 5, "PHJ": 6}
 
 
 class Config(ctypes.Structure):

@@ -55,6 +55,8 @@ std::vector<iirx_config> enumerate(int32_t ns, uint32_t flags)
         c.push_back(x);
     };
     for (int N = 8; N <= 128; N *= 2) add(IIRX_KERNEL_PH, 64, N, 0);
+    if (ns >= 2)
+        for (int N = 8; N <= 128; N *= 2) add(IIRX_KERNEL_PH_JOINT, 64, N, 0);
     if (!(flags & IIRX_TUNE_NO_CR))
         for (int k : {IIRX_KERNEL_STCR, IIRX_KERNEL_DTCR})
             for (int N = 8; N <= 128; N *= 2)
@@ -63,6 +65,9 @@ std::vector<iirx_config> enumerate(int32_t ns, uint32_t flags)
         for (int G : {16, 32, 64})
             for (int L : {32, 64, 128})
                 if (L * G <= 4096) add(IIRX_KERNEL_BLOCKSTATE, G, L, 0);
+    if (!(flags & IIRX_TUNE_NO_BLOCKSTATE))
+        for (int G : {16, 32})
+            for (int L : {64, 128}) add(IIRX_KERNEL_BLOCKSTATE_BF16, G, L, 0);
     if (ns <= 2 || (flags & IIRX_TUNE_PLR_ALL_ORDERS)) add(IIRX_KERNEL_PLR, 0, 0, 0);
     return c;
 }

@@ -372,6 +372,156 @@ __device__ __forceinline__ Carry2 lookback2(uint64_t *lb, uint32_t chunk, int se
     return X;
 }
 
+// lookback2 for a pair of sections (sec, sec + 1), whose records lie side by
+// side: one predecessor per lane, its 8 words in one round trip, used whole
+// (both sections full, or both partial).  The carry is the pair's joint state
+// (y_s2, y_s1, y_t2, y_t1); M its chunk transition (4 x 4 row-major, block
+// lower triangular: y_s does not see y_t; uniform); z the state before the
+// chain start.
+struct Carry4 {
+    float v[4];
+};
+
+__device__ __forceinline__ Carry4 lookback_pair(uint64_t *lb, uint32_t chunk, int sec, int ns, uint32_t part_flag,
+                                               uint32_t full_flag, const cfloat *M, int depth, uint32_t spin_limit,
+                                               uint32_t *err, const float *z)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const long pred = (long)chunk - 1 - lane;
+    const int window = (depth > 0 && depth < kWave) ? depth : kWave;
+    const bool active = lane < window;
+    const uint64_t below_depth = (depth >= kWave) ? ~0ull : ((1ull << depth) - 1ull);
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // 0 not ready, 1 partial, 2 full, 3 before the signal start, 4 outside the window
+    int st = !active ? 4 : (pred < 0 ? 3 : 0);
+    if (st == 3) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = z[i];
+    }
+    int D = 0;
+    bool from_full = false;
+    uint32_t spins = 0;
+    for (;;) {
+        if (st < 2) {
+            uint64_t *r = rec2(lb, (uint32_t)pred, sec, ns);
+            uint64_t w[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) w[i] = rec_load(r + i);
+            // words: s partial y2, y1, s full y2, y1, t partial y2, y1, t full y2, y1
+            if (rec_flag(w[2]) == full_flag && rec_flag(w[3]) == full_flag && rec_flag(w[6]) == full_flag &&
+                rec_flag(w[7]) == full_flag) {
+                st = 2;
+                v[0] = rec_val(w[2]);
+                v[1] = rec_val(w[3]);
+                v[2] = rec_val(w[6]);
+                v[3] = rec_val(w[7]);
+            } else if (st == 0 && rec_flag(w[0]) == part_flag && rec_flag(w[1]) == part_flag &&
+                       rec_flag(w[4]) == part_flag && rec_flag(w[5]) == part_flag) {
+                st = 1;
+                v[0] = rec_val(w[0]);
+                v[1] = rec_val(w[1]);
+                v[2] = rec_val(w[4]);
+                v[3] = rec_val(w[5]);
+            }
+        }
+        const uint64_t zmask = __ballot(st == 0);
+        const uint64_t fmask = __ballot(st == 2 || st == 3);
+        if (fmask) {
+            const int f = __builtin_ctzll(fmask);
+            const uint64_t upto = (f == kWave - 1) ? ~0ull : ((1ull << (f + 1)) - 1ull);
+            if ((zmask & upto) == 0) {
+                D = f;
+                from_full = true;
+                break;
+            }
+        }
+        if (depth > 0 && (zmask & below_depth) == 0) {
+            D = depth;
+            break;
+        }
+        if (++spins > spin_limit) {
+            if (lane == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            D = 0;
+            from_full = false;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    Carry4 X;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) X.v[i] = from_full ? readlane_f(v[i], D) : 0.0f;
+    for (int d = D - 1; d >= 0; --d) {
+        float p[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) p[i] = readlane_f(v[i], d);
+        const float n0 = fmaf(M[1], X.v[1], fmaf(M[0], X.v[0], p[0]));
+        const float n1 = fmaf(M[5], X.v[1], fmaf(M[4], X.v[0], p[1]));
+        const float n2 = fmaf(M[11], X.v[3], fmaf(M[10], X.v[2], fmaf(M[9], X.v[1], fmaf(M[8], X.v[0], p[2]))));
+        const float n3 = fmaf(M[15], X.v[3], fmaf(M[14], X.v[2], fmaf(M[13], X.v[1], fmaf(M[12], X.v[0], p[3]))));
+        X.v[0] = n0;
+        X.v[1] = n1;
+        X.v[2] = n2;
+        X.v[3] = n3;
+    }
+    return X;
+}
+
+// Inclusive scan over the wave's 64 lanes of the pair's joint lane end
+// states e (4 floats) with the uniform operator J, the ph_lane_scan scheme
+// with 4 x 4 block lower triangular operators: Jpow = J^(2^k), k = 0..4
+// (uniform), Jrow = J^(j+1), j = 0..15 (per lane).
+__device__ __forceinline__ void mat4_lt_apply(const float *P, const float t[4], float e[4])
+{
+    e[0] = fmaf(P[1], t[1], fmaf(P[0], t[0], e[0]));
+    e[1] = fmaf(P[5], t[1], fmaf(P[4], t[0], e[1]));
+    e[2] = fmaf(P[11], t[3], fmaf(P[10], t[2], fmaf(P[9], t[1], fmaf(P[8], t[0], e[2]))));
+    e[3] = fmaf(P[15], t[3], fmaf(P[14], t[2], fmaf(P[13], t[1], fmaf(P[12], t[0], e[3]))));
+}
+
+__device__ __forceinline__ void mat4_lt_apply(const cfloat *P, const float t[4], float e[4])
+{
+    e[0] = fmaf(P[1], t[1], fmaf(P[0], t[0], e[0]));
+    e[1] = fmaf(P[5], t[1], fmaf(P[4], t[0], e[1]));
+    e[2] = fmaf(P[11], t[3], fmaf(P[10], t[2], fmaf(P[9], t[1], fmaf(P[8], t[0], e[2]))));
+    e[3] = fmaf(P[15], t[3], fmaf(P[14], t[2], fmaf(P[13], t[1], fmaf(P[12], t[0], e[3]))));
+}
+
+__device__ __forceinline__ void pair_lane_scan(const cfloat *Jpow, const float *Jrow, int lane, float e[4])
+{
+    static_for<0, 4, 1>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        float t[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) t[i] = row_shr_zero<1 << k>(e[i]);
+        mat4_lt_apply(Jpow + 16 * k, t, e);
+    });
+    const cfloat *P = Jpow + 64;  // J^16
+    float a[4], b[4], c[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        a[i] = readlane_f(e[i], 15);
+        b[i] = readlane_f(e[i], 31);
+        c[i] = readlane_f(e[i], 47);
+    }
+    mat4_lt_apply(P, a, b);
+    mat4_lt_apply(P, b, c);
+    const int row = lane >> 4;
+    float k4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) k4[i] = row == 1 ? a[i] : row == 2 ? b[i] : row == 3 ? c[i] : 0.0f;
+    float J[16];
+    const float4 *jr = reinterpret_cast<const float4 *>(Jrow + 16 * (lane & 15));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 v = jr[q];
+        J[4 * q] = v.x;
+        J[4 * q + 1] = v.y;
+        J[4 * q + 2] = v.z;
+        J[4 * q + 3] = v.w;
+    }
+    mat4_lt_apply(J, k4, e);
+}
+
 // Lookback over K-word records ([K partial words][K full words] per chunk,
 // `rec_stride` words apart), for carries of K floats (PLR's direct-form
 // history, the block-state form's state vector).  Predecessors are scanned

@@ -83,6 +83,7 @@ bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
 // the block-state form and its operand split
 bool is_bs(int kernel) { return kernel == IIRX_KERNEL_BLOCKSTATE || kernel == IIRX_KERNEL_BLOCKSTATE_BF16; }
+bool is_ph(int kernel) { return kernel == IIRX_KERNEL_PH || kernel == IIRX_KERNEL_PH_JOINT; }
 int bs_split_of(int kernel) { return kernel == IIRX_KERNEL_BLOCKSTATE_BF16 ? kBsSplitBF16 : kBsSplitF16; }
 
 // rows of the filter contract: finite, b0 == 1 and a0 == 1 (iirx_normalize_sos makes them)
@@ -105,6 +106,7 @@ int validate(const iirx_config *cfg, int64_t n)
     if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && cfg->schedule != IIRX_SCHEDULE_TICKET) return IIRX_ERR_INVALID;
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH:
+    case IIRX_KERNEL_PH_JOINT:
     case IIRX_KERNEL_STCR:
     case IIRX_KERNEL_DTCR: {
         const int B = cfg->block_size, N = cfg->n_blocks;
@@ -181,7 +183,7 @@ int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double
     // PH / STCR / DTCR: a chunk is BLOCK_SIZE x N_BLOCKS samples as in the
     // reference (test_accuracy.cu:37-38); a wave64 holds it as 64 lanes of
     // 32 N / 64 samples when BLOCK_SIZE is 32
-    if (cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR || cfg->kernel == IIRX_KERNEL_DTCR)
+    if (is_ph(cfg->kernel) || cfg->kernel == IIRX_KERNEL_STCR || cfg->kernel == IIRX_KERNEL_DTCR)
         sh.n_blocks = cfg->n_blocks * cfg->block_size / 64;
     sh.threads = 64;
     sh.chunks_per_block = 1;
@@ -189,6 +191,9 @@ int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH:
         sh.coef_stride = ph_tables(sos, ns, 64, sh.n_blocks, tables);
+        break;
+    case IIRX_KERNEL_PH_JOINT:
+        sh.coef_stride = ph_joint_tables(sos, ns, 64, sh.n_blocks, tables);
         break;
     case IIRX_KERNEL_STCR:
     case IIRX_KERNEL_DTCR:
@@ -249,6 +254,7 @@ const void *kernel_fn(const iirx_config *cfg, const Shape &sh)
 {
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH: return kernel_ph(sh.lanes, sh.n_blocks);
+    case IIRX_KERNEL_PH_JOINT: return kernel_ph(sh.lanes, sh.n_blocks, false, true);
     case IIRX_KERNEL_STCR: return kernel_stcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_DTCR: return kernel_dtcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_PLR: return kernel_plr(2 * cfg->n_sections, sh.plr_block, sh.plr_x);
@@ -273,15 +279,15 @@ int plan_grid(const iirx_config *cfg, const Shape &sh, uint32_t main_chunks, con
     const int64_t blocks_needed = ((int64_t)main_chunks + sh.chunks_per_block - 1) / sh.chunks_per_block;
     grid = (uint32_t)blocks_needed;
     const int wv = ph_ticket_waves(sh.n_blocks);
-    const bool grouped = (cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR) &&
+    const bool grouped = (is_ph(cfg->kernel) || cfg->kernel == IIRX_KERNEL_STCR) &&
                          cfg->schedule == IIRX_SCHEDULE_TICKET && main_chunks > 0 && main_chunks % (uint32_t)wv == 0;
     if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && !grouped) return IIRX_OK;
     const void *occ_fn = fn;
     int occ_threads = threads;
     int64_t want = blocks_needed;
     if (grouped) {
-        occ_fn = cfg->kernel == IIRX_KERNEL_PH ? kernel_ph(sh.lanes, sh.n_blocks, true)
-                                               : kernel_stcr(sh.lanes, sh.n_blocks, cfg->unrolled, true);
+        occ_fn = is_ph(cfg->kernel) ? kernel_ph(sh.lanes, sh.n_blocks, true, cfg->kernel == IIRX_KERNEL_PH_JOINT)
+                                    : kernel_stcr(sh.lanes, sh.n_blocks, cfg->unrolled, true);
         if (!occ_fn) return IIRX_ERR_GEOMETRY;
         occ_threads = 64 * wv;
         want = main_chunks / (uint32_t)wv;
@@ -361,7 +367,7 @@ int iirx_plan_create(const iirx_config *cfg, const float *sos, int64_t n_samples
     const size_t lb_count = sh.lb_count;
 
     p->d_bsmap = nullptr;
-    p->split_last = (cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR ||
+    p->split_last = (is_ph(cfg->kernel) || cfg->kernel == IIRX_KERNEL_STCR ||
                      cfg->kernel == IIRX_KERNEL_DTCR || is_bs(cfg->kernel)) &&
                     n_samples % sh.chunk != 0;
     p->main_chunks = p->num_chunks - (p->split_last ? 1u : 0u);
@@ -529,11 +535,13 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
     e = hipErrorInvalidValue;
     switch (p->cfg.kernel) {
     case IIRX_KERNEL_PH:
+    case IIRX_KERNEL_PH_JOINT:
     case IIRX_KERNEL_STCR:
     case IIRX_KERNEL_DTCR: {
         // the main launch, then the partial last chunk (which records zf)
         auto run = [&](const LaunchArgs &la, bool last) {
-            if (p->cfg.kernel == IIRX_KERNEL_PH) return launch_ph(p->lanes, p->n_blocks, la, s, last);
+            if (is_ph(p->cfg.kernel))
+                return launch_ph(p->lanes, p->n_blocks, la, s, last, p->cfg.kernel == IIRX_KERNEL_PH_JOINT);
             if (p->cfg.kernel == IIRX_KERNEL_STCR) return launch_stcr(p->lanes, p->n_blocks, la, s, last);
             return launch_dtcr(p->lanes, p->n_blocks, la, s, last);
         };
@@ -776,7 +784,7 @@ extern "C" int iirx_execute_state_noalloc(const iirx_config *cfg, const float *s
                                           void *stream)
 {
     if (!cfg || !sos || !d_in || !d_out) return IIRX_ERR_INVALID;
-    if (!(cfg->kernel == IIRX_KERNEL_PH || cfg->kernel == IIRX_KERNEL_STCR || cfg->kernel == IIRX_KERNEL_DTCR))
+    if (!(is_ph(cfg->kernel) || cfg->kernel == IIRX_KERNEL_STCR || cfg->kernel == IIRX_KERNEL_DTCR))
         return IIRX_ERR_INVALID;
     int st = validate(cfg, n_samples);
     if (st != IIRX_OK) return st;

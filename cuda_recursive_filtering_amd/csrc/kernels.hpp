@@ -80,11 +80,53 @@ constexpr PhLayout ph_layout(int L, int N)
     return p;
 }
 
+// PH with sections in pairs (s, t = s + 1): the pair's joint state
+// U = (y_s[-2], y_s[-1], y_t[-2], y_t[-1]) at a lane's start, J its N-step
+// transition (zero input), block lower triangular (y_s does not see y_t).
+// Per pair, after the per-section tables of all sections:
+//   kJpow  J^(2^k), k = 0..4, 16 floats each (row-major; uniform)
+//   kM     J^L, the chunk transition (uniform)
+//   kDepth lookback depth for J^L; kDepth + 1: 1 if the pair runs joined,
+//          0 if its tables are too large (kPairLimit) and the two sections
+//          run one by one
+//   kJrow  J^(j+1), j = 0..15, per lane of a 16-lane row
+//   kJfin  J^(l+1), l = 0..L-1, per lane
+//   kH     H_m[n], m = 0..3, n = 0..N-1: y_t at sample n of a lane from the
+//          unit joint state e_m at the lane's start (uniform)
+// A pair runs joined only while every entry of its tables (H, J^(l+1), J^L)
+// stays within kPairLimit of the states' own scale.  The cross terms grow
+// like 1/(distance between the two sections' poles): for Butterworth
+// lowpass filters about 3, 13, 72, 480 and 6600 at Wn = 0.4, 0.2, 0.1, 0.05
+// and 0.02 (order 4), and they cancel in the correction sum y_t += H U, so
+// the float32 error grows with them (order 4, Wn = 0.002: 14x the
+// sequential float32 cascade's error joined, against 1x one by one).
+constexpr double kPairLimit = 256.0;
+
+struct PhPairLayout {
+    int kJpow, kM, kDepth, kJrow, kJfin, kH, stride;
+};
+
+constexpr PhPairLayout ph_pair_layout(int L, int N)
+{
+    PhPairLayout p{};
+    p.kJpow = 0;
+    p.kM = p.kJpow + 16 * 5;
+    p.kDepth = p.kM + 16;
+    p.kJrow = p.kDepth + 4;
+    p.kJfin = p.kJrow + 16 * 16;
+    p.kH = p.kJfin + 16 * L;
+    p.stride = align4_c(p.kH + 4 * N);
+    return p;
+}
+
 // Launchers (one per algorithm); `lanes` and `n_blocks` pick the template
 // instance.  Return hipErrorInvalidValue when no instance exists.
 // last = true: the one-workgroup launch of the partial last chunk (chunk
 // a.num_chunks, a.last_valid samples), after the launch of the others
-hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last = false);
+// joint: sections in pairs (IIRX_KERNEL_PH_JOINT); the pair tables follow the
+// per-section ones at a.coef + a.ns * a.coef_stride
+hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last = false,
+                     bool joint = false);
 // PH and STCR under the ticket schedule: chunks (waves) per group, one
 // atomic ticket per group; used when it divides the launch's chunk count
 // (else one chunk per workgroup).  16 waves at N <= 32 (135 KB of LDS at
@@ -99,7 +141,7 @@ hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s
 constexpr int ph_ticket_waves(int N) { return N <= 32 ? 16 : (N == 64 ? 4 : 2); }
 // grouped: the ticket-schedule instance with ph_ticket_waves(n_blocks) chunks
 // per workgroup
-const void *kernel_ph(int lanes, int n_blocks, bool grouped = false);
+const void *kernel_ph(int lanes, int n_blocks, bool grouped = false, bool joint = false);
 
 }  // namespace iirx
 

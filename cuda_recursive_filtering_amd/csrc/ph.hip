@@ -28,12 +28,25 @@ namespace iirx {
 // left (no lookback), samples past the end read as zero and are not written,
 // and with a.tail set it records the end state there.
 // WV > 1: the ticket schedule with WV chunks per workgroup, one wave and one
-// LDS tile each (one atomic ticket per workgroup).
-template <int N, bool LAST, int WV = 1>
+// LDS tile each (one atomic ticket per group).
+// JOINT: sections in pairs (s, s + 1) with one lookback of the pair's joint
+// state per chunk (an odd last section alone, and so a pair whose tables
+// exceed kPairLimit).  Both sections' zero-state
+// passes run first, the second on the first's output with zero history
+// (that history is part of the joint state); the lane end states go through
+// one 4 x 4 scan; the chunk partial of both sections is published, one
+// lookback returns the joint carry X, and every sample of the second section
+// is corrected by the response of the joint state U at its lane's start,
+// y_t[n] += sum_m H_m[n] U_m.  Per sample that is the arithmetic of two PH
+// sections (8 FMA zero-state + 4 FMA correction against 2 x (4 + 2)); the
+// scan takes 12 FMA per level against 2 x 4, and a chunk waits for one
+// lookback instead of two.
+template <int N, bool LAST, int WV = 1, bool JOINT = false>
 __global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
 {
     constexpr int L = 64;
     constexpr PhLayout Y = ph_layout(L, N);
+    constexpr PhPairLayout YP = ph_pair_layout(L, N);
     __shared__ float lds_w[WV][L * (N + 1)];
     float *lds = lds_w[WV == 1 ? 0 : threadIdx.x >> 6];
 
@@ -70,6 +83,106 @@ __global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
         if (a.tail) tail_pair<N>(r, lds, a.last_valid, xi1, a.tail);
 
     for (int sec = 0; sec < a.ns; ++sec) {
+        if constexpr (JOINT && !LAST) {
+            const float *pv = a.coef + (size_t)a.ns * a.coef_stride + (size_t)(sec >> 1) * YP.stride;
+            const cfloat *pc = as_const(pv);
+            // pairs start at even sections; a pair whose tables are too large
+            // runs one section at a time
+            if ((sec & 1) == 0 && sec + 1 < a.ns && pc[YP.kDepth + 1] != 0.0f) {
+                const int t = sec + 1;
+                const cfloat *cs = as_const(a.coef + (size_t)sec * a.coef_stride);
+                const cfloat *ct = as_const(a.coef + (size_t)t * a.coef_stride);
+                const float b1 = cs[Y.kB + 0], b2 = cs[Y.kB + 1], a1 = cs[Y.kB + 2], a2 = cs[Y.kB + 3];
+                const float d1 = ct[Y.kB + 0], d2 = ct[Y.kB + 1], c1 = ct[Y.kB + 2], c2 = ct[Y.kB + 3];
+
+                // both zero-state passes; section t's input history is section
+                // s's output history, zero at the lane start like its own
+                float x2 = xi2, x1 = xi1, y2 = 0.0f, y1 = 0.0f, v2 = 0.0f, v1 = 0.0f;
+#pragma unroll
+                for (int n = 0; n < N; ++n) {
+                    const float xc = r[n];
+                    float yc = fmaf(b1, x1, fmaf(b2, x2, xc));
+                    if (n == 1) yc = fmaf(a1, y1, yc);
+                    else if (n > 1) yc = fmaf(a1, y1, fmaf(a2, y2, yc));
+                    float vc = yc;
+                    if (n == 1) vc = fmaf(c1, v1, fmaf(d1, y1, vc));
+                    else if (n > 1) vc = fmaf(c1, v1, fmaf(c2, v2, fmaf(d1, y1, fmaf(d2, y2, vc))));
+                    r[n] = vc;
+                    x2 = x1;
+                    x1 = xc;
+                    y2 = y1;
+                    y1 = yc;
+                    v2 = v1;
+                    v1 = vc;
+                }
+
+                // lane scan of the joint end states
+                float e[4] = {y2, y1, r[N - 2], r[N - 1]};
+                pair_lane_scan(pc + YP.kJpow, pv + YP.kJrow, lane, e);
+
+                // chunk partial of both sections, one lookback, chunk full
+                float p[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) p[i] = readlane_f(e[i], L - 1);
+                uint64_t *rs = rec2(a.lb, chunk, sec, a.ns);
+                if (lane == 0) {
+                    publish2(rs, false, part_flag, p[0], p[1]);
+                    publish2(rs + 4, false, part_flag, p[2], p[3]);
+                }
+                float z[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (a.zi) {
+                    z[0] = a.zi[2 + 2 * sec];
+                    z[1] = a.zi[3 + 2 * sec];
+                    z[2] = a.zi[2 + 2 * t];
+                    z[3] = a.zi[3 + 2 * t];
+                }
+                // J^(lane+1): its loads are in flight during the lookback
+                const float4 *jf = reinterpret_cast<const float4 *>(pv + YP.kJfin + 16 * lane);
+                float Jl[16];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float4 w4 = jf[q];
+                    Jl[4 * q] = w4.x;
+                    Jl[4 * q + 1] = w4.y;
+                    Jl[4 * q + 2] = w4.z;
+                    Jl[4 * q + 3] = w4.w;
+                }
+                const Carry4 X = lookback_pair(a.lb, chunk, sec, a.ns, part_flag, full_flag, pc + YP.kM,
+                                               (int)pc[YP.kDepth], a.spin_limit, a.err, z);
+                if (lane == 0) {
+                    float f[4] = {p[0], p[1], p[2], p[3]};
+                    mat4_lt_apply(pc + YP.kM, X.v, f);
+                    publish2(rs, true, full_flag, f[0], f[1]);
+                    publish2(rs + 4, true, full_flag, f[2], f[3]);
+                }
+
+                // per-lane end state with the chunk carry injected, J^(lane+1) X;
+                // incoming state of each lane = end state of the previous lane
+                mat4_lt_apply(Jl, X.v, e);
+                // (the shuffle runs on every lane: a source lane masked off by a
+                // branch would not deliver its value)
+                float u[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) u[i] = lane_up(e[i], 1);
+                if (lane == 0) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) u[i] = X.v[i];
+                }
+
+                // correction of section t's output by the joint state
+#pragma unroll
+                for (int n = 0; n < N - 2; ++n)
+                    r[n] = fmaf(pc[YP.kH + 3 * N + n], u[3],
+                                fmaf(pc[YP.kH + 2 * N + n], u[2],
+                                     fmaf(pc[YP.kH + N + n], u[1], fmaf(pc[YP.kH + n], u[0], r[n]))));
+                r[N - 2] = e[2];
+                r[N - 1] = e[3];
+                xi2 = u[2];
+                xi1 = u[3];
+                ++sec;
+                continue;
+            }
+        }
         const float *cv = a.coef + (size_t)sec * a.coef_stride;  // per-lane vector loads
         const cfloat *c = as_const(cv);                           // uniform scalar loads
         const float b1 = c[Y.kB + 0], b2 = c[Y.kB + 1], a1 = c[Y.kB + 2], a2 = c[Y.kB + 3];
@@ -169,46 +282,49 @@ hipError_t launch_ph_end_state(const LaunchArgs &a, int chunk_samples, float *zf
     return hipGetLastError();
 }
 
-template <int N>
+// the partial last chunk always runs the per-section instance: it reads the
+// full records of its predecessor per section, which the joint instance
+// writes in the same layout
+template <int N, bool J>
 static hipError_t launch_n(const LaunchArgs &a, bool last, hipStream_t s)
 {
     constexpr int WV = ph_ticket_waves(N);
     if (last) hipLaunchKernelGGL((ph_kernel<N, true>), dim3(1), dim3(64), 0, s, a);
     else if (!a.persistent && a.num_chunks % WV == 0)
-        hipLaunchKernelGGL((ph_kernel<N, false, WV>), dim3(a.grid), dim3(64 * WV), 0, s, a);
-    else hipLaunchKernelGGL((ph_kernel<N, false>), dim3(a.grid), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((ph_kernel<N, false, WV, J>), dim3(a.grid), dim3(64 * WV), 0, s, a);
+    else hipLaunchKernelGGL((ph_kernel<N, false, 1, J>), dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last)
+hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last, bool joint)
 {
     if (lanes != 64) return hipErrorInvalidValue;
     switch (n_blocks) {
-    case 8: return launch_n<8>(a, last, s);
-    case 16: return launch_n<16>(a, last, s);
-    case 32: return launch_n<32>(a, last, s);
-    case 64: return launch_n<64>(a, last, s);
-    case 128: return launch_n<128>(a, last, s);
+    case 8: return joint ? launch_n<8, true>(a, last, s) : launch_n<8, false>(a, last, s);
+    case 16: return joint ? launch_n<16, true>(a, last, s) : launch_n<16, false>(a, last, s);
+    case 32: return joint ? launch_n<32, true>(a, last, s) : launch_n<32, false>(a, last, s);
+    case 64: return joint ? launch_n<64, true>(a, last, s) : launch_n<64, false>(a, last, s);
+    case 128: return joint ? launch_n<128, true>(a, last, s) : launch_n<128, false>(a, last, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template <int N>
+template <int N, bool J>
 static const void *fn_n(bool grouped)
 {
-    if (grouped) return (const void *)ph_kernel<N, false, ph_ticket_waves(N)>;
-    return (const void *)ph_kernel<N, false>;
+    if (grouped) return (const void *)ph_kernel<N, false, ph_ticket_waves(N), J>;
+    return (const void *)ph_kernel<N, false, 1, J>;
 }
 
-const void *kernel_ph(int lanes, int n_blocks, bool grouped)
+const void *kernel_ph(int lanes, int n_blocks, bool grouped, bool joint)
 {
     if (lanes != 64) return nullptr;
     switch (n_blocks) {
-    case 8: return fn_n<8>(grouped);
-    case 16: return fn_n<16>(grouped);
-    case 32: return fn_n<32>(grouped);
-    case 64: return fn_n<64>(grouped);
-    case 128: return fn_n<128>(grouped);
+    case 8: return joint ? fn_n<8, true>(grouped) : fn_n<8, false>(grouped);
+    case 16: return joint ? fn_n<16, true>(grouped) : fn_n<16, false>(grouped);
+    case 32: return joint ? fn_n<32, true>(grouped) : fn_n<32, false>(grouped);
+    case 64: return joint ? fn_n<64, true>(grouped) : fn_n<64, false>(grouped);
+    case 128: return joint ? fn_n<128, true>(grouped) : fn_n<128, false>(grouped);
     default: return nullptr;
     }
 }

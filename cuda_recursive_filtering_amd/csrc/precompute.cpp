@@ -82,6 +82,103 @@ int ph_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &ou
     return Y.stride;
 }
 
+// 4 x 4 row-major double matrices
+static void m4_mul(const double *p, const double *q, double *r)
+{
+    double t[16];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < 4; ++k) acc += p[4 * i + k] * q[4 * k + j];
+            t[4 * i + j] = acc;
+        }
+    std::memcpy(r, t, sizeof(t));
+}
+
+static void put16(float *dst, const double *m)
+{
+    for (int i = 0; i < 16; ++i) dst[i] = (float)m[i];
+}
+
+// smallest d in [1, 64] with ||M^d||_inf <= 2^-36 for a 4 x 4 M, 0 if none
+static int lookback_depth4(const double *M)
+{
+    double P[16];
+    std::memcpy(P, M, sizeof(P));
+    for (int d = 1; d <= 64; ++d) {
+        double nrm = 0.0;
+        for (int i = 0; i < 4; ++i)
+            nrm = std::max(nrm, std::fabs(P[4 * i]) + std::fabs(P[4 * i + 1]) + std::fabs(P[4 * i + 2]) +
+                                    std::fabs(P[4 * i + 3]));
+        if (nrm <= kLookbackTail) return d;
+        m4_mul(P, M, P);
+    }
+    return 0;
+}
+
+void ph_pair_response(const float *sos, int s, int N, double J[16], std::vector<double> &H)
+{
+    const float *rs = sos + 6 * s, *rt = sos + 6 * (s + 1);
+    const double as1 = -(double)rs[4], as2 = -(double)rs[5];
+    const double bt1 = rt[1], bt2 = rt[2], at1 = -(double)rt[4], at2 = -(double)rt[5];
+    H.assign((size_t)4 * N, 0.0);
+    for (int m = 0; m < 4; ++m) {
+        // zero input from the unit joint state e_m: section s's output is
+        // section t's input, whose history is section s's history
+        double ys2 = m == 0, ys1 = m == 1, yt2 = m == 2, yt1 = m == 3;
+        for (int n = 0; n < N; ++n) {
+            const double ys = as1 * ys1 + as2 * ys2;
+            const double yt = ys + bt1 * ys1 + bt2 * ys2 + at1 * yt1 + at2 * yt2;
+            H[(size_t)m * N + n] = yt;
+            ys2 = ys1;
+            ys1 = ys;
+            yt2 = yt1;
+            yt1 = yt;
+        }
+        J[0 * 4 + m] = ys2;
+        J[1 * 4 + m] = ys1;
+        J[2 * 4 + m] = yt2;
+        J[3 * 4 + m] = yt1;
+    }
+}
+
+int ph_joint_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &out)
+{
+    const int stride = ph_tables(sos, ns, lanes, N, out);
+    const PhPairLayout P = ph_pair_layout(lanes, N);
+    const size_t base = out.size();
+    out.resize(base + (size_t)P.stride * (size_t)(ns / 2), 0.0f);
+    for (int p = 0; p < ns / 2; ++p) {
+        float *t = out.data() + base + (size_t)p * P.stride;
+        double J[16], Q[16], R[16];
+        std::vector<double> H;
+        ph_pair_response(sos, 2 * p, N, J, H);
+        std::memcpy(Q, J, sizeof(Q));
+        for (int k = 0; k < 5; ++k) {
+            put16(t + P.kJpow + 16 * k, Q);
+            m4_mul(Q, Q, Q);
+        }
+        double big = 0.0;
+        std::memcpy(R, J, sizeof(R));
+        for (int l = 0; l < lanes; ++l) {
+            for (int i = 0; i < 16; ++i) big = std::max(big, std::fabs(R[i]));
+            if (l < 16) put16(t + P.kJrow + 16 * l, R);
+            put16(t + P.kJfin + 16 * l, R);
+            if (l == lanes - 1) {
+                put16(t + P.kM, R);
+                t[P.kDepth] = (float)lookback_depth4(R);
+            }
+            m4_mul(R, J, R);
+        }
+        for (int i = 0; i < 4 * N; ++i) {
+            t[P.kH + i] = (float)H[(size_t)i];
+            big = std::max(big, std::fabs(H[(size_t)i]));
+        }
+        t[P.kDepth + 1] = (big <= kPairLimit) ? 1.0f : 0.0f;
+    }
+    return stride;
+}
+
 }  // namespace iirx
 
 namespace iirx {

@@ -33,6 +33,16 @@ int lookback_depth(const M2 &M);
 // per-section stride in floats.
 int ph_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &out);
 
+// PH tables for all sections followed by the tables of the section pairs
+// (0, 1), (2, 3), ... (layout kernels.hpp:ph_pair_layout); returns the
+// per-section stride in floats.
+int ph_joint_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &out);
+
+// The pair's joint N-step transition J (4 x 4 row-major) and the responses
+// H[m * N + n] of the second section's output to the unit joint states, in
+// FP64, for sections (s, t) of `sos`.
+void ph_pair_response(const float *sos, int s, int N, double J[16], std::vector<double> &H);
+
 // CR tables for all sections (layout kernels.hpp:cr_layout).  Returns the
 // stride, or -1 when the CR factors are ill-conditioned for this (filter, N):
 // some |e_n| is so small that the elimination multipliers f/e, d, c exceed

@@ -85,6 +85,7 @@ bool parse(int argc, char **argv, Options &o)
             else if (k == "PLR") o.kernel = IIRX_KERNEL_PLR;
             else if (k == "BS") o.kernel = IIRX_KERNEL_BLOCKSTATE;
             else if (k == "BS_BF16") o.kernel = IIRX_KERNEL_BLOCKSTATE_BF16;
+            else if (k == "PHJ") o.kernel = IIRX_KERNEL_PH_JOINT;
             else return false;
         } else if (a == "--block-size") o.block_size = std::atoi(v);
         else if (a == "--n-blocks") o.n_blocks = std::atoi(v);

@@ -86,11 +86,21 @@ typedef enum {
                                    (32, 64 or 128), block_size = tiles per
                                    chunk G (16, 32 or 64; 16 or 32 at L =
                                    128), chunk = G * L samples */
-    IIRX_KERNEL_BLOCKSTATE_BF16 = 5  /* the block-state form with the Toeplitz
+    IIRX_KERNEL_BLOCKSTATE_BF16 = 5, /* the block-state form with the Toeplitz
                                         and input products split three ways
                                         into BF16 (hi, mid, lo; six MFMA
                                         products per K slab) instead of FP16
                                         hi/lo; L = 64 or 128, G = 16 or 32 */
+    IIRX_KERNEL_PH_JOINT = 6   /* no counterpart in the reference: PH with
+                                  sections taken in pairs, one decoupled
+                                  lookback of the pair's joint 4-float state
+                                  per chunk instead of one per section (an
+                                  odd last section alone, and a pair whose
+                                  joint tables exceed 256, poles close
+                                  together near the unit circle, one section
+                                  at a time); same geometry, records,
+                                  streaming state and any-length rules as
+                                  PH */
 } iirx_kernel;
 
 /*

@@ -54,7 +54,8 @@ def test_plans_reject_rows_outside_the_contract():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", sorted(DESIGNS))
-@pytest.mark.parametrize("kernel,B,N", [("PH", 64, 32), ("STCR", 32, 64), ("DTCR", 64, 64), ("BS", 64, 64)])
+@pytest.mark.parametrize("kernel,B,N", [("PH", 64, 32), ("PHJ", 64, 32), ("STCR", 32, 64), ("DTCR", 64, 64),
+                                         ("BS", 64, 64)])
 def test_general_designs_on_the_gpu(device, name, kernel, B, N):
     import torch
     print(f"case general sos {name} {kernel}", flush=True)

@@ -21,9 +21,12 @@ pytestmark = pytest.mark.gpu
 GEOMS = [(32, 32), (32, 64), (64, 64), (64, 128)]
 ORDERS = [2, 4, 8, 16]
 # agreement with the float32 restatement, in float32 eps * max|ref|
-TOL_ULPS = {"PH": 64.0, "STCR": 64.0, "DTCR": 64.0}
-RESTATE = {"PH": O.ph, "STCR": O.stcr, "DTCR": O.dtcr}
-KERNELS = ["PH", "STCR", "DTCR"]
+# PHJ (PH with sections in pairs, one lookback per pair) computes the PH
+# algorithm with the carries of two sections combined before the correction,
+# so it is held to the PH restatement at the same tolerance
+TOL_ULPS = {"PH": 64.0, "PHJ": 64.0, "STCR": 64.0, "DTCR": 64.0}
+RESTATE = {"PH": O.ph, "PHJ": O.ph, "STCR": O.stcr, "DTCR": O.dtcr}
+KERNELS = ["PH", "PHJ", "STCR", "DTCR"]
 
 
 def run_gpu(device, kernel, sos, x, B, N, launches=1, **kw):
@@ -113,7 +116,8 @@ def test_launch_versioning_stale_flags(device, kernel, schedule):
             assert info["grid"] <= info["num_chunks"]
 
 
-@pytest.mark.parametrize("kernel,B,N,waves", [("PH", 64, 32, 16), ("STCR", 64, 64, 4), ("PH", 64, 16, 16)])
+@pytest.mark.parametrize("kernel,B,N,waves", [("PH", 64, 32, 16), ("STCR", 64, 64, 4), ("PH", 64, 16, 16),
+                                              ("PHJ", 64, 32, 16)])
 def test_ticket_groups_loop(device, kernel, B, N, waves):
     """Ticket schedule with more groups of chunks than resident workgroups:
     every workgroup claims several groups, one atomic ticket each, and a
@@ -185,7 +189,7 @@ def test_watchdog_terminates_and_reports(device, kernel):
 
 
 # offset of the lookback depth in the first section's table (kernels.hpp)
-DEPTH_AT = {"PH": 4 + 4 * 6 + 4, "STCR": 8 * 10 + 4 * 6 + 4, "DTCR": 8 * 10 + 4 * 6 + 4}
+DEPTH_AT = {"PH": 4 + 4 * 6 + 4, "PHJ": 4 + 4 * 6 + 4, "STCR": 8 * 10 + 4 * 6 + 4, "DTCR": 8 * 10 + 4 * 6 + 4}
 
 
 def resonator(r, theta):
@@ -221,8 +225,64 @@ def test_lookback_depths(device, kernel, filt, N, depth):
     assert ok or rel <= 10.0 * casc, rel
 
 
+def _pair_depth(sos, N):
+    """Smallest d <= 64 with ||M^d||_inf <= 2^-36 for the chunk transition M of
+    the pair (sections 0, 1) in FP64 from its one-step matrix, 0 if none."""
+    as1, as2 = -float(sos[0, 4]), -float(sos[0, 5])
+    bt1, bt2, at1, at2 = float(sos[1, 1]), float(sos[1, 2]), -float(sos[1, 4]), -float(sos[1, 5])
+    A4 = np.array([[0.0, 1.0, 0.0, 0.0], [as2, as1, 0.0, 0.0], [0.0, 0.0, 0.0, 1.0],
+                   [as2 + bt2, as1 + bt1, at2, at1]])
+    M = np.linalg.matrix_power(A4, 64 * N)
+    P = M.copy()
+    for d in range(1, 65):
+        if np.abs(P).sum(axis=1).max() <= 2.0 ** -36:
+            return d
+        P = P @ M
+    return 0
+
+
+@pytest.mark.parametrize("r,N,depth", [(0.995, 8, 11), (0.999, 16, 28), (0.9995, 8, 0)])
+def test_joint_lookback_depths(device, r, N, depth):
+    """The pair lookback of PHJ on two resonators (angles 1.0 and 0.5: poles
+    apart, so the pair runs joined): depth from the pair's chunk transition
+    in FP64, several partials folded, and at r = 0.9995 with 512-sample
+    chunks the unbounded wait for full records."""
+    n = 64 * N * 300
+    print(f"case joint lookback-depth r={r} N={N}", flush=True)
+    sos = np.concatenate([resonator(r, 1.0), resonator(r, 0.5)])
+    x = (1e-3 * iirx.fill_normal(12, n)).astype(np.float32)
+    tab, stride = iirx.plan_tables("PHJ", sos, n, block_size=64, n_blocks=N)
+    got_depth, joined = int(tab[2 * stride + 96]), tab[2 * stride + 97]
+    print(f"  depth={got_depth} joined={joined}")
+    assert got_depth == depth == _pair_depth(sos, N) and joined == 1.0
+    y, _ = run_gpu(device, "PHJ", sos, x, 64, N)
+    ref = O.golden(sos, x)
+    ok, err, mref, rel = O.gate(y, ref)
+    casc = O.gate(O.cascade_f32(sos, x), ref)[3]
+    print(f"  rel={rel:.2e} sequential-f32={casc:.2e}")
+    assert ok or rel <= 10.0 * casc, rel
+
+
+@pytest.mark.parametrize("order,wn,N", [(4, 0.002, 16), (8, 0.02, 32), (16, 0.05, 32)])
+def test_joint_falls_back_on_large_tables(device, order, wn, N):
+    """Narrow lowpass filters: the pairs' tables exceed the limit, so PHJ runs
+    the sections one at a time -- the PH arithmetic, bit for bit."""
+    n = 64 * N * 200
+    print(f"case joint fallback order={order} wn={wn}", flush=True)
+    sos, g, x = normal_workload(order, n, seed=14, wn=wn)
+    tab, stride = iirx.plan_tables("PHJ", sos, n, block_size=64, n_blocks=N)
+    ns = sos.shape[0]
+    pstride = (356 + 16 * 64 + 4 * N + 3) // 4 * 4
+    flags = [tab[ns * stride + p * pstride + 97] for p in range(ns // 2)]
+    assert max(flags) == 0.0, flags
+    y, _ = run_gpu(device, "PHJ", sos, x, 64, N)
+    y0, _ = run_gpu(device, "PH", sos, x, 64, N)
+    assert np.array_equal(y, y0)
+
+
 @pytest.mark.slow
-@pytest.mark.parametrize("kernel,B,N", [("PH", 64, 32), ("PH", 64, 64), ("STCR", 64, 64), ("DTCR", 64, 64)])
+@pytest.mark.parametrize("kernel,B,N", [("PH", 64, 32), ("PH", 64, 64), ("PHJ", 64, 32), ("STCR", 64, 64),
+                                         ("DTCR", 64, 64)])
 @pytest.mark.parametrize("order", ORDERS)
 def test_full_size_gate(device, kernel, B, N, order):
     """2^25 samples, the measured configuration (PH 64 x 32 is the bench's

@@ -159,6 +159,87 @@ def test_config_check(kernel, B, N, n, ok):
     assert (st == 0) == ok, st
 
 
+def _phj_pair_stride(N, L=64):
+    kH = 16 * 5 + 16 + 4 + 16 * 16 + 16 * L
+    return kH, (kH + 4 * N + 3) // 4 * 4
+
+
+@pytest.mark.parametrize("order,wn,N", [(4, 0.1, 32), (8, 0.05, 16), (6, 0.3, 64), (16, 0.1, 8), (4, 0.02, 32)])
+def test_ph_joint_tables_reproduce_the_cascade(order, wn, N):
+    """PH with sections in pairs: the pair tables (joint transition J and its
+    powers, the chunk transition, the responses H of the second section to the
+    joint lane state) drive a lane-by-lane FP64 recursion over the whole
+    signal -- zero-state passes of both sections per lane, y_t += H U, lane
+    end state e + J U -- that must equal the FP64 cascade.  A wrong table
+    entry, index or sign fails it; order 6 has an odd last section."""
+    sos, g = iirx.butterworth_sos(order, wn)
+    ns = sos.shape[0]
+    L = 64
+    n = L * N * 3
+    tab, stride = iirx.plan_tables("PHJ", sos, n, block_size=64, n_blocks=N)
+    kH, pstride = _phj_pair_stride(N, L)
+    base = ns * stride
+    assert tab.size == base + (ns // 2) * pstride
+    x = (g * iirx.fill_normal(5, n).astype(np.float64)).astype(np.float32)
+    y = x.astype(np.float64)
+    sec = 0
+    while sec < ns:
+        if sec + 1 == ns:
+            y = signal.sosfilt(sos[sec:sec + 1].astype(np.float64), y)
+            sec += 1
+            continue
+        t = tab[base + (sec // 2) * pstride: base + (sec // 2 + 1) * pstride].astype(np.float64)
+        # the joint one-step matrix in FP64 from the sections: state
+        # (y_s[n-2], y_s[n-1], y_t[n-2], y_t[n-1]), zero input
+        as1, as2 = -float(sos[sec, 4]), -float(sos[sec, 5])
+        bt1, bt2 = float(sos[sec + 1, 1]), float(sos[sec + 1, 2])
+        at1, at2 = -float(sos[sec + 1, 4]), -float(sos[sec + 1, 5])
+        A4 = np.array([[0.0, 1.0, 0.0, 0.0], [as2, as1, 0.0, 0.0], [0.0, 0.0, 0.0, 1.0],
+                       [as2 + bt2, as1 + bt1, at2, at1]])
+        J64 = np.linalg.matrix_power(A4, N)
+
+        def close(got, P, tol=2e-6):
+            # float32 rounding of each entry, on the scale of the matrix
+            # (entries far below it, or below float32's range, round away)
+            return np.max(np.abs(got.reshape(4, 4) - P)) <= tol * np.max(np.abs(P)) + 1e-37
+
+        for k in range(5):
+            assert close(t[16 * k:16 * k + 16], np.linalg.matrix_power(J64, 2 ** k)), k
+        assert close(t[80:96], np.linalg.matrix_power(J64, L), 1e-5)
+        for j in (0, 7, 15):
+            assert close(t[100 + 16 * j:116 + 16 * j], np.linalg.matrix_power(J64, j + 1), 1e-5), j
+        for l in (0, 31, 63):
+            assert close(t[356 + 16 * l:372 + 16 * l], np.linalg.matrix_power(J64, l + 1), 1e-5), l
+        Href = np.array([np.linalg.matrix_power(A4, k + 1)[3] for k in range(N)]).T  # H[m][n]
+        assert np.max(np.abs(t[kH:kH + 4 * N] - Href.reshape(-1))) <= 2e-6 * np.max(np.abs(Href))
+        # joined only while every table entry stays within 256
+        big = max(np.abs(Href).max(), max(np.abs(np.linalg.matrix_power(J64, l + 1)).max() for l in range(L)))
+        assert t[97] == (1.0 if big <= 256.0 else 0.0), big
+        J = t[0:16].reshape(4, 4)
+        H = t[kH:kH + 4 * N].reshape(4, N)
+        b1, b2, a1, a2 = (float(v) for v in tab[sec * stride: sec * stride + 4])
+        d1, d2, c1, c2 = (float(v) for v in tab[(sec + 1) * stride: (sec + 1) * stride + 4])
+        out = np.empty_like(y)
+        U = np.zeros(4)
+        xh2 = xh1 = 0.0
+        for ls in range(0, n, N):
+            xs = y[ls:ls + N]
+            x2, x1, y2, y1, v2, v1 = xh2, xh1, 0.0, 0.0, 0.0, 0.0
+            zs = np.empty(N)
+            for k in range(N):
+                yc = xs[k] + b1 * x1 + b2 * x2 + a1 * y1 + a2 * y2
+                vc = yc + d1 * y1 + d2 * y2 + c1 * v1 + c2 * v2
+                zs[k] = vc
+                x2, x1, y2, y1, v2, v1 = x1, xs[k], y1, yc, v1, vc
+            out[ls:ls + N] = zs + H.T @ U
+            U = np.array([y2, y1, zs[N - 2], zs[N - 1]]) + J @ U
+            xh2, xh1 = xs[N - 2], xs[N - 1]
+        y = out
+        sec += 2
+    ref = signal.sosfilt(sos.astype(np.float64), x.astype(np.float64))
+    assert np.max(np.abs(y - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
 def _bs_tables(sos, L, G=64, kernel="BS"):
     """Dequantised block-state tables (layout of kernels.hpp:bs_layout):
     FP16 hi/lo planes (BS) or BF16 hi/mid/lo planes (BS_BF16)."""

@@ -18,6 +18,7 @@ import oracle_lib as O
 @pytest.mark.parametrize("bad", [np.nan, np.inf])
 @pytest.mark.parametrize("kernel,B,N,order,early", [
     ("PH", 64, 32, 8, 0),
+    ("PHJ", 64, 32, 8, 0),
     ("STCR", 64, 64, 4, 0),
     ("DTCR", 64, 64, 4, 0),
     ("PLR", None, 64, 4, 0),

@@ -14,7 +14,7 @@ import oracle_lib as O
 from workloads import normal_workload
 
 WNS = [0.02, 0.1, 0.4]
-GEOM = {"PH": (64, 64), "STCR": (64, 64), "DTCR": (64, 64), "BS": (64, 64)}
+GEOM = {"PH": (64, 64), "PHJ": (64, 32), "STCR": (64, 64), "DTCR": (64, 64), "BS": (64, 64)}
 
 
 @functools.lru_cache(maxsize=8)

@@ -15,7 +15,8 @@ import cuda_recursive_filtering_amd as iirx
 import oracle_lib as O
 from workloads import normal_workload
 
-CASES = [("PH", (64, 32), False), ("STCR", (64, 64), False), ("STCR", (32, 32), True),
+CASES = [("PH", (64, 32), False), ("PHJ", (64, 32), False), ("PHJ", (32, 64), False), ("STCR", (64, 64), False),
+         ("STCR", (32, 32), True),
          ("DTCR", (64, 64), False), ("DTCR", (32, 32), True)]
 
 
