@@ -157,7 +157,8 @@ __device__ __forceinline__ void batch_bs_chunk(const BatchArgs &a, uint32_t chun
         *reinterpret_cast<f4 *>(lds + (16 * nb + col) * kBsTilePitch + 16 * mb + 4 * g) = y;
     };
     bs_chunk<L, G, 4>(xr, chunk, part_flag, full_flag, a.lb, coef + Y.kMc, (int)ctab[Y.kDepth], lp, a.spin_limit,
-                      a.err, first, (int)ctab[Y.kScale + 0], (int)ctab[Y.kScale + 1], ops, store);
+                      a.err, first, (int)ctab[Y.kScale + 0], (int)ctab[Y.kScale + 1],
+                      (int)ctab[Y.kScale + kBsScaleFp32Level], (int)ctab[Y.kScale + kBsScaleLevels], ops, store);
     __syncthreads();
     float *dst = a.out + base + lane;
     if (r == CH && a.vec16) {

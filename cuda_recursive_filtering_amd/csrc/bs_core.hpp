@@ -188,7 +188,42 @@ __device__ __forceinline__ void shift_tiles_d(const d4 (&Z)[NB], d4 (&S)[NB])
 }
 
 template <int G>
-constexpr int bs_levels() { return G == 16 ? 4 : G == 32 ? 5 : 6; }
+constexpr int bs_levels() { return bs_scan_levels(G); }
+
+// an FP32 operand block (word 4 lane + m = M[l%16][l/16 + 4 m]) with its rows
+// reordered for the FP32 matrix core: row 4 g + r takes state g + 4 r, so that
+// the 16 x 16 x 4 FP32 product lands in the FP64 result layout (lane g,
+// register r = state g + 4 r)
+__device__ __forceinline__ f4 rows_for_fp32(const f4 &M)
+{
+    const int l = threadIdx.x & 63;
+    const int src = (l & 48) | bs_state_of_row(l & 15);
+    f4 r;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) r[m] = __shfl(M[m], src, 64);
+    return r;
+}
+
+// Z += M S on the FP32 matrix core: S rounded to float32 and shifted by D
+// tiles, M with rows_for_fp32 applied, the product added to Z in FP64
+template <int D, int NSL, int NB>
+__device__ __forceinline__ void level_fp32(const f4 &Mp, d4 (&Z)[NB])
+{
+    f4 Zs[NB], S[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Zs[nb][j] = (j < NSL) ? (float)Z[nb][j] : 0.0f;
+    shift_tiles<D, NSL, NB>(Zs, S);
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        if (D < 16 || nb >= (D >> 4)) {
+            const f4 acc = apply<NSL>(Mp, S[nb], f4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+            for (int j = 0; j < NSL; ++j) Z[nb][j] += (double)acc[j];
+        }
+    }
+}
 
 template <int NB, int KQ>
 struct BsRaw {
@@ -199,7 +234,8 @@ template <int L, int G, int NSL, int SPLIT = kBsSplitF16, class Ops, class Store
 __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32_t chunk, uint32_t part_flag,
                                          uint32_t full_flag, uint64_t *lb, const float *Mc, int depth, float *lp,
                                          uint32_t spin_limit, uint32_t *err, uint32_t first, int sT, int sG,
-                                         const Ops &ops, const Store &store, const float *start = nullptr,
+                                         int lv32, int nlv, const Ops &ops, const Store &store,
+                                         const float *start = nullptr,
                                          float *tail = nullptr)
 {
     constexpr int MB = L / 16;  // output row blocks of T
@@ -252,15 +288,24 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
     }
     // 3. tile end states, log2(G)-level scan on the FP64 matrix core (FP32
     //    operands Phi^(2^j), FP64 accumulation; levels D >= 16 skip the
-    //    all-zero blocks)
+    //    all-zero blocks).  Only the first nlv levels run (the longer lags
+    //    weigh at most 2^-36 in total), those from lv32 on with the product on
+    //    the FP32 matrix core (||Phi^(2^j)||_inf <= 2^-10 there)
     static_for<0, bs_levels<G>(), 1>([&](auto LV) {
-        constexpr int D = 1 << decltype(LV)::value;
-        d4 S[NB];
-        shift_tiles_d<D, NSL, NB>(Z, S);
-        const f4 P = ops.pow2(decltype(LV)::value);
+        constexpr int j = decltype(LV)::value;
+        constexpr int D = 1 << j;
+        if (j < nlv) {
+            if (j < lv32) {
+                d4 S[NB];
+                shift_tiles_d<D, NSL, NB>(Z, S);
+                const f4 P = ops.pow2(j);
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-            if (D < 16 || nb >= (D >> 4)) Z[nb] = apply_d<NSL>(P, S[nb], Z[nb]);
+                for (int nb = 0; nb < NB; ++nb)
+                    if (D < 16 || nb >= (D >> 4)) Z[nb] = apply_d<NSL>(P, S[nb], Z[nb]);
+            } else {
+                level_fp32<D, NSL, NB>(rows_for_fp32(ops.pow2(j)), Z);
+            }
+        }
     });
 
     // 4. chunk partial = Z(:, G-1): lanes 15, 31, 47, 63 hold states g + 4 j
@@ -351,13 +396,35 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
         U[0] = apply_d<NSL>(ops.phi(), b0, d4{0.0, 0.0, 0.0, 0.0});
     }
     static_for<0, 4, 1>([&](auto LV) {
-        constexpr int D = 1 << decltype(LV)::value;
-        d4 S[1];
-        shift_tiles_d<D, NSL, 1>(U, S);
-        U[0] = apply_d<NSL>(ops.pow2(decltype(LV)::value), S[0], U[0]);
+        constexpr int j = decltype(LV)::value;
+        constexpr int D = 1 << j;
+        if (j < nlv) {
+            if (j < lv32) {
+                d4 S[1];
+                shift_tiles_d<D, NSL, 1>(U, S);
+                U[0] = apply_d<NSL>(ops.pow2(j), S[0], U[0]);
+            } else {
+                level_fp32<D, NSL, 1>(rows_for_fp32(ops.pow2(j)), U);
+            }
+        }
     });
+    // Phi^(16 nb) W for the blocks past the first (lags >= 16: level 4)
+    if (nlv > 4) {
+        if (4 < lv32) {
 #pragma unroll
-    for (int nb = 1; nb < NB; ++nb) Z[nb] = apply_d<NSL>(ops.pow16(nb), U[0], Z[nb]);
+            for (int nb = 1; nb < NB; ++nb) Z[nb] = apply_d<NSL>(ops.pow16(nb), U[0], Z[nb]);
+        } else {
+            f4 Us;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Us[j] = (j < NSL) ? (float)U[0][j] : 0.0f;
+#pragma unroll
+            for (int nb = 1; nb < NB; ++nb) {
+                const f4 acc = apply<NSL>(rows_for_fp32(ops.pow16(nb)), Us, f4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+                for (int j = 0; j < NSL; ++j) Z[nb][j] += (double)acc[j];
+            }
+        }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) Z[0][j] += U[0][j];
 

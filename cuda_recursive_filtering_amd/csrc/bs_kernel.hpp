@@ -54,6 +54,7 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
     const cfloat *ctab = as_const(a.coef);
     const int sT = (int)ctab[Y.kScale + 0], sG = (int)ctab[Y.kScale + 1];
     const int depth = (int)ctab[Y.kDepth];
+    const int lv32 = (int)ctab[Y.kScale + kBsScaleFp32Level], nlv = (int)ctab[Y.kScale + kBsScaleLevels];
     f4 *xbuf = xbuf_w[wave];
     float *lp = lp_w[wave];
 
@@ -119,7 +120,7 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
         // streaming: a.zi is the balanced state before the signal, a.tail
         // receives the state entering the last chunk's last tile
         bs_chunk<L, G, NSL, SPLIT>(xr, chunk, part_flag, full_flag, a.lb, tab + Y.kMc, depth, lp, a.spin_limit, a.err, 0u,
-                            sT, sG, ops, store, a.zi, chunk + 1u == a.num_chunks ? a.tail : nullptr);
+                            sT, sG, lv32, nlv, ops, store, a.zi, chunk + 1u == a.num_chunks ? a.tail : nullptr);
     };
 
     const uint32_t GW = gridDim.x * W, gw = blockIdx.x * W + wave;  // this wave among the grid's

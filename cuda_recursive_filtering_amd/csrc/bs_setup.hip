@@ -299,6 +299,47 @@ __global__ __launch_bounds__(kSetupThreads) void bs_setup_kernel(const BsSetupJo
         mm16(Pw, Mc, Pw);
     }
     if (t == 0) out[Y.kDepth] = (float)depth;
+
+    // levels of the in-chunk tile scan (precompute.cpp: bs_tables)
+    auto norm_inf = [&](const double *m) {
+        if (t < 16) {
+            double s = 0.0;
+            for (int c = 0; c < 16; ++c) s += __builtin_fabs(m[t * 16 + c]);
+            red[t] = s;
+        }
+        __syncthreads();
+        double nrm = 0.0;
+        for (int r = 0; r < 16; ++r) nrm = fmax(nrm, red[r]);
+        __syncthreads();
+        return nrm;
+    };
+    __shared__ double nrmv[65];
+    const int lvG = bs_scan_levels(G);
+    Pw[t] = Pm[0][t];
+    __syncthreads();
+    bool finite = true;
+    for (int i = 1; i <= G; ++i) {
+        const double v = norm_inf(Pw);
+        finite = finite && __builtin_isfinite(v);
+        if (t == 0) nrmv[i] = v;
+        mm16(Pw, Pm[0], Pw);
+    }
+    int nlv = lvG, lv32 = lvG;
+    if (finite) {
+        for (int j = 0; j < lvG; ++j) {
+            double tail = 0.0;
+            for (int i = 1 << j; i <= G; ++i) tail += nrmv[i];
+            if (tail <= kLookbackTail) {
+                nlv = j;
+                break;
+            }
+        }
+        for (int j = lvG - 1; j >= 0 && norm_inf(Pm[j]) <= kBsFp32Norm; --j) lv32 = j;
+    }
+    if (t == 0) {
+        out[Y.kScale + kBsScaleFp32Level] = (float)lv32;
+        out[Y.kScale + kBsScaleLevels] = (float)nlv;
+    }
 }
 
 }  // namespace

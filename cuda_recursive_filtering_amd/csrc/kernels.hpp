@@ -355,7 +355,19 @@ __host__ __device__ inline uint16_t bf16x3_bits(double v, int hl)
     }
     return b;
 }
-// scale slots: 0 T, 1 Gam
+// scale slots: 0 T, 1 Gam, then the levels of the in-chunk tile scan
+// Z_k = sum_i Phi^i V_(k-i) (level j adds the lags 2^j .. 2^(j+1) - 1):
+// kBsScaleLevels = levels run (the lags from 2^levels on weigh at most 2^-36
+// in total, sum ||Phi^i||_inf up to the chunk transition, the bound the
+// lookback folds to), kBsScaleFp32Level = the first level run on the FP32
+// matrix core instead of the FP64 one (from it on every Phi^(2^j) has
+// ||.||_inf <= 2^-10, so its product stays below 2^-10 of the state and its
+// float32 rounding below 2^-34 of it).  The chunk-carry injection Phi^(k+1)
+// X_in follows the same levels.
+constexpr int kBsScaleFp32Level = 2;
+constexpr int kBsScaleLevels = 3;
+constexpr double kBsFp32Norm = 0.0009765625;  // 2^-10
+constexpr int bs_scan_levels(int G) { return G == 16 ? 4 : G == 32 ? 5 : 6; }
 
 // ---------------------------------------------------------------------------
 // Ragged batch (batch.hip): channel c holds samples [offset, offset + length)

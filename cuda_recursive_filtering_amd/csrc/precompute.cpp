@@ -347,6 +347,17 @@ Mat mat_mul16(const Mat &a, const Mat &b)
     return c;
 }
 
+double norm_inf16(const Mat &m)
+{
+    double nrm = 0.0;
+    for (int r = 0; r < 16; ++r) {
+        double s = 0.0;
+        for (int c = 0; c < 16; ++c) s += std::fabs(m[r * 16 + c]);
+        nrm = std::max(nrm, s);
+    }
+    return nrm;
+}
+
 // scale exponent s so that max|m| * 2^s < 2^14 (fp16 holds up to 65504)
 int pow2_scale(const double *m, size_t n)
 {
@@ -554,6 +565,36 @@ int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out, i
         Pw = mat_mul16(Pw, Mc);
     }
     out[Y.kDepth] = (float)depth;
+    // levels of the in-chunk tile scan (kernels.hpp: kBsScaleFp32Level,
+    // kBsScaleLevels): the lags i >= 2^j are dropped once sum_{i=2^j}^{G}
+    // ||Phi^i||_inf <= 2^-36, and the levels from the first one whose
+    // Phi^(2^j), and every higher one, has ||.||_inf <= 2^-10 run on the FP32
+    // matrix core
+    {
+        const int lvG = bs_scan_levels(G);
+        std::vector<double> nrm((size_t)G + 1, 0.0);
+        Mat Pi = Phi;
+        bool finite = true;
+        for (int i = 1; i <= G; ++i) {
+            nrm[i] = norm_inf16(Pi);
+            finite = finite && std::isfinite(nrm[i]);
+            Pi = mat_mul16(Pi, Phi);
+        }
+        int nlv = lvG, lv32 = lvG;
+        if (finite) {
+            for (int j = 0; j < lvG; ++j) {
+                double tail = 0.0;
+                for (int i = 1 << j; i <= G; ++i) tail += nrm[i];
+                if (tail <= kLookbackTail) {
+                    nlv = j;
+                    break;
+                }
+            }
+            for (int j = lvG - 1; j >= 0 && norm_inf16(P[j]) <= kBsFp32Norm; --j) lv32 = j;
+        }
+        out[Y.kScale + kBsScaleFp32Level] = (float)lv32;
+        out[Y.kScale + kBsScaleLevels] = (float)nlv;
+    }
     return Y.stride;
 }
 

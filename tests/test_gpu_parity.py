@@ -266,7 +266,10 @@ def test_joint_lookback_depths(device, r, N, depth):
 @pytest.mark.parametrize("order,wn,N", [(4, 0.002, 16), (8, 0.02, 32), (16, 0.05, 32)])
 def test_joint_falls_back_on_large_tables(device, order, wn, N):
     """Narrow lowpass filters: the pairs' tables exceed the limit, so PHJ runs
-    the sections one at a time -- the PH arithmetic, bit for bit."""
+    the sections one at a time -- the PH arithmetic.  Two launches of either
+    kernel need not agree bit for bit: a chunk folds the partials after the
+    nearest predecessor that is already full, so the rounding of its carry
+    depends on timing.  The two agree within PH's own error."""
     n = 64 * N * 200
     print(f"case joint fallback order={order} wn={wn}", flush=True)
     sos, g, x = normal_workload(order, n, seed=14, wn=wn)
@@ -277,7 +280,11 @@ def test_joint_falls_back_on_large_tables(device, order, wn, N):
     assert max(flags) == 0.0, flags
     y, _ = run_gpu(device, "PHJ", sos, x, 64, N)
     y0, _ = run_gpu(device, "PH", sos, x, 64, N)
-    assert np.array_equal(y, y0)
+    ref = O.golden(sos, x)
+    e0 = float(np.max(np.abs(y0.astype(np.float64) - ref)))
+    d = float(np.max(np.abs(y.astype(np.float64) - y0)))
+    print(f"  PH err={e0:.3e} |PHJ - PH|={d:.3e}")
+    assert d <= max(2.0 * e0, 64 * np.finfo(np.float32).eps * np.max(np.abs(ref))), (d, e0)
 
 
 @pytest.mark.slow

@@ -331,6 +331,41 @@ def test_blockstate_tables_reproduce_the_cascade(order, kernel, L, G):
     assert depth >= 1
 
 
+@pytest.mark.parametrize("wn", [0.02, 0.1, 0.4])
+@pytest.mark.parametrize("order", [2, 4, 8, 16])
+@pytest.mark.parametrize("L,G", [(32, 64), (64, 64), (64, 32), (128, 16)])
+def test_blockstate_scan_levels(order, wn, L, G):
+    """The in-chunk scan levels in the scale slots 2 and 3: with `levels`
+    levels run, the dropped lags i >= 2^levels weigh sum ||Phi^i||_inf <=
+    2^-36 up to the chunk transition Phi^G, and one level fewer would not
+    (levels is the smallest such); every level from `fp32_level` on has
+    ||Phi^(2^j)||_inf <= 2^-10 and the one before it does not.  Norms from the
+    dequantised float32 Phi in numpy; cases within 0.1% of a threshold are
+    not held to minimality."""
+    sos, _ = iirx.butterworth_sos(order, wn)
+    tab, _ = iirx.plan_tables("BS", sos, G * L * 4, block_size=G, n_blocks=L)
+    T, Om, Gm, P0, Phi, Mc, depth = _bs_tables(sos, L, G)
+    MB, KQ = L // 16, L // 32
+    kS = 2 * MB * KQ * 256 + 2 * KQ * 256 + 6 * 256 + 3 * 256 + 256 + MB * 256
+    lv32, nlv = int(tab[kS + 2]), int(tab[kS + 3])
+    lvG = {16: 4, 32: 5, 64: 6}[G]
+    assert 0 <= nlv <= lvG and 0 <= lv32 <= lvG
+    nrm = [0.0] + [np.abs(np.linalg.matrix_power(Phi, i)).sum(axis=1).max() for i in range(1, G + 1)]
+    tail = lambda j: sum(nrm[1 << j:G + 1])
+    bound = 2.0 ** -36
+    assert tail(nlv) <= bound * 1.001 or nlv == lvG
+    if nlv > 0 and abs(tail(nlv - 1) / bound - 1) > 1e-3:
+        assert tail(nlv - 1) > bound
+    pn = [np.abs(np.linalg.matrix_power(Phi, 1 << j)).sum(axis=1).max() for j in range(lvG)]
+    for j in range(lv32, lvG):
+        assert pn[j] <= 2.0 ** -10 * 1.001
+    if lv32 > 0 and abs(pn[lv32 - 1] / 2.0 ** -10 - 1) > 1e-3:
+        assert pn[lv32 - 1] > 2.0 ** -10
+    # at Wn = 0.1 the long lags are dropped: the filters decay well within a chunk
+    if wn == 0.1 and G * L >= 4096:
+        assert nlv < lvG
+
+
 @pytest.mark.parametrize("L,G", [(64, 16), (128, 32)])
 @pytest.mark.parametrize("order", [8, 16])
 def test_blockstate_bf16_split_holds_24_bits(L, G, order):
