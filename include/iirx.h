@@ -67,8 +67,10 @@ typedef enum {
 typedef enum {
     IIRX_SCHEDULE_PERSISTENT = 0, /* one workgroup per resident slot, chunks
                                      assigned round-robin; no per-chunk atomic */
-    IIRX_SCHEDULE_TICKET = 1      /* one workgroup per chunk, chunk order from
-                                     an atomic ticket (ph_kernels.cuh:227-229) */
+    IIRX_SCHEDULE_TICKET = 1      /* chunk order from an atomic ticket
+                                     (ph_kernels.cuh:227-229): one per chunk,
+                                     or one per group of consecutive chunks
+                                     (PH, STCR) */
 } iirx_schedule;
 
 /* Algorithm selection (the KERNEL_* build flag, test_accuracy.cu:55-66). */
@@ -127,8 +129,11 @@ typedef enum {
  *               the device holds at once (occupancy API) and relies on them
  *               being co-resident; a bounded-spin watchdog turns a violated
  *               assumption into IIRX_ERR_TIMEOUT instead of a hang.  TICKET
- *               needs no co-residency but pays one serialised device-scope
- *               atomic per chunk (about 12 ns each on MI355X).
+ *               needs no co-residency: workgroups draw device-scope atomic
+ *               tickets in order (about 12 ns each, serialised, on MI355X),
+ *               one per group of consecutive chunks for PH and STCR (16 at
+ *               N_BLOCKS <= 32), one per chunk for DTCR, PLR and the
+ *               block-state form.
  */
 typedef struct {
     int32_t kernel;
