@@ -230,7 +230,7 @@ struct BsRaw {
     f4 a[NB][KQ], b[NB][KQ];
 };
 
-template <int L, int G, int NSL, int SPLIT = kBsSplitF16, class Ops, class Store>
+template <int L, int G, int NSL, int SPLIT = kBsSplitF16, bool LINES = false, class Ops, class Store>
 __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32_t chunk, uint32_t part_flag,
                                          uint32_t full_flag, uint64_t *lb, const float *Mc, int depth, float *lp,
                                          uint32_t spin_limit, uint32_t *err, uint32_t first, int sT, int sG,
@@ -290,22 +290,27 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
     //    operands Phi^(2^j), FP64 accumulation; levels D >= 16 skip the
     //    all-zero blocks).  Only the first nlv levels run (the longer lags
     //    weigh at most 2^-36 in total), those from lv32 on with the product on
-    //    the FP32 matrix core (||Phi^(2^j)||_inf <= 2^-10 there)
+    //    the FP32 matrix core (||Phi^(2^j)||_inf <= 2^-4 there)
+    // (the FP64 levels, then the FP32 ones, each a uniform branch that leaves
+    // Z in place: a three-way branch per level made the compiler copy Z
+    // between the paths, about 300 extra moves per chunk)
+    const int n64 = lv32 < nlv ? lv32 : nlv;
     static_for<0, bs_levels<G>(), 1>([&](auto LV) {
         constexpr int j = decltype(LV)::value;
         constexpr int D = 1 << j;
-        if (j < nlv) {
-            if (j < lv32) {
-                d4 S[NB];
-                shift_tiles_d<D, NSL, NB>(Z, S);
-                const f4 P = ops.pow2(j);
+        if (j < n64) {
+            d4 S[NB];
+            shift_tiles_d<D, NSL, NB>(Z, S);
+            const f4 P = ops.pow2(j);
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb)
-                    if (D < 16 || nb >= (D >> 4)) Z[nb] = apply_d<NSL>(P, S[nb], Z[nb]);
-            } else {
-                level_fp32<D, NSL, NB>(rows_for_fp32(ops.pow2(j)), Z);
-            }
+            for (int nb = 0; nb < NB; ++nb)
+                if (D < 16 || nb >= (D >> 4)) Z[nb] = apply_d<NSL>(P, S[nb], Z[nb]);
         }
+    });
+    static_for<0, bs_levels<G>(), 1>([&](auto LV) {
+        constexpr int j = decltype(LV)::value;
+        constexpr int D = 1 << j;
+        if (j >= n64 && j < nlv) level_fp32<D, NSL, NB>(rows_for_fp32(ops.pow2(j)), Z);
     });
 
     // 4. chunk partial = Z(:, G-1): lanes 15, 31, 47, 63 hold states g + 4 j
@@ -398,15 +403,16 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
     static_for<0, 4, 1>([&](auto LV) {
         constexpr int j = decltype(LV)::value;
         constexpr int D = 1 << j;
-        if (j < nlv) {
-            if (j < lv32) {
-                d4 S[1];
-                shift_tiles_d<D, NSL, 1>(U, S);
-                U[0] = apply_d<NSL>(ops.pow2(j), S[0], U[0]);
-            } else {
-                level_fp32<D, NSL, 1>(rows_for_fp32(ops.pow2(j)), U);
-            }
+        if (j < n64) {
+            d4 S[1];
+            shift_tiles_d<D, NSL, 1>(U, S);
+            U[0] = apply_d<NSL>(ops.pow2(j), S[0], U[0]);
         }
+    });
+    static_for<0, 4, 1>([&](auto LV) {
+        constexpr int j = decltype(LV)::value;
+        constexpr int D = 1 << j;
+        if (j >= n64 && j < nlv) level_fp32<D, NSL, 1>(rows_for_fp32(ops.pow2(j)), U);
     });
     // Phi^(16 nb) W for the blocks past the first (lags >= 16: level 4)
     if (nlv > 4) {
@@ -447,14 +453,42 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
 #pragma unroll
         for (int j = 0; j < 4; ++j) tail[g + 4 * j] = (j < NSL) ? S[NB - 1][j] : 0.0f;
     }
-    // a store covers 64 B of each of 16 tiles; the two halves of a 128-byte
-    // line (row blocks mb, mb + 1) are issued back to back and through the L2
-    // (not nontemporal), so that they merge there: measured WRITE_SIZE 1.06x
-    // the output bytes this way, 1.43x with mb outer and nontemporal stores
+    if constexpr (LINES) {
+        // whole 128-byte lines: row blocks mb and mb + 1 of a tile are one
+        // line, held by lanes col and col + 8 of the two blocks; a DPP
+        // rotation by 8 inside each 16-lane row swaps the halves so that one
+        // store writes the lines of tiles 0..7 of the block and the next those
+        // of tiles 8..15.  store(tile, row, y): samples row .. row + 3 of tile.
+        const bool lo = col < 8;
+        const int row = 16 * (col >> 3) + 4 * g;
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
+        for (int nb = 0; nb < NB; ++nb) {
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) store(nb, mb, apply<NSL>(Ob[mb], S[nb], Y0[mb][nb]));
+            for (int mb = 0; mb < MB; mb += 2) {
+                const f4 ya = apply<NSL>(Ob[mb], S[nb], Y0[mb][nb]);
+                const f4 yb = apply<NSL>(Ob[mb + 1], S[nb], Y0[mb + 1][nb]);
+                f4 A, B;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float ra = row_ror<8>(ya[j]), rb = row_ror<8>(yb[j]);
+                    A[j] = lo ? ya[j] : rb;
+                    B[j] = lo ? ra : yb[j];
+                }
+                store(16 * nb + (col & 7), 16 * mb + row, A);
+                store(16 * nb + 8 + (col & 7), 16 * mb + row, B);
+            }
+        }
+    } else {
+        // a store covers 64 B of each of 16 tiles; the two halves of a 128-byte
+        // line (row blocks mb, mb + 1) are issued back to back and through the L2
+        // (not nontemporal), so that they merge there: measured WRITE_SIZE 1.06x
+        // the output bytes this way, 1.43x with mb outer and nontemporal stores.
+        // store(nb, mb, y): row block mb of tile block nb.
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+#pragma unroll
+            for (int mb = 0; mb < MB; ++mb) store(nb, mb, apply<NSL>(Ob[mb], S[nb], Y0[mb][nb]));
+        }
     }
 }
 

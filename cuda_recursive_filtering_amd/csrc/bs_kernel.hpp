@@ -73,7 +73,9 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
     // X as x32 B operands: tile 16 nb + col, K slab q, K rows 8 g + j =
     // samples 32 q + 4 g + j (a) and 32 q + 16 + 4 g + j - 4 (b).  The chunk
     // is fetched HBM -> LDS by global_load_lds (no registers held while it
-    // is in flight); read_chunk waits for it.
+    // is in flight); read_chunk waits for it.  (Fetching whole 128-byte lines
+    // per instruction, the two halves of a tile's line on lanes col and col +
+    // 8, measured no faster at orders 2 and 8 and 25% slower at order 16.)
     typedef BsRaw<NB, KQ> Raw;
     auto fetch_chunk = [&](uint32_t chunk) {
         const long base = (long)chunk * CH;
@@ -110,16 +112,13 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
     const BsLdsOps<L, SPLIT> ops{lT, lG, lP, tab, lane};
     auto process = [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag, const Raw &xr) {
         float *out = a.out + (long)chunk * CH;
-        // a store covers 64 B of each of 16 tiles; the two halves of a 128-byte
-        // line (row blocks mb, mb + 1) are issued back to back and through the
-        // L2 (not nontemporal), so that they merge there: measured WRITE_SIZE
-        // 1.06x the output bytes this way, 1.43x with mb outer and nontemporal stores
-        auto store = [&](int nb, int mb, const f4 &y) {
-            *reinterpret_cast<f4 *>(out + (long)(16 * nb + col) * L + 16 * mb + 4 * g) = y;
+        // whole 128-byte lines per store (bs_chunk's LINES form), through the L2
+        auto store = [&](int tile, int row, const f4 &y) {
+            *reinterpret_cast<f4 *>(out + (long)tile * L + row) = y;
         };
         // streaming: a.zi is the balanced state before the signal, a.tail
         // receives the state entering the last chunk's last tile
-        bs_chunk<L, G, NSL, SPLIT>(xr, chunk, part_flag, full_flag, a.lb, tab + Y.kMc, depth, lp, a.spin_limit, a.err, 0u,
+        bs_chunk<L, G, NSL, SPLIT, true>(xr, chunk, part_flag, full_flag, a.lb, tab + Y.kMc, depth, lp, a.spin_limit, a.err, 0u,
                             sT, sG, lv32, nlv, ops, store, a.zi, chunk + 1u == a.num_chunks ? a.tail : nullptr);
     };
 

@@ -568,7 +568,7 @@ int bs_tables(const float *sos, int ns, int L, int G, std::vector<float> &out, i
     // levels of the in-chunk tile scan (kernels.hpp: kBsScaleFp32Level,
     // kBsScaleLevels): the lags i >= 2^j are dropped once sum_{i=2^j}^{G}
     // ||Phi^i||_inf <= 2^-36, and the levels from the first one whose
-    // Phi^(2^j), and every higher one, has ||.||_inf <= 2^-10 run on the FP32
+    // Phi^(2^j), and every higher one, has ||.||_inf <= 2^-4 run on the FP32
     // matrix core
     {
         const int lvG = bs_scan_levels(G);
