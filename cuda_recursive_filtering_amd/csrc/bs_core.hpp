@@ -321,26 +321,28 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
     }
     // the predecessor's record is in flight while Y0 = T X runs on the matrix cores
     const uint64_t poll0 = (chunk > first) ? lookback_first_poll<16>(lb, 32, chunk) : 0ull;
+    // row block mb of tile block nb of T X (FP16 split)
+    auto tx_block = [&](int mb, int nb) {
+        const int qmax = bs_tq(mb);  // T(16 mb + r, 32 q + c) = 0 beyond
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) {
+            if (q > qmax) break;
+            const h8 th = ops.tmat(0, mb, q), tl = ops.tmat(1, mb, q);
+            acc = mfma_h(tl, xh[nb][q], acc);
+            acc = mfma_h(th, xl[nb][q], acc);
+            acc = mfma_h(th, xh[nb][q], acc);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = ldexpf(acc[j], -(sT + eX));
+        return acc;
+    };
     f4 Y0[MB][NB];
     if constexpr (SPLIT == kBsSplitF16) {
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) {
-            const int qmax = bs_tq(mb);  // T(16 mb + r, 32 q + c) = 0 beyond
+        for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                f4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int q = 0; q < KQ; ++q) {
-                    if (q > qmax) break;
-                    const h8 th = ops.tmat(0, mb, q), tl = ops.tmat(1, mb, q);
-                    acc = mfma_h(tl, xh[nb][q], acc);
-                    acc = mfma_h(th, xl[nb][q], acc);
-                    acc = mfma_h(th, xh[nb][q], acc);
-                }
-#pragma unroll
-                for (int j = 0; j < 4; ++j) Y0[mb][nb][j] = ldexpf(acc[j], -(sT + eX));
-            }
-        }
+            for (int nb = 0; nb < NB; ++nb) Y0[mb][nb] = tx_block(mb, nb);
     } else {
         // slab by slab, so that each slab's split is formed once for all row blocks
 #pragma unroll

@@ -91,9 +91,51 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
                                                      16, 0, 0);
                 }
     };
+    // STAGED (bs_staged geometries): the chunk moves through the wave's LDS
+    // buffer with contiguous 1 KB accesses both ways.  Tile t's 16-byte piece p
+    // (samples 4 p .. 4 p + 3) sits at slot 16 t + ((p + t) & 15), a rotation
+    // by the tile index, so that the B-operand reads (piece p of 16 tiles
+    // across a row of lanes), the output writes and the contiguous DMA fills
+    // and read-backs all touch distinct 16-byte bank groups (32 or 64 banks).
+    // The fill of instruction i takes tiles 4 i .. 4 i + 3: lane j fetches the
+    // piece that belongs at slot 64 i + j.
+    constexpr bool STAGED = bs_staged(L, G, SPLIT);
+    static_assert(!STAGED || L == 64, "the staged layout rotates the 16 pieces of a 64-sample tile");
+    constexpr int NST = CH / 256;  // 1 KB instructions per chunk
+    // the fill / store offset of lane j in instruction i: tile 4 i + j / 16,
+    // piece ((j & 15) - t) & 15, which repeats with i mod 4 (floats, without
+    // the 256 i of the instruction)
+    int goff[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int t = 4 * k + (lane >> 4);
+        goff[k] = (lane >> 4) * L + 4 * (((lane & 15) - t) & 15);
+    }
+    auto fetch_staged = [&](uint32_t chunk) {
+        const float *src = a.in + (long)chunk * CH;
+#pragma unroll
+        for (int i = 0; i < NST; ++i)
+            __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void *)(src + 256 * i + goff[i & 3]),
+                                             (__attribute__((address_space(3))) void *)(&xbuf[i * 64]), 16, 0, 0);
+    };
     // vmcnt counts loads and stores together and retires them in issue
     // order, so after the first chunk the fill has landed once at most the
-    // MB * NB output stores issued after it are outstanding
+    // MB * NB output stores issued after it are outstanding (STAGED: NST / 2)
+    auto read_staged = [&](Raw &r, bool first) {
+        if (first)
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(%0)" ::"n"(NST / 2) : "memory");
+        // tile 16 nb + col: slot 256 nb + 16 col + ((piece + col) & 15)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int q = 0; q < KQ; ++q) {
+                r.a[nb][q] = xbuf[256 * nb + 16 * col + ((8 * q + g + col) & 15)];
+                r.b[nb][q] = xbuf[256 * nb + 16 * col + ((8 * q + 4 + g + col) & 15)];
+            }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // reads done before the buffer is reused
+    };
     auto read_chunk = [&](Raw &r, bool first) {
         if (first)
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -110,6 +152,34 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
     };
 
     const BsLdsOps<L, SPLIT> ops{lT, lG, lP, tab, lane};
+    // STAGED: the output blocks into the wave's LDS buffer (free once the
+    // chunk's input is in registers), then read back 1 KB per instruction;
+    // the next chunk's fill is issued between the read-back and the stores,
+    // so that the next read's vmcnt wait covers the fill with the last NST / 2
+    // stores still in flight
+    auto process_staged = [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag, const Raw &xr, uint32_t next) {
+        auto to_lds = [&](int nb, int mb, const f4 &y) { xbuf[256 * nb + 16 * col + ((4 * mb + g + col) & 15)] = y; };
+        bs_chunk<L, G, NSL, SPLIT, false>(xr, chunk, part_flag, full_flag, a.lb, tab + Y.kMc, depth, lp,
+                                                      a.spin_limit, a.err, 0u, sT, sG, lv32, nlv, ops, to_lds, a.zi,
+                                                      chunk + 1u == a.num_chunks ? a.tail : nullptr);
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        // read back and store in two halves (fewer registers); the fill of the
+        // next chunk goes out after the last read-back, before the last half
+        float *out = a.out + (long)chunk * CH;
+        constexpr int H = NST / 2;
+        f4 o[H];
+#pragma unroll
+        for (int i = 0; i < H; ++i) o[i] = xbuf[i * 64 + lane];
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+        for (int i = 0; i < H; ++i) *reinterpret_cast<f4 *>(out + 256 * i + goff[i & 3]) = o[i];
+#pragma unroll
+        for (int i = 0; i < H; ++i) o[i] = xbuf[(H + i) * 64 + lane];
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        if (next < a.num_chunks) fetch_staged(next);
+#pragma unroll
+        for (int i = 0; i < H; ++i) *reinterpret_cast<f4 *>(out + 256 * (H + i) + goff[i & 3]) = o[i];
+    };
     auto process = [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag, const Raw &xr) {
         float *out = a.out + (long)chunk * CH;
         // whole 128-byte lines per store (bs_chunk's LINES form), through the L2
@@ -118,8 +188,9 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
         };
         // streaming: a.zi is the balanced state before the signal, a.tail
         // receives the state entering the last chunk's last tile
-        bs_chunk<L, G, NSL, SPLIT, true>(xr, chunk, part_flag, full_flag, a.lb, tab + Y.kMc, depth, lp, a.spin_limit, a.err, 0u,
-                            sT, sG, lv32, nlv, ops, store, a.zi, chunk + 1u == a.num_chunks ? a.tail : nullptr);
+        bs_chunk<L, G, NSL, SPLIT, true>(xr, chunk, part_flag, full_flag, a.lb, tab + Y.kMc, depth, lp,
+                                                      a.spin_limit, a.err, 0u, sT, sG, lv32, nlv, ops, store, a.zi,
+                                                      chunk + 1u == a.num_chunks ? a.tail : nullptr);
     };
 
     const uint32_t GW = gridDim.x * W, gw = blockIdx.x * W + wave;  // this wave among the grid's
@@ -130,20 +201,29 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
         epoch_arrive(a.arrive, epoch);
         const uint32_t pf = 2u * epoch + 1u, ff = pf + 1u;
         uint32_t c = gw;
-        if (c < a.num_chunks) {
-            // first chunk peeled: every entry into the loop below has this
-            // chunk's output stores behind the last fill
-            fetch_chunk(c);
-            Raw cur;
-            read_chunk(cur, true);
-            if (c + GW < a.num_chunks) fetch_chunk(c + GW);
-            process(c, pf, ff, cur);
-        }
-        for (c += GW; c < a.num_chunks; c += GW) {
-            Raw cur;
-            read_chunk(cur, false);
-            if (c + GW < a.num_chunks) fetch_chunk(c + GW);
-            process(c, pf, ff, cur);
+        if constexpr (STAGED) {
+            if (c < a.num_chunks) fetch_staged(c);
+            for (bool first = true; c < a.num_chunks; c += GW, first = false) {
+                Raw cur;
+                read_staged(cur, first);
+                process_staged(c, pf, ff, cur, c + GW);
+            }
+        } else {
+            if (c < a.num_chunks) {
+                // first chunk peeled: every entry into the loop below has this
+                // chunk's output stores behind the last fill
+                fetch_chunk(c);
+                Raw cur;
+                read_chunk(cur, true);
+                if (c + GW < a.num_chunks) fetch_chunk(c + GW);
+                process(c, pf, ff, cur);
+            }
+            for (c += GW; c < a.num_chunks; c += GW) {
+                Raw cur;
+                read_chunk(cur, false);
+                if (c + GW < a.num_chunks) fetch_chunk(c + GW);
+                process(c, pf, ff, cur);
+            }
         }
         // the owner of the last chunk advances the epoch after publishing it
         if ((a.num_chunks - 1u) % GW == gw) epoch_advance(a.epoch, a.arrive, epoch, GW, a.spin_limit, a.err);
@@ -158,9 +238,15 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
         const uint32_t chunk = (uint32_t)(tk - ep * a.num_chunks);
         const uint32_t pf = (uint32_t)(2ull * ep + 1ull);
         Raw cur;
-        fetch_chunk(chunk);
-        read_chunk(cur, true);
-        process(chunk, pf, pf + 1u, cur);
+        if constexpr (STAGED) {
+            fetch_staged(chunk);
+            read_staged(cur, true);
+            process_staged(chunk, pf, pf + 1u, cur, a.num_chunks);
+        } else {
+            fetch_chunk(chunk);
+            read_chunk(cur, true);
+            process(chunk, pf, pf + 1u, cur);
+        }
     }
 }
 

@@ -300,6 +300,9 @@ constexpr int bs_tidx(int mb)
     for (int m = 0; m < mb; ++m) s += bs_tq(m) + 1;
     return s;
 }
+// the chunk staged through the wave's LDS buffer with contiguous accesses
+// both ways (bs_kernel STAGED): 64 x 64 with the FP16 split
+constexpr bool bs_staged(int L, int G, int split) { return L == 64 && G == 64 && split == 2; }
 // waves per SIMD the register budget is set for, and waves per workgroup
 // (the workgroup shares T, Gam and Phi^(2^j) in LDS; each wave runs its own
 // chunks with its own input buffer of G L floats and 16 lookback slots)
