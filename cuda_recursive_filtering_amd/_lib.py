@@ -22,7 +22,7 @@ IIRX_ERR_ACCURACY = 9
 
 SCHEDULES = {"persistent": 0, "ticket": 1}
 
-KERNELS = {"PH": 0, "STCR": 1, "DTCR": 2, "PLR": 3, "BS": 4, "BS_BF16": 5, "PHJ": 6}
+KERNELS = {"PH": 0, "STCR": 1, "DTCR": 2, "PLR": 3, "BS": 4, "BS_BF16": 5, "PHJ": 6, "PHH": 7}
 
 
 class Config(ctypes.Structure):
@@ -52,7 +52,7 @@ class PlanInfo(ctypes.Structure):
         ("launches", ctypes.c_int64),
         ("grid", ctypes.c_int64),
         ("schedule", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("halo", ctypes.c_int32),
     ]
 
 

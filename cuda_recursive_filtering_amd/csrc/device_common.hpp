@@ -83,6 +83,26 @@ __device__ __forceinline__ float row_shr_zero(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + D, 0xF, 0xF, false));
 }
 
+// sum of v over the wave, uniform: the rotations sum each row of 16 lanes,
+// row_bcast:15 / row_bcast:31 carry rows 0 and 2 into rows 1 and 3, then
+// rows 0-1 into rows 2-3, so that lane 63 holds the total
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_or_zero(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+    v += dpp_or_zero<0x121, 0xF>(v);  // row_ror:1
+    v += dpp_or_zero<0x122, 0xF>(v);  // row_ror:2
+    v += dpp_or_zero<0x124, 0xF>(v);  // row_ror:4
+    v += dpp_or_zero<0x128, 0xF>(v);  // row_ror:8
+    v += dpp_or_zero<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_or_zero<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return readlane_f(v, 63);
+}
+
 // ------------------------------------------------------------------------
 // compile-time loops: every index a literal (the "hand-unrolled" forms)
 // ------------------------------------------------------------------------

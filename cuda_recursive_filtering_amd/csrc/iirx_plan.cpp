@@ -31,6 +31,7 @@ struct Shape {
     uint32_t num_chunks;
     size_t lb_count;
     int coef_stride;
+    bool halo;  // PH_HALO: the chunk carries come from the halo (the cascade forgets within kPhHalo samples)
 };
 
 }  // namespace
@@ -83,7 +84,10 @@ bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
 // the block-state form and its operand split
 bool is_bs(int kernel) { return kernel == IIRX_KERNEL_BLOCKSTATE || kernel == IIRX_KERNEL_BLOCKSTATE_BF16; }
-bool is_ph(int kernel) { return kernel == IIRX_KERNEL_PH || kernel == IIRX_KERNEL_PH_JOINT; }
+bool is_ph(int kernel)
+{
+    return kernel == IIRX_KERNEL_PH || kernel == IIRX_KERNEL_PH_JOINT || kernel == IIRX_KERNEL_PH_HALO;
+}
 int bs_split_of(int kernel) { return kernel == IIRX_KERNEL_BLOCKSTATE_BF16 ? kBsSplitBF16 : kBsSplitF16; }
 
 // rows of the filter contract: finite, b0 == 1 and a0 == 1 (iirx_normalize_sos makes them)
@@ -107,6 +111,7 @@ int validate(const iirx_config *cfg, int64_t n)
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH:
     case IIRX_KERNEL_PH_JOINT:
+    case IIRX_KERNEL_PH_HALO:
     case IIRX_KERNEL_STCR:
     case IIRX_KERNEL_DTCR: {
         const int B = cfg->block_size, N = cfg->n_blocks;
@@ -188,12 +193,16 @@ int make_shape(const iirx_config *cfg, const float *sos, int64_t n, const double
     sh.threads = 64;
     sh.chunks_per_block = 1;
     sh.plr_block = sh.plr_x = 0;
+    sh.halo = false;
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH:
         sh.coef_stride = ph_tables(sos, ns, 64, sh.n_blocks, tables);
         break;
     case IIRX_KERNEL_PH_JOINT:
         sh.coef_stride = ph_joint_tables(sos, ns, 64, sh.n_blocks, tables);
+        break;
+    case IIRX_KERNEL_PH_HALO:
+        sh.coef_stride = ph_halo_tables(sos, ns, 64, sh.n_blocks, tables, &sh.halo);
         break;
     case IIRX_KERNEL_STCR:
     case IIRX_KERNEL_DTCR:
@@ -255,6 +264,7 @@ const void *kernel_fn(const iirx_config *cfg, const Shape &sh)
     switch (cfg->kernel) {
     case IIRX_KERNEL_PH: return kernel_ph(sh.lanes, sh.n_blocks);
     case IIRX_KERNEL_PH_JOINT: return kernel_ph(sh.lanes, sh.n_blocks, false, true);
+    case IIRX_KERNEL_PH_HALO: return kernel_ph(sh.lanes, sh.n_blocks, false, false, sh.halo);
     case IIRX_KERNEL_STCR: return kernel_stcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_DTCR: return kernel_dtcr(sh.lanes, sh.n_blocks, cfg->unrolled);
     case IIRX_KERNEL_PLR: return kernel_plr(2 * cfg->n_sections, sh.plr_block, sh.plr_x);
@@ -286,7 +296,7 @@ int plan_grid(const iirx_config *cfg, const Shape &sh, uint32_t main_chunks, con
     int occ_threads = threads;
     int64_t want = blocks_needed;
     if (grouped) {
-        occ_fn = is_ph(cfg->kernel) ? kernel_ph(sh.lanes, sh.n_blocks, true, cfg->kernel == IIRX_KERNEL_PH_JOINT)
+        occ_fn = is_ph(cfg->kernel) ? kernel_ph(sh.lanes, sh.n_blocks, true, cfg->kernel == IIRX_KERNEL_PH_JOINT, sh.halo)
                                     : kernel_stcr(sh.lanes, sh.n_blocks, cfg->unrolled, true);
         if (!occ_fn) return IIRX_ERR_GEOMETRY;
         occ_threads = 64 * wv;
@@ -536,12 +546,14 @@ static int launch(iirx_plan *p, const float *d_in, float *d_out, hipStream_t s, 
     switch (p->cfg.kernel) {
     case IIRX_KERNEL_PH:
     case IIRX_KERNEL_PH_JOINT:
+    case IIRX_KERNEL_PH_HALO:
     case IIRX_KERNEL_STCR:
     case IIRX_KERNEL_DTCR: {
         // the main launch, then the partial last chunk (which records zf)
         auto run = [&](const LaunchArgs &la, bool last) {
             if (is_ph(p->cfg.kernel))
-                return launch_ph(p->lanes, p->n_blocks, la, s, last, p->cfg.kernel == IIRX_KERNEL_PH_JOINT);
+                return launch_ph(p->lanes, p->n_blocks, la, s, last, p->cfg.kernel == IIRX_KERNEL_PH_JOINT,
+                                 p->shape.halo);
             if (p->cfg.kernel == IIRX_KERNEL_STCR) return launch_stcr(p->lanes, p->n_blocks, la, s, last);
             return launch_dtcr(p->lanes, p->n_blocks, la, s, last);
         };
@@ -638,6 +650,7 @@ int iirx_plan_info_get(const iirx_plan *p, iirx_plan_info *info)
     info->launches = p->launches;
     info->grid = p->grid;
     info->schedule = p->cfg.schedule;
+    info->halo = p->shape.halo ? 1 : 0;
     const void *fn = p->fn;
     if (fn) {
         hipFuncAttributes at;

@@ -119,14 +119,39 @@ constexpr PhPairLayout ph_pair_layout(int L, int N)
     return p;
 }
 
+// PH with halo carries (IIRX_KERNEL_PH_HALO).  When the cascade forgets its
+// state within kPhHalo samples -- ||A^kPhHalo||_inf <= kHaloTail for A, the
+// zero-input one-sample transition of the whole cascade's state
+// (x[-2], x[-1], y_1[-2], y_1[-1], ..., y_ns[-2], y_ns[-1]) -- the state
+// entering a chunk is, to that bound (2^-34 of the largest state entry, a
+// thousandth of a float32 ulp), the state the cascade reaches from
+// zero over the kPhHalo input samples before the chunk:
+//   y_s[-2] = sum_i K2_s[i] x[i - kPhHalo],  y_s[-1] = sum_i K1_s[i] x[i - kPhHalo]
+// with K1_s[i] = h_s[kPhHalo - 1 - i] and K2_s[i] = h_s[kPhHalo - 2 - i]
+// (h_s the impulse response of sections 1..s).  A chunk then reads 4 more
+// samples per lane and needs no record of another chunk: no lookback.
+// After the per-section tables of all sections:
+//   [0]                           1 if the carries come from the halo, 0 if
+//                                 the cascade's memory is longer (the
+//                                 kernel then runs PH's lookback); 3 floats
+//                                 of padding
+//   [4 + 2 kPhHalo s]             K2_s[i], i < kPhHalo
+//   [4 + 2 kPhHalo s + kPhHalo]   K1_s[i]
+constexpr int kPhHalo = 256;
+constexpr double kHaloTail = 5.820766091346741e-11;  // 2^-34
+constexpr int ph_halo_floats(int ns) { return 4 + 2 * kPhHalo * ns; }
+
 // Launchers (one per algorithm); `lanes` and `n_blocks` pick the template
 // instance.  Return hipErrorInvalidValue when no instance exists.
 // last = true: the one-workgroup launch of the partial last chunk (chunk
 // a.num_chunks, a.last_valid samples), after the launch of the others
 // joint: sections in pairs (IIRX_KERNEL_PH_JOINT); the pair tables follow the
 // per-section ones at a.coef + a.ns * a.coef_stride
+// halo: carries from the kPhHalo samples before each chunk (IIRX_KERNEL_PH_HALO
+// when its tables mark the halo usable); the halo block follows the
+// per-section tables at a.coef + a.ns * a.coef_stride
 hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last = false,
-                     bool joint = false);
+                     bool joint = false, bool halo = false);
 // PH and STCR under the ticket schedule: chunks (waves) per group, one
 // atomic ticket per group; used when it divides the launch's chunk count
 // (else one chunk per workgroup).  16 waves at N <= 32 (135 KB of LDS at
@@ -141,7 +166,7 @@ hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s
 constexpr int ph_ticket_waves(int N) { return N <= 32 ? 16 : (N == 64 ? 4 : 2); }
 // grouped: the ticket-schedule instance with ph_ticket_waves(n_blocks) chunks
 // per workgroup
-const void *kernel_ph(int lanes, int n_blocks, bool grouped = false, bool joint = false);
+const void *kernel_ph(int lanes, int n_blocks, bool grouped = false, bool joint = false, bool halo = false);
 
 }  // namespace iirx
 

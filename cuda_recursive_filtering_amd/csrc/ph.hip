@@ -29,6 +29,11 @@ namespace iirx {
 // and with a.tail set it records the end state there.
 // WV > 1: the ticket schedule with WV chunks per workgroup, one wave and one
 // LDS tile each (one atomic ticket per group).
+// HALO: every section's chunk carry from the kPhHalo input samples before the
+// chunk (4 per lane, loaded with the chunk): y_s[-2], y_s[-1] = wave sums of
+// the lane's samples times its K2_s, K1_s weights.  No chunk reads another's
+// record, so there is no lookback; the full records are still written (the
+// partial last chunk and the end state read the last one's).
 // JOINT: sections in pairs (s, s + 1) with one lookback of the pair's joint
 // state per chunk (an odd last section alone, and so a pair whose tables
 // exceed kPairLimit).  Both sections' zero-state
@@ -41,7 +46,7 @@ namespace iirx {
 // sections (8 FMA zero-state + 4 FMA correction against 2 x (4 + 2)); the
 // scan takes 12 FMA per level against 2 x 4, and a chunk waits for one
 // lookback instead of two.
-template <int N, bool LAST, int WV = 1, bool JOINT = false>
+template <int N, bool LAST, int WV = 1, bool JOINT = false, bool HALO = false>
 __global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
 {
     constexpr int L = 64;
@@ -55,6 +60,10 @@ __global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
     const long base = (long)chunk * (L * N);
 
     chunk_sync<WV>();  // LDS of the previous chunk fully consumed
+    // HALO: samples base - kPhHalo + 4 lane .. + 3, in flight with the chunk's loads
+    f4 hx = {0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (HALO && !LAST)
+        if (chunk > 0) hx = *reinterpret_cast<const f4 *>(a.in + base - kPhHalo + 4 * lane);
     if constexpr (LAST) stage_in_part<L, N>(a.in + base, lds, a.last_valid);
     else stage_in<L, N>(a.in + base, lds, lane);
     chunk_sync<WV>();
@@ -186,6 +195,14 @@ __global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
         const float *cv = a.coef + (size_t)sec * a.coef_stride;  // per-lane vector loads
         const cfloat *c = as_const(cv);                           // uniform scalar loads
         const float b1 = c[Y.kB + 0], b2 = c[Y.kB + 1], a1 = c[Y.kB + 2], a2 = c[Y.kB + 3];
+        // HALO: this lane's halo weights of the section, K2_s and K1_s
+        f4 k2 = {0.0f, 0.0f, 0.0f, 0.0f}, k1 = k2;
+        if constexpr (HALO && !LAST) {
+            const f4 *kh = reinterpret_cast<const f4 *>(a.coef + (size_t)a.ns * a.coef_stride + 4 +
+                                                        (size_t)sec * 2 * kPhHalo);
+            k2 = kh[lane];
+            k1 = kh[kPhHalo / 4 + lane];
+        }
 
         // hot loop 1: zero-state biquad (ph_kernels.cuh:265-287)
         {
@@ -212,16 +229,26 @@ __global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
         const float m0 = c[Y.kM + 0], m1 = c[Y.kM + 1], m2 = c[Y.kM + 2], m3 = c[Y.kM + 3];
         const float p2 = readlane_f(s2, L - 1), p1 = readlane_f(s1, L - 1);
         uint64_t *rec = rec2(a.lb, chunk, sec, a.ns);
-        if (!LAST && lane == 0) publish2(rec, false, part_flag, p2, p1);
+        if (!LAST && !HALO && lane == 0) publish2(rec, false, part_flag, p2, p1);
         const int depth = (int)c[Y.kDepth];
         // the chain starts from the streamed-in state of this section (zero by default)
         const float z2 = a.zi ? a.zi[2 + 2 * sec] : 0.0f, z1 = a.zi ? a.zi[3 + 2 * sec] : 0.0f;
         // C^(lane+1), the per-lane carry injection: its load is in flight
         // during the lookback instead of after it
         const float4 cf = *reinterpret_cast<const float4 *>(cv + Y.kCfin + 4 * lane);
-        const Carry2 X = LAST ? prior_state2(a.lb, chunk, sec, a.ns, z2, z1)
-                              : lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth,
-                                          a.spin_limit, a.err, 0u, z2, z1);
+        Carry2 X;
+        if constexpr (LAST) {
+            X = prior_state2(a.lb, chunk, sec, a.ns, z2, z1);
+        } else if constexpr (HALO) {
+            X = Carry2{z2, z1};
+            if (chunk > 0) {
+                X.y2 = wave_sum(fmaf(k2[3], hx[3], fmaf(k2[2], hx[2], fmaf(k2[1], hx[1], k2[0] * hx[0]))));
+                X.y1 = wave_sum(fmaf(k1[3], hx[3], fmaf(k1[2], hx[2], fmaf(k1[1], hx[1], k1[0] * hx[0]))));
+            }
+        } else {
+            X = lookback2(a.lb, chunk, sec, a.ns, part_flag, full_flag, m0, m1, m2, m3, depth, a.spin_limit, a.err,
+                          0u, z2, z1);
+        }
         if (!LAST && lane == 0) {
             const float f2 = fmaf(m1, X.y1, fmaf(m0, X.y2, p2));
             const float f1 = fmaf(m3, X.y1, fmaf(m2, X.y2, p1));
@@ -285,46 +312,62 @@ hipError_t launch_ph_end_state(const LaunchArgs &a, int chunk_samples, float *zf
 // the partial last chunk always runs the per-section instance: it reads the
 // full records of its predecessor per section, which the joint instance
 // writes in the same layout
-template <int N, bool J>
+template <int N, bool J, bool H>
 static hipError_t launch_n(const LaunchArgs &a, bool last, hipStream_t s)
 {
     constexpr int WV = ph_ticket_waves(N);
     if (last) hipLaunchKernelGGL((ph_kernel<N, true>), dim3(1), dim3(64), 0, s, a);
     else if (!a.persistent && a.num_chunks % WV == 0)
-        hipLaunchKernelGGL((ph_kernel<N, false, WV, J>), dim3(a.grid), dim3(64 * WV), 0, s, a);
-    else hipLaunchKernelGGL((ph_kernel<N, false, 1, J>), dim3(a.grid), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((ph_kernel<N, false, WV, J, H>), dim3(a.grid), dim3(64 * WV), 0, s, a);
+    else hipLaunchKernelGGL((ph_kernel<N, false, 1, J, H>), dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last, bool joint)
+template <int N>
+static hipError_t launch_jh(const LaunchArgs &a, bool last, hipStream_t s, bool joint, bool halo)
 {
-    if (lanes != 64) return hipErrorInvalidValue;
+    if (joint) return launch_n<N, true, false>(a, last, s);
+    if (halo) return launch_n<N, false, true>(a, last, s);
+    return launch_n<N, false, false>(a, last, s);
+}
+
+hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s, bool last, bool joint, bool halo)
+{
+    if (lanes != 64 || (joint && halo)) return hipErrorInvalidValue;
     switch (n_blocks) {
-    case 8: return joint ? launch_n<8, true>(a, last, s) : launch_n<8, false>(a, last, s);
-    case 16: return joint ? launch_n<16, true>(a, last, s) : launch_n<16, false>(a, last, s);
-    case 32: return joint ? launch_n<32, true>(a, last, s) : launch_n<32, false>(a, last, s);
-    case 64: return joint ? launch_n<64, true>(a, last, s) : launch_n<64, false>(a, last, s);
-    case 128: return joint ? launch_n<128, true>(a, last, s) : launch_n<128, false>(a, last, s);
+    case 8: return launch_jh<8>(a, last, s, joint, halo);
+    case 16: return launch_jh<16>(a, last, s, joint, halo);
+    case 32: return launch_jh<32>(a, last, s, joint, halo);
+    case 64: return launch_jh<64>(a, last, s, joint, halo);
+    case 128: return launch_jh<128>(a, last, s, joint, halo);
     default: return hipErrorInvalidValue;
     }
 }
 
-template <int N, bool J>
+template <int N, bool J, bool H>
 static const void *fn_n(bool grouped)
 {
-    if (grouped) return (const void *)ph_kernel<N, false, ph_ticket_waves(N), J>;
-    return (const void *)ph_kernel<N, false, 1, J>;
+    if (grouped) return (const void *)ph_kernel<N, false, ph_ticket_waves(N), J, H>;
+    return (const void *)ph_kernel<N, false, 1, J, H>;
 }
 
-const void *kernel_ph(int lanes, int n_blocks, bool grouped, bool joint)
+template <int N>
+static const void *fn_jh(bool grouped, bool joint, bool halo)
 {
-    if (lanes != 64) return nullptr;
+    if (joint) return fn_n<N, true, false>(grouped);
+    if (halo) return fn_n<N, false, true>(grouped);
+    return fn_n<N, false, false>(grouped);
+}
+
+const void *kernel_ph(int lanes, int n_blocks, bool grouped, bool joint, bool halo)
+{
+    if (lanes != 64 || (joint && halo)) return nullptr;
     switch (n_blocks) {
-    case 8: return joint ? fn_n<8, true>(grouped) : fn_n<8, false>(grouped);
-    case 16: return joint ? fn_n<16, true>(grouped) : fn_n<16, false>(grouped);
-    case 32: return joint ? fn_n<32, true>(grouped) : fn_n<32, false>(grouped);
-    case 64: return joint ? fn_n<64, true>(grouped) : fn_n<64, false>(grouped);
-    case 128: return joint ? fn_n<128, true>(grouped) : fn_n<128, false>(grouped);
+    case 8: return fn_jh<8>(grouped, joint, halo);
+    case 16: return fn_jh<16>(grouped, joint, halo);
+    case 32: return fn_jh<32>(grouped, joint, halo);
+    case 64: return fn_jh<64>(grouped, joint, halo);
+    case 128: return fn_jh<128>(grouped, joint, halo);
     default: return nullptr;
     }
 }

@@ -179,6 +179,69 @@ int ph_joint_tables(const float *sos, int ns, int lanes, int N, std::vector<floa
     return stride;
 }
 
+// one zero-input step of the cascade; st = (x[-2], x[-1], y_1[-2], y_1[-1], ...)
+static void cascade_step(const float *sos, int ns, double *st, double x)
+{
+    double u2 = st[0], u1 = st[1], u = x;  // this section's input history and input
+    st[0] = st[1];
+    st[1] = x;
+    for (int s = 0; s < ns; ++s) {
+        const float *r = sos + 6 * s;
+        double &y2 = st[2 + 2 * s], &y1 = st[3 + 2 * s];
+        const double y = u + (double)r[1] * u1 + (double)r[2] * u2 - (double)r[4] * y1 - (double)r[5] * y2;
+        u2 = y2;
+        u1 = y1;
+        u = y;
+        y2 = y1;
+        y1 = y;
+    }
+}
+
+double cascade_transition_norm(const float *sos, int ns, int n)
+{
+    const int K = 2 + 2 * ns;
+    std::vector<double> A((size_t)K * K), st((size_t)K);
+    for (int m = 0; m < K; ++m) {  // column m: the state after n steps from e_m
+        std::fill(st.begin(), st.end(), 0.0);
+        st[(size_t)m] = 1.0;
+        for (int k = 0; k < n; ++k) cascade_step(sos, ns, st.data(), 0.0);
+        for (int r = 0; r < K; ++r) A[(size_t)r * K + m] = st[(size_t)r];
+    }
+    double nrm = 0.0;
+    for (int r = 0; r < K; ++r) {
+        double row = 0.0;
+        for (int m = 0; m < K; ++m) row += std::fabs(A[(size_t)r * K + m]);
+        nrm = std::max(nrm, row);
+    }
+    return nrm;
+}
+
+int ph_halo_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &out, bool *forgets)
+{
+    const int stride = ph_tables(sos, ns, lanes, N, out);
+    const size_t base = out.size();
+    out.resize(base + (size_t)ph_halo_floats(ns), 0.0f);
+    float *t = out.data() + base;
+    const double nrm = cascade_transition_norm(sos, ns, kPhHalo);
+    const bool ok = std::isfinite(nrm) && nrm <= kHaloTail;
+    t[0] = ok ? 1.0f : 0.0f;
+    if (forgets) *forgets = ok;
+    // h_s[k]: output of section s at sample k from a unit impulse at sample 0
+    std::vector<double> st((size_t)(2 + 2 * ns), 0.0), h((size_t)ns * kPhHalo);
+    for (int k = 0; k < kPhHalo; ++k) {
+        cascade_step(sos, ns, st.data(), k == 0 ? 1.0 : 0.0);
+        for (int s = 0; s < ns; ++s) h[(size_t)s * kPhHalo + k] = st[(size_t)(3 + 2 * s)];
+    }
+    for (int s = 0; s < ns; ++s) {
+        float *k2 = t + 4 + 2 * kPhHalo * s, *k1 = k2 + kPhHalo;
+        for (int i = 0; i < kPhHalo; ++i) {
+            k1[i] = (float)h[(size_t)s * kPhHalo + kPhHalo - 1 - i];
+            k2[i] = i <= kPhHalo - 2 ? (float)h[(size_t)s * kPhHalo + kPhHalo - 2 - i] : 0.0f;
+        }
+    }
+    return stride;
+}
+
 }  // namespace iirx
 
 namespace iirx {

@@ -38,6 +38,18 @@ int ph_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &ou
 // per-section stride in floats.
 int ph_joint_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &out);
 
+// PH tables for all sections followed by the halo block (layout
+// kernels.hpp, kPhHalo): the FP64 impulse responses of the cascade prefixes
+// as the halo weights K2_s, K1_s, and the flag that the cascade forgets its
+// state within kPhHalo samples (||A^kPhHalo||_inf <= 2^-34, A the zero-input
+// one-sample transition of the cascade state; *forgets receives it).
+// Returns the per-section stride in floats.
+int ph_halo_tables(const float *sos, int ns, int lanes, int N, std::vector<float> &out, bool *forgets = nullptr);
+
+// ||A^n||_inf of the cascade's zero-input state transition over n samples,
+// state (x[-2], x[-1], y_1[-2], y_1[-1], ..., y_ns[-2], y_ns[-1]), in FP64.
+double cascade_transition_norm(const float *sos, int ns, int n);
+
 // The pair's joint N-step transition J (4 x 4 row-major) and the responses
 // H[m * N + n] of the second section's output to the unit joint states, in
 // FP64, for sections (s, t) of `sos`.

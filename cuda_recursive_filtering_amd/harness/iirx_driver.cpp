@@ -6,7 +6,7 @@
 // (run_accuracy.sh:155-166) and builds one binary per configuration; here they
 // are command-line options of a runtime plan:
 //
-//   iirx_driver accuracy|performance --kernel PH|STCR|DTCR|PLR|BS
+//   iirx_driver accuracy|performance --kernel PH|PHJ|PHH|STCR|DTCR|PLR|BS|BS_BF16
 //       [--block-size B] [--n-blocks N] [--sections NS] [--log2n L]
 //       [--unrolled] [--plr-x-max X] [--schedule persistent|ticket]
 //       [--taps FILE] [--write-taps FILE]
@@ -56,7 +56,7 @@ struct Options {
 int usage(const char *prog)
 {
     std::printf("usage: %s taps --sections NS --write-taps FILE\n"
-                "       %s accuracy|performance --kernel PH|STCR|DTCR|PLR|BS [--block-size B] [--n-blocks N]\n"
+                "       %s accuracy|performance --kernel PH|PHJ|PHH|STCR|DTCR|PLR|BS|BS_BF16 [--block-size B] [--n-blocks N]\n"
                 "       [--sections NS] [--log2n L] [--unrolled] [--plr-x-max X] [--schedule persistent|ticket]\n"
                 "       [--taps FILE] [--write-taps FILE] [--reference FILE] [--warmup W] [--batches K] [--batch M]\n",
                 prog, prog);
@@ -86,6 +86,7 @@ bool parse(int argc, char **argv, Options &o)
             else if (k == "BS") o.kernel = IIRX_KERNEL_BLOCKSTATE;
             else if (k == "BS_BF16") o.kernel = IIRX_KERNEL_BLOCKSTATE_BF16;
             else if (k == "PHJ") o.kernel = IIRX_KERNEL_PH_JOINT;
+            else if (k == "PHH") o.kernel = IIRX_KERNEL_PH_HALO;
             else return false;
         } else if (a == "--block-size") o.block_size = std::atoi(v);
         else if (a == "--n-blocks") o.n_blocks = std::atoi(v);

@@ -93,7 +93,7 @@ typedef enum {
                                         into BF16 (hi, mid, lo; six MFMA
                                         products per K slab) instead of FP16
                                         hi/lo; L = 64 or 128, G = 16 or 32 */
-    IIRX_KERNEL_PH_JOINT = 6   /* no counterpart in the reference: PH with
+    IIRX_KERNEL_PH_JOINT = 6,  /* no counterpart in the reference: PH with
                                   sections taken in pairs, one decoupled
                                   lookback of the pair's joint 4-float state
                                   per chunk instead of one per section (an
@@ -103,6 +103,18 @@ typedef enum {
                                   at a time); same geometry, records,
                                   streaming state and any-length rules as
                                   PH */
+    IIRX_KERNEL_PH_HALO = 7    /* no counterpart in the reference: PH whose
+                                  chunks take every section's incoming state
+                                  from the 256 input samples before the chunk
+                                  (weights precomputed in FP64) instead of a
+                                  decoupled lookback, when the cascade forgets
+                                  its state within 256 samples
+                                  (||A^256||_inf <= 2^-34 for its one-sample
+                                  state transition A); otherwise (poles close
+                                  to the unit circle) exactly PH.
+                                  iirx_plan_info.halo says which.  Same
+                                  geometry, records, streaming state and
+                                  any-length rules as PH */
 } iirx_kernel;
 
 /*
@@ -163,7 +175,10 @@ typedef struct {
     int64_t launches;           /* launches executed so far on this plan */
     int64_t grid;               /* workgroups per launch */
     int32_t schedule;
-    int32_t pad;
+    int32_t halo;               /* IIRX_KERNEL_PH_HALO: 1 if the chunk
+                                   carries come from the halo, 0 if the
+                                   plan runs PH's lookback (0 for every
+                                   other kernel) */
 } iirx_plan_info;
 
 /*

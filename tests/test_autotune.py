@@ -26,19 +26,20 @@ def test_candidates_cover_every_algorithm():
     sos, _ = iirx.butterworth_sos(8, 0.1)
     k = _by_kernel(iirx.tune_candidates(sos, 1 << 22))
     assert len(k["PH"]) == 5 and len(k["STCR"]) == 10 and len(k["DTCR"]) == 10 and len(k["BS"]) == 8
-    assert len(k["PHJ"]) == 5 and len(k["BS_BF16"]) == 4
+    assert len(k["PHJ"]) == 5 and len(k["PHH"]) == 4 and len(k["BS_BF16"]) == 4
     assert "PLR" not in k  # the direct form is offered at orders <= 4 only
     assert all(e["status"] == 0 for es in k.values() for e in es)
     k2 = _by_kernel(iirx.tune_candidates(iirx.butterworth_sos(4, 0.1)[0], 1 << 22))
     assert len(k2["PLR"]) == 1 and k2["PLR"][0]["status"] == 0
     # one section: nothing to pair
-    assert "PHJ" not in _by_kernel(iirx.tune_candidates(iirx.butterworth_sos(2, 0.1)[0], 1 << 22))
+    k1 = _by_kernel(iirx.tune_candidates(iirx.butterworth_sos(2, 0.1)[0], 1 << 22))
+    assert "PHJ" not in k1 and "PHH" not in k1
 
 
 def test_candidates_flags():
     sos, _ = iirx.butterworth_sos(8, 0.1)
     k = _by_kernel(iirx.tune_candidates(sos, 1 << 22, flags=iirx._lib.TUNE_NO_BLOCKSTATE | iirx._lib.TUNE_NO_CR))
-    assert set(k) == {"PH", "PHJ"}
+    assert set(k) == {"PH", "PHJ", "PHH"}
     k = _by_kernel(iirx.tune_candidates(sos, 1 << 22, flags=iirx._lib.TUNE_PLR_ALL_ORDERS))
     assert "PLR" in k
     k = _by_kernel(iirx.tune_candidates(sos, 1 << 22, flags=iirx._lib.TUNE_TICKET))

@@ -24,9 +24,11 @@ ORDERS = [2, 4, 8, 16]
 # PHJ (PH with sections in pairs, one lookback per pair) computes the PH
 # algorithm with the carries of two sections combined before the correction,
 # so it is held to the PH restatement at the same tolerance
-TOL_ULPS = {"PH": 64.0, "PHJ": 64.0, "STCR": 64.0, "DTCR": 64.0}
-RESTATE = {"PH": O.ph, "PHJ": O.ph, "STCR": O.stcr, "DTCR": O.dtcr}
-KERNELS = ["PH", "PHJ", "STCR", "DTCR"]
+# PHH (PH with the chunk carries from the 256 samples before each chunk where
+# the cascade forgets within them, else PH) likewise
+TOL_ULPS = {"PH": 64.0, "PHJ": 64.0, "PHH": 64.0, "STCR": 64.0, "DTCR": 64.0}
+RESTATE = {"PH": O.ph, "PHJ": O.ph, "PHH": O.ph, "STCR": O.stcr, "DTCR": O.dtcr}
+KERNELS = ["PH", "PHJ", "PHH", "STCR", "DTCR"]
 
 
 def run_gpu(device, kernel, sos, x, B, N, launches=1, **kw):
@@ -117,7 +119,7 @@ def test_launch_versioning_stale_flags(device, kernel, schedule):
 
 
 @pytest.mark.parametrize("kernel,B,N,waves", [("PH", 64, 32, 16), ("STCR", 64, 64, 4), ("PH", 64, 16, 16),
-                                              ("PHJ", 64, 32, 16)])
+                                              ("PHJ", 64, 32, 16), ("PHH", 64, 32, 16)])
 def test_ticket_groups_loop(device, kernel, B, N, waves):
     """Ticket schedule with more groups of chunks than resident workgroups:
     every workgroup claims several groups, one atomic ticket each, and a
@@ -189,7 +191,7 @@ def test_watchdog_terminates_and_reports(device, kernel):
 
 
 # offset of the lookback depth in the first section's table (kernels.hpp)
-DEPTH_AT = {"PH": 4 + 4 * 6 + 4, "PHJ": 4 + 4 * 6 + 4, "STCR": 8 * 10 + 4 * 6 + 4, "DTCR": 8 * 10 + 4 * 6 + 4}
+DEPTH_AT = {"PH": 4 + 4 * 6 + 4, "PHJ": 4 + 4 * 6 + 4, "PHH": 4 + 4 * 6 + 4, "STCR": 8 * 10 + 4 * 6 + 4, "DTCR": 8 * 10 + 4 * 6 + 4}
 
 
 def resonator(r, theta):
@@ -288,8 +290,8 @@ def test_joint_falls_back_on_large_tables(device, order, wn, N):
 
 
 @pytest.mark.slow
-@pytest.mark.parametrize("kernel,B,N", [("PH", 64, 32), ("PH", 64, 64), ("PHJ", 64, 32), ("STCR", 64, 64),
-                                         ("DTCR", 64, 64)])
+@pytest.mark.parametrize("kernel,B,N", [("PH", 64, 32), ("PH", 64, 64), ("PHJ", 64, 32), ("PHH", 64, 32),
+                                         ("STCR", 64, 64), ("DTCR", 64, 64)])
 @pytest.mark.parametrize("order", ORDERS)
 def test_full_size_gate(device, kernel, B, N, order):
     """2^25 samples, the measured configuration (PH 64 x 32 is the bench's

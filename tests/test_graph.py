@@ -26,6 +26,7 @@ def _close(y, ref):
 @pytest.mark.parametrize("kernel,B,N,order", [
     ("PH", 64, 32, 4),
     ("PHJ", 64, 32, 8),
+    ("PHH", 64, 32, 4),
     ("STCR", 32, 64, 4),
     ("DTCR", 64, 64, 2),
     ("PLR", None, 64, 4),

@@ -1,0 +1,188 @@
+"""PH with halo carries (PHH, IIRX_KERNEL_PH_HALO; no counterpart in the
+reference).  When the cascade forgets its state within 256 samples
+(||A^256||_inf <= 2^-34 for its one-sample zero-input state transition A),
+every chunk takes each section's incoming state (y_s[-2], y_s[-1]) from the
+256 input samples before it, weighted by the FP64 impulse responses of the
+cascade prefixes, instead of a decoupled lookback; otherwise the plan runs
+PH unchanged.
+
+Host tests pin the halo tables against what the mathematics fixes: the
+eligibility flag against the state-space matrix power built here in numpy,
+the weights against scipy's FP64 impulse responses, and the truncated carry
+against the exact FP64 state of the whole prefix.  GPU tests hold the kernel
+to the FP64 gate and to the oracle's PH restatement."""
+import numpy as np
+import pytest
+from scipy import signal
+
+import cuda_recursive_filtering_amd as iirx
+import oracle_lib as O
+from workloads import normal_workload, ulp_scaled
+
+HALO = 256
+
+
+def _one_step_matrix(sos):
+    """Zero-input one-sample transition of the cascade state (x[-2], x[-1],
+    y_1[-2], y_1[-1], ..., y_ns[-2], y_ns[-1]) in FP64, row by row from the
+    difference equations y_s = u_s + b1 u_s[-1] + b2 u_s[-2] - a1 y_s[-1] -
+    a2 y_s[-2] with u_1 = x (zero) and u_s = y_(s-1)."""
+    ns = sos.shape[0]
+    K = 2 + 2 * ns
+    A = np.zeros((K, K))
+    A[0, 1] = 1.0  # x[-2] <- x[-1]; x[-1] <- 0
+    u = np.zeros(K)  # the new input of the section, as a row over the old state
+    for s in range(ns):
+        b1, b2, a1, a2 = (float(sos[s, i]) for i in (1, 2, 4, 5))
+        h2, h1 = (0, 1) if s == 0 else (2 * s, 2 * s + 1)  # the input history: x or y_(s-1)
+        row = u.copy()
+        row[h1] += b1
+        row[h2] += b2
+        row[3 + 2 * s] -= a1
+        row[2 + 2 * s] -= a2
+        A[2 + 2 * s, 3 + 2 * s] = 1.0
+        A[3 + 2 * s] = row
+        u = row
+    return A
+
+
+def _halo_block(sos, N=32):
+    n = 64 * N * 4
+    tab, stride = iirx.plan_tables("PHH", sos, n, block_size=64, n_blocks=N)
+    ns = sos.shape[0]
+    base = ns * stride
+    assert tab.size == base + 4 + 2 * HALO * ns
+    return tab[base:]
+
+
+@pytest.mark.parametrize("order,wn", [(2, 0.1), (4, 0.1), (6, 0.1), (8, 0.1), (4, 0.05), (2, 0.02), (8, 0.4),
+                                      (16, 0.4), (4, 0.4)])
+def test_halo_flag_is_the_transition_bound(order, wn):
+    """The flag says ||A^256||_inf <= 2^-34, with A from the difference
+    equations here (not from the library's step).  At Wn = 0.1 orders 2 and
+    4 qualify (their states fade within about 120 and 220 samples), 6 and 8
+    do not."""
+    sos, _ = iirx.butterworth_sos(order, wn)
+    A = _one_step_matrix(sos.astype(np.float64))
+    nrm = np.abs(np.linalg.matrix_power(A, HALO)).sum(axis=1).max()
+    flag = _halo_block(sos)[0]
+    print(f"order={order} wn={wn} ||A^256||={nrm:.3e} flag={flag}")
+    assert flag == (1.0 if nrm <= 2.0 ** -34 else 0.0)
+    if wn == 0.1:
+        assert flag == (1.0 if order <= 4 else 0.0)
+
+
+@pytest.mark.parametrize("order,wn", [(2, 0.1), (4, 0.1), (8, 0.1), (4, 0.4)])
+def test_halo_weights_are_prefix_impulse_responses(order, wn):
+    """K1_s[i] = h_s[255 - i] and K2_s[i] = h_s[254 - i] (0 at i = 255), h_s
+    the FP64 impulse response of sections 1..s (scipy), rounded once to
+    float32.  Written for every filter, used only when the flag is set."""
+    sos, _ = iirx.butterworth_sos(order, wn)
+    blk = _halo_block(sos)
+    imp = np.zeros(HALO)
+    imp[0] = 1.0
+    for s in range(sos.shape[0]):
+        h = signal.sosfilt(sos[:s + 1].astype(np.float64), imp)
+        k2 = blk[4 + 2 * HALO * s: 4 + 2 * HALO * s + HALO]
+        k1 = blk[4 + 2 * HALO * s + HALO: 4 + 2 * HALO * (s + 1)]
+        want1 = h[::-1]
+        want2 = np.concatenate([h[:-1][::-1], [0.0]])
+        tol = 2.0 ** -23 * np.abs(h).max()
+        assert np.allclose(k1, want1, rtol=2.0 ** -23, atol=tol), s
+        assert np.allclose(k2, want2, rtol=2.0 ** -23, atol=tol), s
+        assert k2[-1] == 0.0
+
+
+@pytest.mark.parametrize("order,wn", [(2, 0.1), (4, 0.1), (4, 0.4), (8, 0.4)])
+def test_halo_carry_is_the_prefix_state(order, wn):
+    """For a filter whose flag is set, the weights applied in FP64 to the 256
+    samples before position p give every section's (y_s[p-2], y_s[p-1]) of
+    the whole prefix, to within the float32 rounding of the weights (2^-23 of
+    sum |K_i x_i|) plus the dropped history (at most ||A^256|| <= 2^-34 of
+    the largest state, 2^-33 allowed).  Two plausible mistakes miss: a
+    window of the last 128 samples (the order-4 filters need more than 200)
+    and the weights of y_s[-2] taken for y_s[-1] (one sample off)."""
+    sos, g = iirx.butterworth_sos(order, wn)
+    blk = _halo_block(sos).astype(np.float64)
+    assert blk[0] == 1.0
+    rng = np.random.default_rng(3)
+    x = (g * rng.standard_normal(6000)).astype(np.float32).astype(np.float64)
+    ns = sos.shape[0]
+    worst, worst_short, worst_shift = 0.0, 0.0, 0.0  # error / allowed
+    for p in (1000, 3001, 6000):
+        y = x[:p]
+        for s in range(ns):
+            y = signal.sosfilt(sos[s:s + 1].astype(np.float64), y)
+            k2 = blk[4 + 2 * HALO * s: 4 + 2 * HALO * s + HALO]
+            k1 = blk[4 + 2 * HALO * s + HALO: 4 + 2 * HALO * (s + 1)]
+            win = x[p - HALO:p]
+            short = np.where(np.arange(HALO) < HALO // 2, 0.0, win)
+            big = np.abs(y).max()
+            for k, want in ((k2, y[-2]), (k1, y[-1])):
+                allowed = 2.0 ** -23 * (np.abs(k) @ np.abs(win)) + 2.0 ** -33 * big
+                worst = max(worst, abs(k @ win - want) / allowed)
+                worst_short = max(worst_short, abs(k @ short - want) / allowed)
+            worst_shift = max(worst_shift, abs(k2 @ win - y[-1]) / (2.0 ** -23 * (np.abs(k1) @ np.abs(win))))
+    print(f"order={order} wn={wn} err/allowed={worst:.2e} 128-sample window: {worst_short:.2e} "
+          f"shifted: {worst_shift:.2e}")
+    assert worst <= 1.0
+    assert worst_shift > 1e3
+    if order == 4 and wn == 0.1:
+        assert worst_short > 1.0
+
+
+# ---------------------------------------------------------------- GPU
+
+
+def _run(device, kernel, sos, x, B, N, **kw):
+    import torch
+    xin = torch.from_numpy(x).to(device)
+    out = torch.empty_like(xin)
+    with iirx.Plan(kernel, sos, x.size, block_size=B, n_blocks=N, **kw) as p:
+        p.execute(xin, out)
+        torch.cuda.synchronize()
+        p.check()
+        info = p.info()
+    return out.cpu().numpy(), info
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,wn,halo", [(2, 0.1, 1), (4, 0.1, 1), (4, 0.4, 1), (8, 0.4, 1), (8, 0.1, 0),
+                                           (6, 0.1, 0)])
+@pytest.mark.parametrize("B,N", [(64, 16), (64, 32), (32, 64), (64, 128)])
+def test_halo_parity(device, order, wn, halo, B, N):
+    """The halo where the flag allows it (plan info says which), PH's
+    lookback where not; both pass the FP64 gate and agree with the oracle's
+    PH restatement within 64 float32 ulps of max|ref|."""
+    print(f"case PHH order={order} wn={wn} {B}x{N}", flush=True)
+    sos, g, x = normal_workload(order, 1 << 20, seed=21, wn=wn)
+    y, info = _run(device, "PHH", sos, x, B, N)
+    assert info["halo"] == halo
+    ref = O.golden(sos, x)
+    ok, err, mref, rel = O.gate(y, ref)
+    assert ok, rel
+    u = ulp_scaled(y, O.ph(sos, x, 64, B * N // 64), ref)
+    print(f"  halo={info['halo']} rel={rel:.2e} vs-restatement={u:.1f} ulps vgprs={info['vgprs']}")
+    assert u <= 64.0, u
+    assert info["scratch_bytes"] == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("schedule", ["persistent", "ticket"])
+def test_halo_launches_alternate(device, schedule):
+    """Nine launches on one plan alternating two inputs, both schedules: no
+    chunk reads another chunk's record, so nothing carries over."""
+    import torch
+    sos, g, xa = normal_workload(4, 1 << 22, seed=22, wn=0.1)
+    _, _, xb = normal_workload(4, 1 << 22, seed=23, wn=0.1)
+    ra, rb = O.golden(sos, xa), O.golden(sos, xb)
+    ta, tb = torch.from_numpy(xa).to(device), torch.from_numpy(xb).to(device)
+    out = torch.empty_like(ta)
+    with iirx.Plan("PHH", sos, xa.size, block_size=64, n_blocks=32, schedule=schedule) as p:
+        assert p.info()["halo"] == 1
+        for i in range(9):
+            src, ref = (ta, ra) if i % 2 == 0 else (tb, rb)
+            out.fill_(float("nan"))
+            p.execute(src, out)
+            p.check()
+            assert O.gate(out.cpu().numpy(), ref)[0], i

@@ -17,6 +17,7 @@ N_LARGE = (1 << 31) + (1 << 16)
 @pytest.mark.parametrize("kernel,B,N,order,tol", [
     ("PH", 64, 32, 8, 1e-5),
     ("PHJ", 64, 32, 8, 1e-5),
+    ("PHH", 64, 32, 4, 1e-5),
     ("STCR", 64, 64, 4, 1e-5),
     ("DTCR", 64, 128, 4, 1e-5),
     ("PLR", None, 64, 4, 1e-4),  # float32 direct form: about 5e-5 of the peak at order 4

@@ -1,7 +1,8 @@
 """A non-finite input sample (NaN or Inf at sample k).  Sequential filtering
 keeps every later output non-finite; here the bounded-depth lookback drops a
 chunk's carry once it has decayed below 2^-36 of the state, which for a
-non-finite carry means dropping it altogether.  The contract pinned here:
+non-finite carry means dropping it altogether (PHH: a chunk's carry sees only
+the 256 samples before it).  The contract pinned here:
 outputs before k are those of the finite signal (the block-state form: before
 the 32-sample K slab holding k, whose Toeplitz product multiplies the
 non-finite value by the zeros above the diagonal), output k is non-finite,
@@ -19,6 +20,7 @@ import oracle_lib as O
 @pytest.mark.parametrize("kernel,B,N,order,early", [
     ("PH", 64, 32, 8, 0),
     ("PHJ", 64, 32, 8, 0),
+    ("PHH", 64, 32, 4, 0),
     ("STCR", 64, 64, 4, 0),
     ("DTCR", 64, 64, 4, 0),
     ("PLR", None, 64, 4, 0),

@@ -60,7 +60,8 @@ def test_stream_many_calls_narrow_filter(device, schedule):
 @pytest.mark.gpu
 @pytest.mark.parametrize("kernel,geom,unrolled", [("STCR", (32, 64), False), ("STCR", (64, 64), True),
                                                   ("DTCR", (64, 64), False), ("DTCR", (32, 32), False),
-                                                  ("PHJ", (64, 32), False), ("PHJ", (64, 16), False)])
+                                                  ("PHJ", (64, 32), False), ("PHJ", (64, 16), False),
+                                                  ("PHH", (64, 32), False), ("PHH", (32, 32), False)])
 @pytest.mark.parametrize("order", [4, 16])
 def test_stream_cr_kernels(device, kernel, geom, unrolled, order):
     """STCR and DTCR carry the same direct-form history between chunks, so the

@@ -15,7 +15,7 @@ import cuda_recursive_filtering_amd as iirx
 import oracle_lib as O
 from workloads import normal_workload
 
-CASES = [("PH", (64, 32), False), ("PHJ", (64, 32), False), ("PHJ", (32, 64), False), ("STCR", (64, 64), False),
+CASES = [("PH", (64, 32), False), ("PHJ", (64, 32), False), ("PHH", (64, 32), False), ("PHJ", (32, 64), False), ("STCR", (64, 64), False),
          ("STCR", (32, 32), True),
          ("DTCR", (64, 64), False), ("DTCR", (32, 32), True)]
 
