@@ -63,9 +63,11 @@ def parse():
                     help="block-state forms (kernel:GxL) timed beside the plain fp32 kernel at orders 8 and 16: "
                          "BS = FP16 hi/lo split, BS_BF16 = BF16 hi/mid/lo split")
     ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA vs plain fp32 comparison")
-    ap.add_argument("--candidates", default="PHJ:64x32,BS:64x64",
+    ap.add_argument("--candidates", default="PHJ:64x32,PHH:64x32,BS:64x64",
                     help="further configurations (kernel:BxN) timed and gated at every order beside --kernel; "
-                         "PHJ = PH with paired sections, BS = block-state form (MFMA)")
+                         "PHJ = PH with paired sections, PHH = PH with the chunk carries from the 256 samples "
+                         "before each chunk (where the filter forgets within them, else PH), "
+                         "BS = block-state form (MFMA)")
     ap.add_argument("--workload", default="single", choices=["single", "ragged", "sweep"],
                     help="single: one 2^25-sample channel per order; ragged: the 4096-channel batch; "
                          "sweep: order 8 at Wn 0.02/0.1/0.4 over lengths 2^16..2^25")
@@ -571,6 +573,8 @@ def main():
                 cands[f"{kname} {geom}"] = str(e)
                 continue
             cands[f"{kname} {geom}"] = gated(round(N_SAMPLES / (rc["ms"] * 1e-3) / 1e9, 3), out, ref, args)
+            if kname == "PHH":  # 1: carries from the halo, 0: the filter's memory is longer and PH ran
+                cands[f"{kname} {geom}"]["halo"] = rc["plan"].info()["halo"]
             rc["plan"].close()
         if cands:
             entry["candidates"] = cands
