@@ -40,7 +40,12 @@ def parse():
     # however short --warmup is: the MFMA-heavy block-state form at order 16
     # read 405 GS/s after 5 launches and 444 after 200 (same box)
     ap.add_argument("--settle", type=int, default=200)
-    ap.add_argument("--kernel", default="PH")
+    # PHH: PH whose chunk carries come from the 256 input samples before each
+    # chunk where the filter forgets its state within them (orders 2 and 4 at
+    # Wn = 0.1), so no chunk waits on another and the launch assumes no
+    # co-residency under either schedule; at orders 8 and 16 the filter's
+    # memory is longer and the plan runs PH unchanged ("halo": 0)
+    ap.add_argument("--kernel", default="PHH")
     ap.add_argument("--geom", default="64x32")
     ap.add_argument("--schedule", default="persistent")
     ap.add_argument("--orders", default="2,4,8,16")
@@ -63,7 +68,7 @@ def parse():
                     help="block-state forms (kernel:GxL) timed beside the plain fp32 kernel at orders 8 and 16: "
                          "BS = FP16 hi/lo split, BS_BF16 = BF16 hi/mid/lo split")
     ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA vs plain fp32 comparison")
-    ap.add_argument("--candidates", default="PHJ:64x32,PHH:64x32,BS:64x64",
+    ap.add_argument("--candidates", default="PH:64x32,PHJ:64x32,BS:64x64",
                     help="further configurations (kernel:BxN) timed and gated at every order beside --kernel; "
                          "PHJ = PH with paired sections, PHH = PH with the chunk carries from the 256 samples "
                          "before each chunk (where the filter forgets within them, else PH), "
@@ -244,6 +249,10 @@ def measured_traffic(args):
         return t.get(f"{args.kernel}_{args.geom}_order2", None)
     except (OSError, ValueError):
         return None
+
+
+def kernel_is_phh(name):
+    return name.upper() in ("PHH", "PH_HALO")
 
 
 def fp64_reference(sos, x):
@@ -513,6 +522,8 @@ def main():
                  "algorithmic_gflops": round(gs * 4 * order, 1), "vgprs": r["info"]["vgprs"],
                  "scratch": r["info"]["scratch_bytes"], "chunks": r["info"]["num_chunks"],
                  "grid": r["info"]["grid"]}
+        if kernel_is_phh(args.kernel):  # 1: carries from the halo, 0: the filter's memory is longer and PH ran
+            entry["halo"] = r["info"]["halo"]
         if ceiling:
             entry["ceiling_frac"] = round(gbs / ceiling["gbs"], 4)
         # the FP64 golden of this order's input (every buffer holds the same samples)
@@ -524,7 +535,8 @@ def main():
         # one call on its own, after a synchronisation
         if args.latency_steps > 0:
             entry["latency_us"] = round(1e3 * latency(torch, r["plan"], bufs, out, stream, args.latency_steps), 2)
-        # the forward-progress-safe schedule (one workgroup per chunk, atomic ticket)
+        # the forward-progress-safe schedule (atomic tickets; PHH with halo
+        # carries needs none and runs the same no-wait launch under it)
         if not args.no_ticket and args.kernel.upper() != "PLR":
             B, N = (int(v) for v in args.geom.lower().split("x"))
             with iirx.Plan(args.kernel, sos, N_SAMPLES, block_size=B, n_blocks=N, schedule="ticket") as pt:
@@ -532,6 +544,8 @@ def main():
                                  warmup=max(args.settle, args.warmup))
             entry["ticket"] = gated(round(N_SAMPLES / (ms_t * 1e-3) / 1e9, 3), out, ref, args)
             entry["ticket"]["ms"] = round(ms_t, 5)
+            if entry.get("halo") == 1:
+                entry["ticket"]["no_wait"] = True
         if not args.no_mfma and order in (8, 16):
             # MFMA block-state form beside the plain fp32 kernel, same input, error vs FP64
             refm = ref if ref is not None else fp64_reference(sos, host)
@@ -558,7 +572,8 @@ def main():
             f0 = forms[0]
             mfma[str(order)] = {
                 "bs_gsamples_per_s": f0["gsamples_per_s"], "bs_geometry": f0["geometry"], "bs_split": f0["split"],
-                "fp32_gsamples_per_s": entry["gsamples_per_s"], "fp32_kernel": f"{args.kernel} {args.geom}",
+                "fp32_gsamples_per_s": entry["gsamples_per_s"],
+                "fp32_kernel": f"{args.kernel} {args.geom}" + (" (no halo: PH)" if entry.get("halo") == 0 else ""),
                 "bs_max_err_over_rms": f0["max_err_over_rms"], "fp32_max_err_over_rms": e_fp32,
                 "bs_rms_err_over_rms": f0["rms_err_over_rms"], "fp32_rms_err_over_rms": r_fp32,
                 "bs_gate_pass": f0["gate_pass"], "bs_gate_rel": f0["gate_rel"], "forms": forms}
@@ -573,7 +588,7 @@ def main():
                 cands[f"{kname} {geom}"] = str(e)
                 continue
             cands[f"{kname} {geom}"] = gated(round(N_SAMPLES / (rc["ms"] * 1e-3) / 1e9, 3), out, ref, args)
-            if kname == "PHH":  # 1: carries from the halo, 0: the filter's memory is longer and PH ran
+            if kernel_is_phh(kname):  # 1: carries from the halo, 0: the filter's memory is longer and PH ran
                 cands[f"{kname} {geom}"]["halo"] = rc["plan"].info()["halo"]
             rc["plan"].close()
         if cands:

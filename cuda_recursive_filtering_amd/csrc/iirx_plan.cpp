@@ -289,9 +289,11 @@ int plan_grid(const iirx_config *cfg, const Shape &sh, uint32_t main_chunks, con
     const int64_t blocks_needed = ((int64_t)main_chunks + sh.chunks_per_block - 1) / sh.chunks_per_block;
     grid = (uint32_t)blocks_needed;
     const int wv = ph_ticket_waves(sh.n_blocks);
-    const bool grouped = (is_ph(cfg->kernel) || cfg->kernel == IIRX_KERNEL_STCR) &&
+    // PH with halo carries has no inter-chunk wait: one workgroup per resident
+    // slot under either schedule, a grid-stride loop with no ticket
+    const bool grouped = (is_ph(cfg->kernel) || cfg->kernel == IIRX_KERNEL_STCR) && !sh.halo &&
                          cfg->schedule == IIRX_SCHEDULE_TICKET && main_chunks > 0 && main_chunks % (uint32_t)wv == 0;
-    if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && !grouped) return IIRX_OK;
+    if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && !grouped && !sh.halo) return IIRX_OK;
     const void *occ_fn = fn;
     int occ_threads = threads;
     int64_t want = blocks_needed;

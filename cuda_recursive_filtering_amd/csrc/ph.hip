@@ -32,8 +32,9 @@ namespace iirx {
 // HALO: every section's chunk carry from the kPhHalo input samples before the
 // chunk (4 per lane, loaded with the chunk): y_s[-2], y_s[-1] = wave sums of
 // the lane's samples times its K2_s, K1_s weights.  No chunk reads another's
-// record, so there is no lookback; the full records are still written (the
-// partial last chunk and the end state read the last one's).
+// record, so there is no lookback and no schedule (no ticket, no epoch,
+// nothing that needs workgroups to be co-resident); the full records are
+// still written (the partial last chunk and the end state read the last one's).
 // JOINT: sections in pairs (s, s + 1) with one lookback of the pair's joint
 // state per chunk (an odd last section alone, and so a pair whose tables
 // exceed kPairLimit).  Both sections' zero-state
@@ -286,8 +287,25 @@ __global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
     if constexpr (LAST) stage_out_part<L, N>(lds, a.out + base, a.last_valid);
     else stage_out<L, N>(lds, a.out + base, lane);
     };
-    if constexpr (LAST) body(a.num_chunks, 0u, 0u);
-    else for_each_chunk<WV>(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err, body);
+    if constexpr (LAST) {
+        body(a.num_chunks, 0u, 0u);
+    } else if constexpr (HALO) {
+        // No chunk waits on another, so no order of the workgroups can stall
+        // the launch: a grid-stride loop with no ticket and no epoch, under
+        // either schedule.  The full records carry flag 0; their only readers
+        // (the partial last chunk and the end state) run after this launch
+        // on the same stream and read the values alone.
+        // Chunk order: workgroups go to the 8 XCDs round-robin by index, so
+        // workgroup b takes position (b mod 8) G/8 + b / 8 of every round of
+        // G chunks: each XCD works through a contiguous run, and a chunk's
+        // halo is its predecessor's input, read through the same L2.
+        static_assert(WV == 1, "the halo instance runs one wave per workgroup");
+        const uint32_t G = gridDim.x, b = blockIdx.x;
+        const uint32_t p = (G % 8u == 0u) ? (b % 8u) * (G / 8u) + b / 8u : b;
+        for (uint32_t c = p; c < a.num_chunks; c += G) body(c, 0u, 0u);
+    } else {
+        for_each_chunk<WV>(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err, body);
+    }
 }
 
 // zf = x[n-2], x[n-1], then (y2, y1) of every section from the last chunk's
@@ -315,9 +333,10 @@ hipError_t launch_ph_end_state(const LaunchArgs &a, int chunk_samples, float *zf
 template <int N, bool J, bool H>
 static hipError_t launch_n(const LaunchArgs &a, bool last, hipStream_t s)
 {
-    constexpr int WV = ph_ticket_waves(N);
+    // the halo instance has no grouped form (no ticket)
+    constexpr int WV = H ? 1 : ph_ticket_waves(N);
     if (last) hipLaunchKernelGGL((ph_kernel<N, true>), dim3(1), dim3(64), 0, s, a);
-    else if (!a.persistent && a.num_chunks % WV == 0)
+    else if (!H && !a.persistent && a.num_chunks % WV == 0)
         hipLaunchKernelGGL((ph_kernel<N, false, WV, J, H>), dim3(a.grid), dim3(64 * WV), 0, s, a);
     else hipLaunchKernelGGL((ph_kernel<N, false, 1, J, H>), dim3(a.grid), dim3(64), 0, s, a);
     return hipGetLastError();
@@ -347,7 +366,7 @@ hipError_t launch_ph(int lanes, int n_blocks, const LaunchArgs &a, hipStream_t s
 template <int N, bool J, bool H>
 static const void *fn_n(bool grouped)
 {
-    if (grouped) return (const void *)ph_kernel<N, false, ph_ticket_waves(N), J, H>;
+    if (grouped && !H) return (const void *)ph_kernel<N, false, H ? 1 : ph_ticket_waves(N), J, H>;
     return (const void *)ph_kernel<N, false, 1, J, H>;
 }
 

@@ -114,7 +114,10 @@ typedef enum {
                                   to the unit circle) exactly PH.
                                   iirx_plan_info.halo says which.  Same
                                   geometry, records, streaming state and
-                                  any-length rules as PH */
+                                  any-length rules as PH.  With halo carries
+                                  no chunk waits on another, so both
+                                  schedules run the same grid-stride launch
+                                  (no ticket, no co-residency assumed) */
 } iirx_kernel;
 
 /*
@@ -145,7 +148,8 @@ typedef enum {
  *               tickets in order (about 12 ns each, serialised, on MI355X),
  *               one per group of consecutive chunks for PH and STCR (16 at
  *               N_BLOCKS <= 32), one per chunk for DTCR, PLR and the
- *               block-state form.
+ *               block-state form.  PH_HALO with halo carries takes neither
+ *               (no chunk waits on another).
  */
 typedef struct {
     int32_t kernel;

@@ -186,3 +186,28 @@ def test_halo_launches_alternate(device, schedule):
             p.execute(src, out)
             p.check()
             assert O.gate(out.cpu().numpy(), ref)[0], i
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("schedule", ["persistent", "ticket"])
+def test_halo_never_waits(device, schedule):
+    """With halo carries no chunk waits on another, so the launch needs no
+    ticket and no co-residency: under a one-poll spin limit (any wait would
+    give up and report IIRX_ERR_TIMEOUT) both schedules finish with no
+    timeout and a gated result, on a grid no larger than the resident slots
+    and with more chunks than workgroups."""
+    import torch
+    print(f"case PHH never-waits schedule={schedule}", flush=True)
+    sos, g, x = normal_workload(4, 1 << 25, seed=24, wn=0.1)
+    xin = torch.from_numpy(x).to(device)
+    out = torch.empty_like(xin)
+    with iirx.Plan("PHH", sos, x.size, block_size=64, n_blocks=32, schedule=schedule) as p:
+        info = p.info()
+        assert info["halo"] == 1
+        assert info["grid"] < info["num_chunks"], info
+        p.set_spin_limit(1)
+        for _ in range(3):
+            p.execute(xin, out)
+            p.check()
+        torch.cuda.synchronize()
+        assert O.gate(out.cpu().numpy(), O.golden(sos, x))[0]
