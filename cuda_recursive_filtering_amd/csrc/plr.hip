@@ -45,13 +45,30 @@ __device__ __forceinline__ float plr_bcast(float v)
     }
 }
 
+// merge factors hoisted per lane (one-wave chunks, K <= 4): level H takes
+// min(K, H) of them, at offset sum over lower levels
+template <int K>
+constexpr int plr_merge_off(int lv)
+{
+    int o = 0;
+    for (int l = 0; l < lv; ++l) o += ((1 << l) < K) ? (1 << l) : K;
+    return o;
+}
+
 template <int K, int B, int X>
 __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
 {
     constexpr int W = B / 64;
     constexpr int CH = B * X;
+    // the chunk sits at xs[HB ..] (16-byte aligned for whole-line staging),
+    // its K samples of input history just before it
+    constexpr int HB = 16;
+    static_assert(K <= HB, "history fits before the chunk");
     constexpr PlrLayout Y = plr_layout(K, CH);
-    __shared__ float xs[K + CH];         // raw chunk with K samples of history
+    // one-wave chunks with K <= 4 keep the merge factors in registers
+    constexpr bool kHoist = (B == 64) && (K <= 4);
+    constexpr int NM = plr_merge_off<K>(6);
+    __shared__ __attribute__((aligned(16))) float xs[HB + CH];  // raw chunk with K samples of history
     __shared__ float sp[W * X * K];      // last K values of every wave and segment
     __shared__ float sv[K];              // segment-chain and carry broadcast
     __shared__ float lp[64 * K];         // lookback partials, nearest first
@@ -60,41 +77,77 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
     const float *tab = a.coef;
     const cfloat *ctab = as_const(a.coef);
     const float *fac = tab + Y.kFac;
+    const cfloat *cfac = ctab + Y.kFac;
+
+    // chunk-invariant per-lane factors, loaded once per workgroup: fac[m][t]
+    // (segment chain and carry injection) and the merge factors
+    float ft[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) ft[m] = fac[m * CH + t];
+    float fmg[kHoist ? NM : 1];
+    if constexpr (kHoist) {
+        static_for<0, 6, 1>([&](auto LV) {
+            constexpr int lv = decltype(LV)::value, H = 1 << lv, M0 = K > H ? K - H : 0;
+            static_for<M0, K, 1>([&](auto MI) {
+                constexpr int M = decltype(MI)::value;
+                fmg[plr_merge_off<K>(lv) + M - M0] = fac[M * CH + (lane & (H - 1))];
+            });
+        });
+    }
+    // whole chunks move as 16-byte accesses staged through LDS when both
+    // buffers are 16-byte aligned (every chunk start then is); otherwise,
+    // and for the partial last chunk, one float per lane and segment
+    const bool vec_io = (X % 4 == 0) &&
+                        ((reinterpret_cast<uintptr_t>(a.in) | reinterpret_cast<uintptr_t>(a.out)) & 15) == 0;
 
     for_each_chunk(a.ticket, a.epoch, a.arrive, a.num_chunks, a.persistent, a.spin_limit, a.err,
                    [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
     const long cs = (long)chunk * CH;
+    const bool whole = vec_io && cs + CH <= a.n_samples;
     __syncthreads();
     float val[X];
+    if (whole) {
+        const f4 *src = reinterpret_cast<const f4 *>(a.in + cs);
+        f4 *dst = reinterpret_cast<f4 *>(xs + HB);
+        f4 q4[X / 4 > 0 ? X / 4 : 1];
 #pragma unroll
-    for (int v = 0; v < X; ++v) {
-        const long idx = cs + t + (long)v * B;
-        val[v] = (idx < a.n_samples) ? a.in[idx] : 0.0f;
-        xs[K + t + v * B] = val[v];
+        for (int q = 0; q < X / 4; ++q) q4[q] = src[t + q * B];
+#pragma unroll
+        for (int q = 0; q < X / 4; ++q) dst[t + q * B] = q4[q];
+    } else {
+#pragma unroll
+        for (int v = 0; v < X; ++v) {
+            const long idx = cs + t + (long)v * B;
+            xs[HB + t + v * B] = (idx < a.n_samples) ? a.in[idx] : 0.0f;
+        }
     }
     if (t < K) {
         // input history; before sample 0 the streamed-in x[-K..-1] (zero by default)
         const long hi = cs - K + t;
-        xs[t] = (hi >= 0) ? a.in[hi] : (a.zi ? a.zi[K + hi] : 0.0f);
+        xs[HB - K + t] = (hi >= 0) ? a.in[hi] : (a.zi ? a.zi[K + hi] : 0.0f);
     }
     __syncthreads();
+#pragma unroll
+    for (int v = 0; v < X; ++v) val[v] = xs[HB + t + v * B];
 
     // 1. FIR: val += b[j] x[i-1-j], j ascending
 #pragma unroll
     for (int v = 0; v < X; ++v) {
 #pragma unroll
-        for (int j = 0; j < K; ++j) val[v] = fmaf(ctab[Y.kB + j], xs[K + t + v * B - 1 - j], val[v]);
+        for (int j = 0; j < K; ++j) val[v] = fmaf(ctab[Y.kB + j], xs[HB + t + v * B - 1 - j], val[v]);
     }
 
     // 2a. merge levels inside the wave
     if constexpr (B == 64) {
         static_for<0, 6, 1>([&](auto LV) {
-            constexpr int H = 1 << decltype(LV)::value;
+            constexpr int lv = decltype(LV)::value, H = 1 << lv, M0 = K > H ? K - H : 0;
             const bool up = (lane & H) != 0;
             const int i = lane & (H - 1);
-            static_for<(K > H ? K - H : 0), K, 1>([&](auto MI) {
+            static_for<M0, K, 1>([&](auto MI) {
                 constexpr int M = decltype(MI)::value;
-                const float f = fac[M * CH + i];
+                float f;
+                if constexpr (kHoist) f = fmg[plr_merge_off<K>(lv) + M - M0];
+                else f = fac[M * CH + i];
 #pragma unroll
                 for (int v = 0; v < X; ++v) {
                     const float sv_ = plr_bcast<H, K, M>(val[v]);
@@ -150,14 +203,14 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
 #pragma unroll
             for (int m = 0; m < K; ++m) {
                 const float pm = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(val[v - 1]), B - K + m));
-                val[v] = fmaf(fac[m * CH + t], pm, val[v]);
+                val[v] = fmaf(ft[m], pm, val[v]);
             }
         } else {
             __syncthreads();
             if (t >= B - K) sv[t - (B - K)] = val[v - 1];
             __syncthreads();
 #pragma unroll
-            for (int m = 0; m < K; ++m) val[v] = fmaf(fac[m * CH + t], sv[m], val[v]);
+            for (int m = 0; m < K; ++m) val[v] = fmaf(ft[m], sv[m], val[v]);
         }
     }
 
@@ -182,19 +235,48 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
         for (int m = 0; m < K; ++m) Xv[m] = sv[m];
     }
 
-    // carry injection (plr:891-899)
+    // carry injection (plr:891-899).  The carry's share of sample t + v B is
+    // sum_m fac[m][t + v B] X_m; it equals the zero-input response at lane t
+    // of the carry's share of the K values before segment v,
+    // W_v[m] = sum_n fac[n][v B - K + m] X_n (W_0 = X), so lane t adds
+    // sum_m fac[m][t] W_v[m] with the K per-lane factors of the segment chain
+    // and uniform W_v, instead of K X per-lane table loads per chunk
 #pragma unroll
-    for (int m = 0; m < K; ++m) {
-        const float Xm = Xv[m];
+    for (int v = 0; v < X; ++v) {
+        float Wv[K];
 #pragma unroll
-        for (int v = 0; v < X; ++v) val[v] = fmaf(fac[m * CH + t + v * B], Xm, val[v]);
+        for (int m = 0; m < K; ++m) {
+            if (v == 0) {
+                Wv[m] = Xv[m];
+            } else {
+                float acc = 0.0f;
+#pragma unroll
+                for (int n = 0; n < K; ++n) acc = fmaf(cfac[n * CH + v * B - K + m], Xv[n], acc);
+                Wv[m] = acc;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < K; ++m) val[v] = fmaf(ft[m], Wv[m], val[v]);
     }
     if (t >= B - K) rec_store(rec + K + (t - (B - K)), full_flag, val[X - 1]);
 
+    if (whole) {
+        // every wave is past its FIR reads of xs (the lookback and merge
+        // barriers, or program order within one wave)
+        __syncthreads();
 #pragma unroll
-    for (int v = 0; v < X; ++v) {
-        const long idx = cs + t + (long)v * B;
-        if (idx < a.n_samples) a.out[idx] = val[v];
+        for (int v = 0; v < X; ++v) xs[HB + t + v * B] = val[v];
+        __syncthreads();
+        const f4 *srcl = reinterpret_cast<const f4 *>(xs + HB);
+        f4 *dst = reinterpret_cast<f4 *>(a.out + cs);
+#pragma unroll
+        for (int q = 0; q < X / 4; ++q) dst[t + q * B] = srcl[t + q * B];
+    } else {
+#pragma unroll
+        for (int v = 0; v < X; ++v) {
+            const long idx = cs + t + (long)v * B;
+            if (idx < a.n_samples) a.out[idx] = val[v];
+        }
     }
     });
 }
