@@ -240,7 +240,18 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
     // of the carry's share of the K values before segment v,
     // W_v[m] = sum_n fac[n][v B - K + m] X_n (W_0 = X), so lane t adds
     // sum_m fac[m][t] W_v[m] with the K per-lane factors of the segment chain
-    // and uniform W_v, instead of K X per-lane table loads per chunk
+    // and uniform W_v, instead of K X per-lane table loads per chunk.  At
+    // K = 16 the K^2 X uniform products (and their scalar table loads) cost
+    // more than the loads they replace: the table form there (order 16, 2^25
+    // samples: 27.7 GS/s against 10.3; order 8: 65 against 121).
+    if constexpr (K >= 16) {
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            const float Xm = Xv[m];
+#pragma unroll
+            for (int v = 0; v < X; ++v) val[v] = fmaf(fac[m * CH + t + v * B], Xm, val[v]);
+        }
+    } else {
 #pragma unroll
     for (int v = 0; v < X; ++v) {
         float Wv[K];
@@ -257,6 +268,7 @@ __global__ __launch_bounds__(B) void plr_kernel(LaunchArgs a)
         }
 #pragma unroll
         for (int m = 0; m < K; ++m) val[v] = fmaf(ft[m], Wv[m], val[v]);
+    }
     }
     if (t >= B - K) rec_store(rec + K + (t - (B - K)), full_flag, val[X - 1]);
 
