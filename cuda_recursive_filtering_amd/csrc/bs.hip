@@ -21,7 +21,11 @@
 //      FP64: Phi X_in in tile 0, a 4-level scan inside one 16-tile block and
 //      Phi^(16 nb) products for the other blocks;
 //   4. Y = Y0 + O S, S_k = state entering tile k rounded to FP32,
-//      accumulated onto Y0 by the FP32 matrix core.
+//      accumulated onto Y0 by the FP32 matrix core; from 9 states on (orders
+//      above 8) by the BF16 one, O and S split hi/mid/lo (six 16 x 16 x 16
+//      products per block against up to four 16 x 16 x 4 FP32 ones of four
+//      times the cycles each; order 16, 2^25 samples: 456 against 452 GS/s,
+//      max error 4.99e-6 against 5.08e-6 of the output RMS).
 // MFMA columns are tiles, so every product feeds the next from registers and
 // the scan's tile shifts are DPP row rotations.  The FP64 shape puts row
 // l/16 + 4 j of its result on lane l, register j: states are kept in that

@@ -105,6 +105,44 @@ __device__ __forceinline__ f4 split_product(const AOp &a, const f4 &xa, const f4
     return acc;
 }
 
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4 mfma_b16(bf4 a, bf4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+// BF16 hi/mid/lo split of the first NSL values of v (the rest zero): each
+// component the remainder so far rounded to nearest, so hi + mid + lo holds
+// v's 24-bit significand
+template <int NSL>
+__device__ __forceinline__ void split4_bf(const f4 &v, bf4 &hi, bf4 &mid, bf4 &lo)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float x = (j < NSL) ? v[j] : 0.0f;
+        const __bf16 h = (__bf16)x;
+        const float r1 = x - (float)h;
+        const __bf16 m = (__bf16)r1;
+        hi[j] = h;
+        mid[j] = m;
+        lo[j] = (__bf16)(r1 - (float)m);
+    }
+}
+
+// acc + O S for one 16 x 16 block on the BF16 matrix core (16 x 16 x 16):
+// lane group g, register r of both operands is state 4 r + g (the FP32
+// slab layout below), taken as K index 4 g + r of the product, so neither
+// operand moves; the six products above 2^-24 of hi hi, smallest first
+__device__ __forceinline__ f4 apply_bf3(const bf4 &oh, const bf4 &om, const bf4 &ol, const bf4 &sh, const bf4 &sm,
+                                        const bf4 &sl, f4 acc)
+{
+    acc = mfma_b16(ol, sh, acc);
+    acc = mfma_b16(oh, sl, acc);
+    acc = mfma_b16(om, sm, acc);
+    acc = mfma_b16(om, sh, acc);
+    acc = mfma_b16(oh, sm, acc);
+    acc = mfma_b16(oh, sh, acc);
+    return acc;
+}
+
 // acc + M S over the first NSL K slabs; M is this lane's float4 of an FP32
 // block (slab m = states 4 m .. 4 m + 3)
 template <int NSL>
@@ -455,6 +493,19 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
 #pragma unroll
         for (int j = 0; j < 4; ++j) tail[g + 4 * j] = (j < NSL) ? S[NB - 1][j] : 0.0f;
     }
+    // Y = Y0 + O S: from three K slabs on (orders 9-16) on the BF16 matrix
+    // core, six 16 x 16 x 16 products per block against NSL FP32 ones of
+    // four times the cycles each; below that the FP32 products
+    constexpr bool kOsBf = NSL >= 3;
+    bf4 oh[kOsBf ? MB : 1], om[kOsBf ? MB : 1], ol[kOsBf ? MB : 1];
+    if constexpr (kOsBf) {
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) split4_bf<NSL>(Ob[mb], oh[mb], om[mb], ol[mb]);
+    }
+    auto out_block = [&](int mb, int nb, const bf4 &sh, const bf4 &sm, const bf4 &sl) {
+        if constexpr (kOsBf) return apply_bf3(oh[mb], om[mb], ol[mb], sh, sm, sl, Y0[mb][nb]);
+        else return apply<NSL>(Ob[mb], S[nb], Y0[mb][nb]);
+    };
     if constexpr (LINES) {
         // whole 128-byte lines: row blocks mb and mb + 1 of a tile are one
         // line, held by lanes col and col + 8 of the two blocks; a DPP
@@ -465,10 +516,12 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
         const int row = 16 * (col >> 3) + 4 * g;
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
+            bf4 sh, sm, sl;
+            if constexpr (kOsBf) split4_bf<NSL>(S[nb], sh, sm, sl);
 #pragma unroll
             for (int mb = 0; mb < MB; mb += 2) {
-                const f4 ya = apply<NSL>(Ob[mb], S[nb], Y0[mb][nb]);
-                const f4 yb = apply<NSL>(Ob[mb + 1], S[nb], Y0[mb + 1][nb]);
+                const f4 ya = out_block(mb, nb, sh, sm, sl);
+                const f4 yb = out_block(mb + 1, nb, sh, sm, sl);
                 f4 A, B;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -488,8 +541,10 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
         // store(nb, mb, y): row block mb of tile block nb.
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
+            bf4 sh, sm, sl;
+            if constexpr (kOsBf) split4_bf<NSL>(S[nb], sh, sm, sl);
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb) store(nb, mb, apply<NSL>(Ob[mb], S[nb], Y0[mb][nb]));
+            for (int mb = 0; mb < MB; ++mb) store(nb, mb, out_block(mb, nb, sh, sm, sl));
         }
     }
 }
