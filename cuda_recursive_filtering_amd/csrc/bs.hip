@@ -22,10 +22,12 @@
 //      Phi^(16 nb) products for the other blocks;
 //   4. Y = Y0 + O S, S_k = state entering tile k rounded to FP32,
 //      accumulated onto Y0 by the FP32 matrix core; from 9 states on (orders
-//      above 8) by the BF16 one, O and S split hi/mid/lo (six 16 x 16 x 16
-//      products per block against up to four 16 x 16 x 4 FP32 ones of four
-//      times the cycles each; order 16, 2^25 samples: 456 against 452 GS/s,
-//      max error 4.99e-6 against 5.08e-6 of the output RMS).
+//      above 8) by the BF16 one, O and S split hi/mid/lo, the six products
+//      above 2^-24 in pairs, three 16 x 16 x 32 MFMAs per block (about 18
+//      clk each) against up to four 16 x 16 x 4 FP32 ones (32 clk each).
+//      Order 16, 2^25 samples: 466 against 452 GS/s, max error 4.88e-6
+//      against 5.08e-6 of the output RMS; at order 8 (two FP32 products) the
+//      BF16 form ran 0.6% faster with a 10% larger max error and is not used.
 // MFMA columns are tiles, so every product feeds the next from registers and
 // the scan's tile shifts are DPP row rotations.  The FP64 shape puts row
 // l/16 + 4 j of its result on lane l, register j: states are kept in that

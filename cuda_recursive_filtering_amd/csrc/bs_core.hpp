@@ -127,19 +127,37 @@ __device__ __forceinline__ void split4_bf(const f4 &v, bf4 &hi, bf4 &mid, bf4 &l
     }
 }
 
-// acc + O S for one 16 x 16 block on the BF16 matrix core (16 x 16 x 16):
-// lane group g, register r of both operands is state 4 r + g (the FP32
-// slab layout below), taken as K index 4 g + r of the product, so neither
-// operand moves; the six products above 2^-24 of hi hi, smallest first
+__device__ __forceinline__ bf8 cat_bf(const bf4 &a, const bf4 &b)
+{
+    return bf8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+}
+
+// acc + O S for one 16 x 16 block on the BF16 matrix core; the six split
+// products above 2^-24 of hi hi, smallest first.  State 4 r + g is register
+// r of lane group g in both operands (the FP32 slab layout below), so
+// neither operand moves between lanes.
+// PAIRS: two products per 16 x 16 x 32 MFMA (about 18 clk): lane group g
+// holds K indices 8 g .. 8 g + 7, of which 8 g + r takes state 4 r + g of
+// the first plane and 8 g + 4 + r the same state of the second; the pairs
+// hi lo + lo hi, hi mid + mid hi, hi hi + mid mid.  Otherwise one product per
+// 16 x 16 x 16 MFMA (about 17.5 clk), K index 4 g + r = state 4 r + g, which
+// needs no concatenated operands (fewer registers: the L = 128 instances).
+template <bool PAIRS>
 __device__ __forceinline__ f4 apply_bf3(const bf4 &oh, const bf4 &om, const bf4 &ol, const bf4 &sh, const bf4 &sm,
                                         const bf4 &sl, f4 acc)
 {
-    acc = mfma_b16(ol, sh, acc);
-    acc = mfma_b16(oh, sl, acc);
-    acc = mfma_b16(om, sm, acc);
-    acc = mfma_b16(om, sh, acc);
-    acc = mfma_b16(oh, sm, acc);
-    acc = mfma_b16(oh, sh, acc);
+    if constexpr (PAIRS) {
+        acc = mfma_b(cat_bf(oh, ol), cat_bf(sl, sh), acc);
+        acc = mfma_b(cat_bf(oh, om), cat_bf(sm, sh), acc);
+        acc = mfma_b(cat_bf(oh, om), cat_bf(sh, sm), acc);
+    } else {
+        acc = mfma_b16(ol, sh, acc);
+        acc = mfma_b16(oh, sl, acc);
+        acc = mfma_b16(om, sm, acc);
+        acc = mfma_b16(om, sh, acc);
+        acc = mfma_b16(oh, sm, acc);
+        acc = mfma_b16(oh, sh, acc);
+    }
     return acc;
 }
 
@@ -494,8 +512,8 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
         for (int j = 0; j < 4; ++j) tail[g + 4 * j] = (j < NSL) ? S[NB - 1][j] : 0.0f;
     }
     // Y = Y0 + O S: from three K slabs on (orders 9-16) on the BF16 matrix
-    // core, six 16 x 16 x 16 products per block against NSL FP32 ones of
-    // four times the cycles each; below that the FP32 products
+    // core (apply_bf3: three or six MFMAs of about 18 clk per block against
+    // NSL FP32 ones of 32 clk); below that the FP32 products
     constexpr bool kOsBf = NSL >= 3;
     bf4 oh[kOsBf ? MB : 1], om[kOsBf ? MB : 1], ol[kOsBf ? MB : 1];
     if constexpr (kOsBf) {
@@ -503,7 +521,7 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
         for (int mb = 0; mb < MB; ++mb) split4_bf<NSL>(Ob[mb], oh[mb], om[mb], ol[mb]);
     }
     auto out_block = [&](int mb, int nb, const bf4 &sh, const bf4 &sm, const bf4 &sl) {
-        if constexpr (kOsBf) return apply_bf3(oh[mb], om[mb], ol[mb], sh, sm, sl, Y0[mb][nb]);
+        if constexpr (kOsBf) return apply_bf3<(MB <= 4)>(oh[mb], om[mb], ol[mb], sh, sm, sl, Y0[mb][nb]);
         else return apply<NSL>(Ob[mb], S[nb], Y0[mb][nb]);
     };
     if constexpr (LINES) {
