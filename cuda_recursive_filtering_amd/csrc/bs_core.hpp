@@ -513,8 +513,11 @@ __device__ __forceinline__ void bs_chunk(const BsRaw<G / 16, L / 32> &xr, uint32
     }
     // Y = Y0 + O S: from three K slabs on (orders 9-16) on the BF16 matrix
     // core (apply_bf3: three or six MFMAs of about 18 clk per block against
-    // NSL FP32 ones of 32 clk); below that the FP32 products
-    constexpr bool kOsBf = NSL >= 3;
+    // NSL FP32 ones of 32 clk); below that the FP32 products.  The BF16
+    // hi/mid/lo split of T and Gam keeps the FP32 products (order 16, 32 x 64:
+    // no faster with the BF16 ones, max error 6.6e-6 against 5.7e-6 of the
+    // output RMS)
+    constexpr bool kOsBf = NSL >= 3 && SPLIT == kBsSplitF16;
     bf4 oh[kOsBf ? MB : 1], om[kOsBf ? MB : 1], ol[kOsBf ? MB : 1];
     if constexpr (kOsBf) {
 #pragma unroll
