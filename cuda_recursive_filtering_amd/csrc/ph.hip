@@ -47,6 +47,16 @@ namespace iirx {
 // sections (8 FMA zero-state + 4 FMA correction against 2 x (4 + 2)); the
 // scan takes 12 FMA per level against 2 x 4, and a chunk waits for one
 // lookback instead of two.
+// Unpadded chunk tile with 16-byte LDS accesses (the halo instance at N =
+// 32): lane l's samples 4 k .. 4 k + 3 sit in float4 slot 8 l + ((k + l +
+// (l >> 3)) mod 8).  Both the per-lane row accesses (slot(l, k) over the 64
+// lanes) and the accesses in address order (float4 g = 64 i + lane at
+// slot(g / 8, g mod 8)) hit 16 distinct 16-byte bank groups in every 16
+// consecutive lanes and 8 in every 8 (checked by brute force for 32 and 64
+// banks); one ds_read/ds_write_b128 replaces four 4-byte ones of the padded
+// tile.
+__device__ __forceinline__ int swz_slot(int l, int k) { return 8 * l + ((k + l + (l >> 3)) & 7); }
+
 template <int N, bool LAST, int WV = 1, bool JOINT = false, bool HALO = false>
 __global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
 {
@@ -57,6 +67,9 @@ __global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
     float *lds = lds_w[WV == 1 ? 0 : threadIdx.x >> 6];
 
     const int lane = WV == 1 ? (int)threadIdx.x : (int)(threadIdx.x & 63);
+    // the 16-byte tile (halo instance, N = 32)
+    constexpr bool SWZ = HALO && !LAST && WV == 1 && N == 32;
+    f4 *lds4 = reinterpret_cast<f4 *>(lds);
     const auto body = [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
     const long base = (long)chunk * (L * N);
 
@@ -65,28 +78,50 @@ __global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
     f4 hx = {0.0f, 0.0f, 0.0f, 0.0f};
     if constexpr (HALO && !LAST)
         if (chunk > 0) hx = *reinterpret_cast<const f4 *>(a.in + base - kPhHalo + 4 * lane);
-    if constexpr (LAST) stage_in_part<L, N>(a.in + base, lds, a.last_valid);
-    else stage_in<L, N>(a.in + base, lds, lane);
-    chunk_sync<WV>();
-
     float r[N];
-#pragma unroll
-    for (int n = 0; n < N; ++n) r[n] = lds[lane * (N + 1) + n];
-
-    // inputs of the first section just before this lane's samples
     float xi2, xi1;
-    if (lane > 0) {
-        xi2 = lds[(lane - 1) * (N + 1) + N - 2];
-        xi1 = lds[(lane - 1) * (N + 1) + N - 1];
-    } else if (chunk > 0) {
-        xi2 = a.in[base - 2];
-        xi1 = a.in[base - 1];
-    } else if (a.zi) {
-        xi2 = a.zi[0];
-        xi1 = a.zi[1];
+    if constexpr (SWZ) {
+        const f4 *src = reinterpret_cast<const f4 *>(a.in + base);
+        f4 v[N / 4];
+#pragma unroll
+        for (int i = 0; i < N / 4; ++i) v[i] = __builtin_nontemporal_load(src + 64 * i + lane);
+#pragma unroll
+        for (int i = 0; i < N / 4; ++i) lds4[swz_slot(8 * i + (lane >> 3), lane & 7)] = v[i];
+        chunk_sync<WV>();
+#pragma unroll
+        for (int k = 0; k < N / 4; ++k) {
+            const f4 w = lds4[swz_slot(lane, k)];
+            r[4 * k] = w[0];
+            r[4 * k + 1] = w[1];
+            r[4 * k + 2] = w[2];
+            r[4 * k + 3] = w[3];
+        }
+        const f4 w = lds4[swz_slot(lane > 0 ? lane - 1 : 0, N / 4 - 1)];
+        xi2 = w[2];
+        xi1 = w[3];
     } else {
-        xi2 = 0.0f;
-        xi1 = 0.0f;
+        if constexpr (LAST) stage_in_part<L, N>(a.in + base, lds, a.last_valid);
+        else stage_in<L, N>(a.in + base, lds, lane);
+        chunk_sync<WV>();
+#pragma unroll
+        for (int n = 0; n < N; ++n) r[n] = lds[lane * (N + 1) + n];
+        xi2 = lds[(lane > 0 ? lane - 1 : 0) * (N + 1) + N - 2];
+        xi1 = lds[(lane > 0 ? lane - 1 : 0) * (N + 1) + N - 1];
+    }
+
+    // inputs of the first section just before this lane's samples: lane l > 0
+    // took the last two of row l - 1 above; lane 0 those before the chunk
+    if (lane == 0) {
+        if (chunk > 0) {
+            xi2 = a.in[base - 2];
+            xi1 = a.in[base - 1];
+        } else if (a.zi) {
+            xi2 = a.zi[0];
+            xi1 = a.zi[1];
+        } else {
+            xi2 = 0.0f;
+            xi1 = 0.0f;
+        }
     }
     // the end state of the signal: samples last_valid - 2 and - 1
     if constexpr (LAST)
@@ -281,11 +316,20 @@ __global__ __launch_bounds__(64 * WV) void ph_kernel(LaunchArgs a)
     }
 
     chunk_sync<WV>();
+    if constexpr (SWZ) {
 #pragma unroll
-    for (int n = 0; n < N; ++n) lds[lane * (N + 1) + n] = r[n];
-    chunk_sync<WV>();
-    if constexpr (LAST) stage_out_part<L, N>(lds, a.out + base, a.last_valid);
-    else stage_out<L, N>(lds, a.out + base, lane);
+        for (int k = 0; k < N / 4; ++k) lds4[swz_slot(lane, k)] = f4{r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]};
+        chunk_sync<WV>();
+        f4 *dst = reinterpret_cast<f4 *>(a.out + base);
+#pragma unroll
+        for (int i = 0; i < N / 4; ++i) dst[64 * i + lane] = lds4[swz_slot(8 * i + (lane >> 3), lane & 7)];
+    } else {
+#pragma unroll
+        for (int n = 0; n < N; ++n) lds[lane * (N + 1) + n] = r[n];
+        chunk_sync<WV>();
+        if constexpr (LAST) stage_out_part<L, N>(lds, a.out + base, a.last_valid);
+        else stage_out<L, N>(lds, a.out + base, lane);
+    }
     };
     if constexpr (LAST) {
         body(a.num_chunks, 0u, 0u);
