@@ -3,12 +3,13 @@
 Metric (the reference's headline, README.md:34-39): GSamples/s and % of the
 HBM roof at 2^25 float32 samples, orders 2-16.  One step = one filter call
 (one grid-wide kernel launch) over 2^25 samples already resident in HBM.
-`value` is the order-2 step of the configured algorithm; every order is
+`value` is the order-2 step of the configured algorithm (PHH 64x32 by
+default: PH with halo carries); every order is
 measured and reported under "orders".  Inputs rotate over enough buffers to
 exceed twice the 256 MB Infinity Cache, so every launch streams from HBM
 (the reference rotates against its L2, test_performance.cu:140-157).
 
-    python bench.py [--steps K] [--warmup W] [--kernel PH] [--geom 64x64]
+    python bench.py [--steps K] [--warmup W] [--kernel PHH] [--geom 64x32]
 """
 import argparse
 import json
@@ -38,8 +39,10 @@ def parse():
     # (the other orders, the candidates, the reference grid, the ticket
     # schedule and the MFMA forms), so that each is timed at settled clocks
     # however short --warmup is: the MFMA-heavy block-state form at order 16
-    # read 405 GS/s after 5 launches and 444 after 200 (same box)
-    ap.add_argument("--settle", type=int, default=200)
+    # read 405 GS/s after 5 launches and 444 after 200 (same box); at 200 the
+    # block-state form at orders 8 / 16 still read 547 / 445 against 566 / 470
+    # after 400 (same box, same build)
+    ap.add_argument("--settle", type=int, default=400)
     # PHH: PH whose chunk carries come from the 256 input samples before each
     # chunk where the filter forgets its state within them (orders 2 and 4 at
     # Wn = 0.1), so no chunk waits on another and the launch assumes no
