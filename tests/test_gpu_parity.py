@@ -417,6 +417,49 @@ def test_plr_high_order_verdict(device, order):
         assert finite and rel > 1e-3, rel
 
 
+@pytest.mark.parametrize("order", [2, 4])
+def test_plr_unaligned_buffers_equal_aligned(device, order):
+    """Whole chunks move with 16-byte accesses staged through LDS only when
+    both buffers are 16-byte aligned; on buffers one float off that alignment
+    every access is one float, with the same arithmetic: the outputs agree bit
+    for bit, and both pass the gate (tail included)."""
+    import torch
+    n = (1 << 20) + 333
+    print(f"case PLR unaligned order={order}", flush=True)
+    sos, g, x = normal_workload(order, n, seed=12, wn=0.1)
+    xa = torch.from_numpy(x).to(device)
+    ya = torch.empty_like(xa)
+    big_in = torch.empty(n + 1, dtype=torch.float32, device=device)
+    big_out = torch.full((n + 1,), float("nan"), dtype=torch.float32, device=device)
+    big_in[1:] = xa
+    xu, yu = big_in[1:], big_out[1:]
+    assert xu.data_ptr() % 16 == 4 and yu.data_ptr() % 16 == 4
+    with iirx.Plan("PLR", sos, n, block_size=0, n_blocks=64) as p:
+        p.execute(xa, ya)
+        p.execute(xu, yu)
+        torch.cuda.synchronize()
+        p.check()
+    a, u = ya.cpu().numpy(), yu.cpu().numpy()
+    assert np.array_equal(a, u)
+    assert O.gate(a, O.golden(sos, x))[0]
+
+
+@pytest.mark.parametrize("n", [4, 1024 * 4, 1024 * 4 + 4, 3 * 1024 * 4 + 100, (1 << 22) + 12])
+def test_copy_ceiling_kernel_copies(device, n):
+    """The HBM-ceiling copy kernel (16 KB blocks per workgroup, the rest of a
+    partial last block element by element) copies every float and nothing
+    past the end."""
+    import torch
+    print(f"case copy ceiling n={n}", flush=True)
+    src = torch.randn(n, dtype=torch.float32, device=device)
+    for mode in (0, 1):
+        dst = torch.full((n + 64,), -1.0, dtype=torch.float32, device=device)
+        iirx.copy_ceiling(src, dst, n, mode)
+        torch.cuda.synchronize()
+        assert torch.equal(dst[:n], src), mode
+        assert bool((dst[n:] == -1.0).all()), mode
+
+
 def test_plr_default_taps_from_sos(device):
     """Without explicit taps the plan derives them from the sections in FP64."""
     n = 1 << 18
