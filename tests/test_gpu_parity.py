@@ -295,8 +295,9 @@ def test_joint_falls_back_on_large_tables(device, order, wn, N):
                                          ("STCR", 64, 64), ("DTCR", 64, 64)])
 @pytest.mark.parametrize("order", ORDERS)
 def test_full_size_gate(device, kernel, B, N, order):
-    """2^25 samples, the measured configuration (PH 64 x 32 is the bench's
-    headline kernel), against the FP64 oracle."""
+    """2^25 samples, the measured configurations (PHH 64 x 32 is the bench's
+    headline kernel; it runs PH where the halo does not apply), against the
+    FP64 oracle."""
     print(f"case full-size {kernel} {B}x{N} order={order}", flush=True)
     sos, g, x = normal_workload(order, 1 << 25, seed=1, wn=0.1)
     y, _ = run_gpu(device, kernel, sos, x, B, N)
