@@ -82,6 +82,32 @@ def test_noalloc_equals_plan_call(device, kernel, B, N, unrolled, schedule):
     assert O.gate(got1, r1)[0] and O.gate(got2, r2)[0]
 
 
+@pytest.mark.parametrize("schedule", ["persistent", "ticket"])
+def test_noalloc_halo_equals_plan_call(device, schedule):
+    """PHH where the halo applies (order 4, Wn 0.1: no records read, no
+    ticket): two calls in a row, the second smaller and not a multiple of the
+    chunk, equal to the plan's calls."""
+    import torch
+    order, B, N = 4, 64, 32
+    cfg = _cfg("PHH", order // 2, B, N, False, _lib.SCHEDULES[schedule])
+    n1, n2 = (1 << 20) + 3 * B * N, 5 * B * N + 77
+    sos, g, x1 = normal_workload(order, n1, seed=74, wn=0.1)
+    _, _, x2 = normal_workload(order, n2, seed=75, wn=0.1)
+    print(f"case noalloc PHH {B}x{N} {schedule} n={n1},{n2}", flush=True)
+    t1, t2 = torch.from_numpy(x1).to(device), torch.from_numpy(x2).to(device)
+    y1, y2 = torch.full_like(t1, np.nan), torch.full_like(t2, np.nan)
+    assert _noalloc(cfg, sos, t1, y1) == 0
+    assert _noalloc(cfg, sos, t2, y2) == 0
+    torch.cuda.synchronize()
+    got1, got2 = y1.cpu().numpy(), y2.cpu().numpy()
+    want1 = _with_plan("PHH", sos, t1, B, N, False, schedule)
+    want2 = _with_plan("PHH", sos, t2, B, N, False, schedule)
+    r1, r2 = O.golden(sos, x1), O.golden(sos, x2)
+    assert ulp_scaled(got1, want1, r1) <= ULPS
+    assert ulp_scaled(got2, want2, r2) <= ULPS
+    assert O.gate(got1, r1)[0] and O.gate(got2, r2)[0]
+
+
 def test_noalloc_streaming_state(device):
     """A state in and out: two calls, the first call's zf the second's zi,
     equal to the plan's calls within the same bound, the state too."""
