@@ -57,9 +57,10 @@ std::vector<iirx_config> enumerate(int32_t ns, uint32_t flags)
     for (int N = 8; N <= 128; N *= 2) add(IIRX_KERNEL_PH, 64, N, 0);
     if (ns >= 2)
         for (int N = 8; N <= 128; N *= 2) add(IIRX_KERNEL_PH_JOINT, 64, N, 0);
-    // halo carries read kPhHalo = 256 more samples per chunk: chunks of 1024 and up
-    if (ns >= 2)
-        for (int N = 16; N <= 128; N *= 2) add(IIRX_KERNEL_PH_HALO, 64, N, 0);
+    // halo carries read kPhHalo = 256 more samples per chunk: chunks of 1024 and
+    // up; at one section too (no lookback and no schedule: 2^25 samples, order
+    // 2, 737-743 GS/s against PH's 706-712)
+    for (int N = 16; N <= 128; N *= 2) add(IIRX_KERNEL_PH_HALO, 64, N, 0);
     if (!(flags & IIRX_TUNE_NO_CR))
         for (int k : {IIRX_KERNEL_STCR, IIRX_KERNEL_DTCR})
             for (int N = 8; N <= 128; N *= 2)
