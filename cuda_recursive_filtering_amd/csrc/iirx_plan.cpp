@@ -157,10 +157,12 @@ int validate(const iirx_config *cfg, int64_t n)
 // PLR geometry, the reference's rule (plr_kernels.cuh:49-112: a per-order
 // PLR_BLOCK_SIZE table, PLR_X = clamp(N / (nextpow2(#SM) * BLOCK), 1,
 // PLR_X_MAX)) with an MI355X table: one wave per chunk (no barriers; the
-// merge and the segment chain stay in registers) and X_MAX = 16 at order 2, 8
-// above, measured at 2^25 samples (64 x 16, 64 x 8, 64 x 8, 64 x 8 at orders
-// 2, 4, 8, 16: 245 / 176 / 94 / 22 GS/s, against 120 / 48 / 28 / 11 with
-// 256-thread blocks and X = 16); the spill calibration below may still halve X.
+// merge and the segment chain stay in registers) and X_MAX = 16 at orders 2
+// and 4, 8 above, measured at 2^25 samples (64 x 16, 64 x 16, 64 x 8, 64 x 8 at
+// orders 2, 4, 8, 16; order 4 at X = 16: 437 GS/s against 425 at X = 8, order
+// 8 at X = 16: 105 against 200; the reference-style 256-thread blocks with
+// X = 16 ran 120 / 48 / 28 / 11 GS/s in an earlier form of the kernel); the
+// spill calibration below may still halve X.
 void plr_geometry(const iirx_config *cfg, int64_t n, int *block, int *x)
 {
     const int k = 2 * cfg->n_sections;
@@ -168,7 +170,7 @@ void plr_geometry(const iirx_config *cfg, int64_t n, int *block, int *x)
     if (B == 0) B = 64;
     int X = cfg->plr_x;
     if (X == 0) {
-        const int cap = (k == 2) ? 16 : 8;
+        const int cap = (k <= 4) ? 16 : 8;
         const int xmax = (cfg->plr_x_max > 0) ? std::min(cfg->plr_x_max, 16) : cap;
         int64_t raw = n / (256LL * B);
         raw = std::max<int64_t>(1, std::min<int64_t>(raw, xmax));

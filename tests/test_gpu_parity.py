@@ -388,8 +388,8 @@ def test_plr_parity(device, order, wn, block, x):
     print(f"  block={B} X={X} rel={rel:.2e} vs-restatement={u:.1f} ulps scratch={info['scratch_bytes']}")
     assert ok, rel
     assert u <= 4096.0, u
-    if block == 0:  # automatic geometry: one wave per chunk, X_MAX = 16 at order 2, 8 above
-        assert B == 64 and X == {2: 16, 4: 8}[order], (B, X)
+    if block == 0:  # automatic geometry: one wave per chunk, X_MAX = 16 at orders 2 and 4, 8 above
+        assert B == 64 and X == {2: 16, 4: 16}[order], (B, X)
 
 
 @pytest.mark.parametrize("order", [8, 16])
