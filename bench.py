@@ -549,6 +549,15 @@ def main():
             entry["ticket"]["ms"] = round(ms_t, 5)
             if entry.get("halo") == 1:
                 entry["ticket"]["no_wait"] = True
+            # the block-state form under the ticket schedule (one ticket per
+            # workgroup of chunks), the safe schedule's best at orders 8 and 16
+            if order >= 8:
+                with iirx.Plan("BS", sos, N_SAMPLES, block_size=64, n_blocks=64, schedule="ticket") as pb:
+                    ms_b = time_plan(torch, pb, bufs, out, stream, args.geom_steps or args.steps,
+                                     warmup=max(args.settle, args.warmup))
+                entry["ticket_blockstate"] = gated(round(N_SAMPLES / (ms_b * 1e-3) / 1e9, 3), out, ref, args)
+                entry["ticket_blockstate"]["ms"] = round(ms_b, 5)
+                entry["ticket_blockstate"]["config"] = "BS 64x64"
         if not args.no_mfma and order in (8, 16):
             # MFMA block-state form beside the plain fp32 kernel, same input, error vs FP64
             refm = ref if ref is not None else fp64_reference(sos, host)

@@ -48,6 +48,7 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
     // per wave: one chunk of X (load i of lane l at [64 i + l]) and the lookback slots
     __shared__ f4 xbuf_w[W][NLD * 64];
     __shared__ float lp_w[W][kBsSlots * 16];
+    __shared__ unsigned long long s_ticket;  // ticket schedule: this workgroup's ticket
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
     const float *tab = a.coef;
@@ -227,25 +228,31 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
         }
         // the owner of the last chunk advances the epoch after publishing it
         if ((a.num_chunks - 1u) % GW == gw) epoch_advance(a.epoch, a.arrive, epoch, GW, a.spin_limit, a.err);
-    } else if (gw < a.num_chunks) {
-        // one ticket per wave, exactly num_chunks per launch (grid = ceil(chunks / W))
-        unsigned long long tk = 0;
-        if (lane == 0) tk = atomicAdd(a.ticket, 1ull);
-        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)tk);
-        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(tk >> 32));
-        tk = ((unsigned long long)hi << 32) | lo;
-        const unsigned long long ep = tk / a.num_chunks;
-        const uint32_t chunk = (uint32_t)(tk - ep * a.num_chunks);
+    } else {
+        // one ticket per workgroup (grid = ceil(chunks / W) tickets per
+        // launch): wave w takes chunk W t + w.  A chunk waits only on chunks
+        // of lower tickets (workgroups dispatched earlier: resident or done)
+        // or of lower waves of its own workgroup, which meets no barrier
+        // after the ticket, so no co-residency is assumed.  (One ticket per
+        // wave serialised 8192 device-scope atomics per launch at 2^25
+        // samples: 234 / 213 GS/s at orders 8 / 16.)
+        if (threadIdx.x == 0) s_ticket = atomicAdd(a.ticket, 1ull);
+        __syncthreads();
+        const unsigned long long tk = s_ticket;
+        const unsigned long long ep = tk / gridDim.x;
+        const uint32_t chunk = (uint32_t)(tk - ep * gridDim.x) * W + wave;
         const uint32_t pf = (uint32_t)(2ull * ep + 1ull);
-        Raw cur;
-        if constexpr (STAGED) {
-            fetch_staged(chunk);
-            read_staged(cur, true);
-            process_staged(chunk, pf, pf + 1u, cur, a.num_chunks);
-        } else {
-            fetch_chunk(chunk);
-            read_chunk(cur, true);
-            process(chunk, pf, pf + 1u, cur);
+        if (chunk < a.num_chunks) {
+            Raw cur;
+            if constexpr (STAGED) {
+                fetch_staged(chunk);
+                read_staged(cur, true);
+                process_staged(chunk, pf, pf + 1u, cur, a.num_chunks);
+            } else {
+                fetch_chunk(chunk);
+                read_chunk(cur, true);
+                process(chunk, pf, pf + 1u, cur);
+            }
         }
     }
 }
