@@ -228,6 +228,27 @@ __attribute__((amdgpu_waves_per_eu(bs_waves(L, G, SPLIT), bs_waves(L, G, SPLIT))
         }
         // the owner of the last chunk advances the epoch after publishing it
         if ((a.num_chunks - 1u) % GW == gw) epoch_advance(a.epoch, a.arrive, epoch, GW, a.spin_limit, a.err);
+    } else if (a.num_chunks % W == 0) {
+        // whole groups of W chunks: workgroups (one per resident slot) loop
+        // over the groups, one atomic ticket each, wave w taking chunk W t + w
+        // (for_each_chunk's grouped schedule: a group's chunks wait only on
+        // lower tickets or on lower waves of the group, which meet at no
+        // barrier before the group ends); the tables stay in LDS across groups
+        // (2^25 samples: 506 / 406 GS/s at orders 8 / 16 against 441 / 350
+        // with one group per workgroup)
+        for_each_chunk<W>(a.ticket, a.epoch, a.arrive, a.num_chunks, 0, a.spin_limit, a.err,
+                          [&](uint32_t chunk, uint32_t part_flag, uint32_t full_flag) {
+            Raw cur;
+            if constexpr (STAGED) {
+                fetch_staged(chunk);
+                read_staged(cur, true);
+                process_staged(chunk, part_flag, full_flag, cur, a.num_chunks);
+            } else {
+                fetch_chunk(chunk);
+                read_chunk(cur, true);
+                process(chunk, part_flag, full_flag, cur);
+            }
+        });
     } else {
         // one ticket per workgroup (grid = ceil(chunks / W) tickets per
         // launch): wave w takes chunk W t + w.  A chunk waits only on chunks

@@ -295,7 +295,11 @@ int plan_grid(const iirx_config *cfg, const Shape &sh, uint32_t main_chunks, con
     // slot under either schedule, a grid-stride loop with no ticket
     const bool grouped = (is_ph(cfg->kernel) || cfg->kernel == IIRX_KERNEL_STCR) && !sh.halo &&
                          cfg->schedule == IIRX_SCHEDULE_TICKET && main_chunks > 0 && main_chunks % (uint32_t)wv == 0;
-    if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && !grouped && !sh.halo) return IIRX_OK;
+    // the block-state form's ticket schedule loops over whole groups of its
+    // workgroup's chunks, one workgroup per resident slot, when they divide
+    const bool bs_grouped = is_bs(cfg->kernel) && cfg->schedule == IIRX_SCHEDULE_TICKET && main_chunks > 0 &&
+                            main_chunks % (uint32_t)sh.chunks_per_block == 0;
+    if (cfg->schedule != IIRX_SCHEDULE_PERSISTENT && !grouped && !sh.halo && !bs_grouped) return IIRX_OK;
     const void *occ_fn = fn;
     int occ_threads = threads;
     int64_t want = blocks_needed;

@@ -147,8 +147,8 @@ typedef enum {
  *               needs no co-residency: workgroups draw device-scope atomic
  *               tickets in order (about 12 ns each, serialised, on MI355X),
  *               one per group of consecutive chunks for PH and STCR (16 at
- *               N_BLOCKS <= 32), one per workgroup of chunks for the
- *               block-state form (one chunk per wave), one per chunk for
+ *               N_BLOCKS <= 32) and for the block-state form (a group is
+ *               one chunk per wave of its workgroup), one per chunk for
  *               DTCR and PLR.  PH_HALO with halo carries takes neither
  *               (no chunk waits on another).
  */
